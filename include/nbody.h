@@ -1,0 +1,212 @@
+/*
+ * nbody.h -- C ABI of the MI355X (gfx950) direct N-body solver.
+ *
+ * The drop-in boundary for the reference's force-plus-leapfrog path.  Every
+ * entry point takes plain pointers and sizes, returns an nb_status, never
+ * throws, and keeps its error text for nb_last_error().  Host arrays are
+ * owned by the caller; device memory is owned by the context.  Arrays use the
+ * reference's struct-of-arrays layout (CPU/nbody_system.h:41-74): a position,
+ * velocity or force argument is `double* const v[3]` (three length-N planes,
+ * like NBody_system::pos[DIM]), masses are one length-N array.  G = 1 as in
+ * the reference (CPU/nbody_system.h:12).
+ *
+ * Softening: every pair term uses (|d|^2 + eps^2)^(3/2) in place of |d|^3;
+ * eps = 0 is the reference's unsoftened force exactly.  Pairs at exactly zero
+ * separation contribute nothing (the reference's self-pair exclusion,
+ * CPU/force_calculation.cpp:530, and the FPGA eq_cond / w==0 mask,
+ * FPGA/device/ring_lrbd_lf_no_sync_local.cl:157).
+ *
+ * Threading: a context is used by one host thread at a time.  Work is ordered
+ * on the context's own HIP stream; entry points that return host data
+ * synchronise that stream before returning.
+ */
+#ifndef NBODY_H
+#define NBODY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define NB_API __attribute__((visibility("default")))
+#else
+#define NB_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nb_ctx nb_ctx;
+
+typedef enum nb_status {
+    NB_OK = 0,
+    NB_ERR_ARG = 1,       /* bad argument: null pointer, size, unknown enum */
+    NB_ERR_DEVICE = 2,    /* no usable gfx950 device / HIP runtime error */
+    NB_ERR_ALLOC = 3,     /* device or host allocation failed */
+    NB_ERR_STATE = 4,     /* call order: e.g. stepping before nb_set_state */
+    NB_ERR_LAUNCH = 5     /* a kernel launch failed */
+} nb_status;
+
+/* Force-evaluation form (the reference's -p dp|sp switch, CPU/main.cpp:247-261,
+ * becomes a precision form). */
+typedef enum nb_form {
+    NB_FORM_FP64 = 0,     /* FP64 differences, rsq + cubic correction, FP64 sums */
+    NB_FORM_MIXED = 1,    /* FP64 differences, FP32 rsqrt (+1 Newton step) and
+                             pair arithmetic, FP32 per-tile sums flushed to FP64 */
+    NB_FORM_LEARNED = 2,  /* nb_step only: exact mixed sum every `period`-th
+                             substep, learned extrapolation in between */
+    NB_FORM_MIXED_RAW = 3 /* as NB_FORM_MIXED without the Newton step (v_rsq_f32
+                             is already accurate to about 1 ulp) */
+} nb_form;
+
+/* Initial conditions (system set-up).  NB_IC_RING is the reference's ring of
+ * N-1 masses around a central mass (CPU/nbody_system.cpp:235-300). */
+typedef enum nb_ic {
+    NB_IC_RING = 0,       /* p0 = R, p1 = omega, p2 = ratio (centre mass / ring mass) */
+    NB_IC_SPHERE = 1,     /* uniform unit sphere, v = 0, m = 1/N (cold collapse) */
+    NB_IC_PLUMMER = 2     /* p0 = scale a, p1 = truncation radius; m = 1/N, G = M = 1 */
+} nb_ic;
+
+/* Text of the last error on this thread ("" if none). */
+NB_API const char* nb_last_error(void);
+
+/* Library version string. */
+NB_API const char* nb_version(void);
+
+/* Create a context on HIP device `device`.  Replaces the reference's system
+ * construction and per-thread scratch allocation (CPU/omp_solver.cpp:259-300). */
+NB_API nb_status nb_create(nb_ctx** out, int device);
+NB_API nb_status nb_destroy(nb_ctx* ctx);
+
+/* Initial conditions on the host, deterministic for (kind, N, seed, params).
+ * NB_IC_RING also writes the analytic force when `force` is non-null
+ * (NBody_system::init_stable_orbiting_particles, CPU/nbody_system.cpp:235). */
+NB_API nb_status nb_init_system(int kind, size_t n, uint64_t seed, const double* params, double* mass,
+                         double* const pos[3], double* const vel[3], double* const force[3]);
+
+/* Yoshida composition coefficients for DKD leapfrog of order 2/4/6/8
+ * (CPU/omp_solver.cpp:677-726).  c has lf_steps entries, d lf_steps-1; both
+ * buffers need room for 16.  Unknown orders fall back to 2, as the reference
+ * does (:703-707). */
+NB_API nb_status nb_yoshida(int order, double* c, double* d, size_t* lf_steps);
+
+/* Upload the system (NBody_system fields mass/pos/vel, CPU/nbody_system.h:41-74).
+ * vel may be null (zero velocities). */
+NB_API nb_status nb_set_state(nb_ctx* ctx, size_t n, const double* mass, const double* const pos[3],
+                       const double* const vel[3]);
+
+/* Download positions and/or velocities (either may be null). */
+NB_API nb_status nb_get_state(nb_ctx* ctx, double* const pos[3], double* const vel[3]);
+
+/* Accelerations a_i = G sum_j m_j d_ij / (|d_ij|^2 + eps^2)^(3/2) of the
+ * current state, written to acc[3][N].  Replaces one evaluation of
+ * compute_forces_avx2_blocked / compute_forces_avx512_rsqrt_4i_blocked
+ * (CPU/force_calculation.cpp:477, :1137) with all ordered pairs evaluated. */
+NB_API nb_status nb_accel(nb_ctx* ctx, int form, double eps, double* const acc[3]);
+
+/* Forces F_i = m_i a_i, the quantity the reference's kernels produce
+ * (CPU/force_calculation.cpp:78-82 sums -G m_k m_q d/|d|^3).  Overwrites
+ * force[3][N]; the reference accumulates (+=) into zeroed arrays. */
+NB_API nb_status nb_compute_forces(nb_ctx* ctx, int form, double eps, double* const force[3]);
+
+/* As nb_accel, but leaves the result in the context's device acceleration
+ * buffer (no host copy): for callers that keep the state resident in HBM. */
+NB_API nb_status nb_accel_device(nb_ctx* ctx, int form, double eps);
+
+/* One ring round on one GPU: accelerations of ni i-particles from nj
+ * j-particles, both interleaved (x, y, z, m) as the FPGA particle_t
+ * (FPGA/host/solver.cpp:721).  Replaces the local-block x received-block
+ * evaluation of hybrid_ring_solver (CPU/hybrid_solver.cpp:84-95), in full
+ * rather than reduced form.  acc is ni x 3, row-major.  i may be a subset of
+ * j: zero-separation pairs contribute nothing.  Host buffers; the j set is
+ * streamed to the device in blocks of at most j_block particles (0 = all). */
+NB_API nb_status nb_accel_block(nb_ctx* ctx, int form, double eps, size_t ni, const double* pi,
+                         size_t nj, const double* pj, size_t j_block, double* acc);
+
+/* As nb_accel_block with DEVICE pointers: d_pi (ni x 4), d_pj (nj x 4) and
+ * d_acc4 (ni x 4 doubles: ax, ay, az, 0) in HBM, work queued on the context's
+ * stream without synchronising.  For inputs already resident on the GPU. */
+NB_API nb_status nb_accel_block_device(nb_ctx* ctx, int form, double eps, size_t ni, const double* d_pi,
+                                size_t nj, const double* d_pj, size_t j_block, double* d_acc4);
+
+/* Advance the resident state by time_steps DKD leapfrog steps of size dt with
+ * coefficients c[lf_steps], d[lf_steps-1] (reduced_solver_omp_lf,
+ * CPU/omp_solver.cpp:257-421): per step, for each substep s < lf_steps-1,
+ * drift c[s], force, kick d[s]; then drift c[lf_steps-1].  Consecutive drifts
+ * are fused into the kick kernel (as FPGA/device/ring_lrbd_lf_no_sync_local.cl:853).
+ * form NB_FORM_LEARNED requires a trained extrapolator (nb_learned_*). */
+NB_API nb_status nb_step(nb_ctx* ctx, size_t time_steps, double dt, const double* c, const double* d,
+                  size_t lf_steps, int form, double eps);
+
+/* Kinetic and potential energy of the resident state:
+ * T = 1/2 sum m v^2 (CPU/force_calculation.cpp:1740), U = -G sum_{i<j} m_i m_j /
+ * sqrt(r^2+eps^2) (:1880, softened as FPGA/host/accuracy_evaluation.cpp:35).
+ * FP64 with compensated, fixed-order sums (bitwise reproducible). */
+NB_API nb_status nb_energy(nb_ctx* ctx, double eps, double* T, double* U);
+
+/* Timing of force kernels on the context's stream, measured with HIP events
+ * around every force launch since the last reset: total milliseconds and the
+ * number of force evaluations.  Enabling adds two event records per launch. */
+NB_API nb_status nb_timing_enable(nb_ctx* ctx, int enable);
+NB_API nb_status nb_timing_read(nb_ctx* ctx, double* force_ms, uint64_t* force_evals, double* step_ms);
+NB_API nb_status nb_timing_reset(nb_ctx* ctx);
+
+/* Wait for all work queued on the context's stream. */
+NB_API nb_status nb_synchronize(nb_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Learned acceleration extrapolator (NB_FORM_LEARNED).  No counterpart in the
+ * reference.  A 15 -> 128 -> 128 -> 128 -> 3 MLP (BF16 weights on MFMA, FP32
+ * accumulate) corrects the cubic extrapolation a_{n+1} ~ 4a_n - 6a_{n-1} +
+ * 4a_{n-2} - a_{n-3} from each particle's last four accelerations and its
+ * velocity.  In nb_step with NB_FORM_LEARNED every `period`-th substep runs
+ * the exact direct sum, compares it with the extrapolation, and while the
+ * max relative acceleration error exceeds `threshold` all substeps are exact.
+ * ------------------------------------------------------------------------- */
+typedef struct nb_learned_stats {
+    uint64_t exact_evals;      /* direct sums run by learned-form steps */
+    uint64_t predicted_evals;  /* substeps served by the extrapolator */
+    uint64_t fallbacks;        /* exact checks that exceeded the threshold */
+    double last_check_err;     /* max relative acceleration error, last check */
+    double max_check_err;      /* largest such error since configure */
+} nb_learned_stats;
+
+/* period >= 2 (exact every period-th substep), threshold > 0 (0 = always
+ * exact), exact_form NB_FORM_FP64 / NB_FORM_MIXED / NB_FORM_MIXED_RAW. */
+NB_API nb_status nb_learned_configure(nb_ctx* ctx, int period, double threshold, int exact_form);
+
+/* 0: cubic extrapolation corrected by the network (default once trained);
+ * 1: cubic extrapolation alone (the method without the network). */
+NB_API nb_status nb_learned_set_mode(nb_ctx* ctx, int mode);
+
+/* Run `time_steps` exact steps from the resident state (advancing it) and
+ * record training pairs (features, target correction) in device memory. */
+NB_API nb_status nb_learned_collect(nb_ctx* ctx, size_t time_steps, double dt, const double* c,
+                             const double* d, size_t lf_steps, int form, double eps);
+
+/* Train the network on the collected pairs with the library's own MFMA
+ * forward/backward kernels and Adam: `iters` minibatches of `batch`. */
+NB_API nb_status nb_learned_train(nb_ctx* ctx, int iters, int batch, double lr, uint64_t seed,
+                           double* final_loss);
+
+NB_API nb_status nb_learned_stats_read(nb_ctx* ctx, nb_learned_stats* out);
+
+/* Network parameters as FP32 master copies, packed W1 b1 W2 b2 W3 b3 W4 b4
+ * (Wk row-major [out][in]).  count receives the number of floats. */
+NB_API nb_status nb_mlp_param_count(size_t* count);
+NB_API nb_status nb_mlp_get_params(nb_ctx* ctx, float* params);
+NB_API nb_status nb_mlp_set_params(nb_ctx* ctx, const float* params);
+
+/* Test hooks for the network kernels on host buffers: forward y = f(x) for
+ * `batch` rows of 15 inputs, and the gradient of the loss
+ * L = sum_rows sum_k (y_k - target_k)^2 / (2 batch) with respect to every
+ * parameter (packed as above). */
+NB_API nb_status nb_mlp_forward(nb_ctx* ctx, size_t batch, const float* x, float* y);
+NB_API nb_status nb_mlp_grad(nb_ctx* ctx, size_t batch, const float* x, const float* target,
+                      float* grads, double* loss);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_H */

@@ -1,0 +1,617 @@
+// nbody_capi.hip -- the C ABI (include/nbody.h) over the gfx950 kernels.
+//
+// One context = one device + one HIP stream (default priority) + the
+// device-resident system: positions+masses, velocities and accelerations as
+// 32-byte records, plus scratch for split partial sums and the energy rows.
+// All work is stream-ordered; calls returning host data synchronise.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nbody.h"
+#include "nbody_host.h"
+#include "nbody_kernels.h"
+#include "nbody_learned.h"
+
+using nbk::P4;
+
+namespace {
+
+thread_local std::string g_err;
+
+nb_status fail(nb_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+
+nb_status hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? NB_ERR_ALLOC : NB_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define NB_HIP(call, what)                          \
+    do {                                            \
+        hipError_t e_ = (call);                     \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+constexpr size_t kMaxN = 100000000;   // the reference's bound (CPU/main.cpp:271)
+
+}  // namespace
+
+namespace nbh {
+
+hipError_t DevBuf::ensure(size_t b) {
+    if (b <= bytes) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    else p = nullptr;
+    return e;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+}  // namespace nbh
+
+using nbh::DevBuf;
+
+struct nb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    size_t n = 0;
+    std::vector<double> mass_h;
+    DevBuf pm, vel, acc, part, rows, eblk;
+    DevBuf bi, bj, bout;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> events;   // pairs (start, stop) per force launch
+    size_t ev_used = 0;
+    double force_ms = 0.0;
+    uint64_t force_evals = 0;
+    double step_ms = 0.0;
+    hipEvent_t step_ev[2] = {nullptr, nullptr};
+    nbl::Learned* learned = nullptr;
+};
+
+namespace nbh {
+
+// i-particles per lane: two when the grid is large enough to fill the chip.
+int choose_ipt(const nb_ctx* c, long ni) { return ni >= long(c->num_cus) * 512 ? 2 : 1; }
+
+// Number of j-chunks so that the grid holds about 8 waves per CU.
+int choose_splits(const nb_ctx* c, long ni, long nj, int ipt) {
+    const long waves = (ni + 64 * ipt - 1) / (64 * ipt);
+    const long target = long(c->num_cus) * 8;
+    long S = 1;
+    if (waves > 0 && waves < target) S = (target + waves - 1) / waves;
+    S = std::min(S, std::max(1L, nj / 64));
+    S = std::min(S, 1024L);
+    return int(S);
+}
+
+// Timing: record an event before a force launch; returns the index or -1.
+static long timing_begin(nb_ctx* c) {
+    if (!c->timing) return -1;
+    if (c->ev_used + 2 > c->events.size()) {
+        if (c->events.size() >= 16384) {   // fold what has completed so far
+            (void)hipStreamSynchronize(c->stream);
+            for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]) == hipSuccess)
+                    c->force_ms += ms;
+            }
+            c->ev_used = 0;
+        } else {
+            for (int k = 0; k < 2; k++) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return -1;
+                c->events.push_back(e);
+            }
+        }
+    }
+    const long idx = long(c->ev_used);
+    (void)hipEventRecord(c->events[idx], c->stream);
+    c->ev_used += 2;
+    return idx;
+}
+
+static void timing_end(nb_ctx* c, long idx) {
+    if (idx < 0) return;
+    (void)hipEventRecord(c->events[idx + 1], c->stream);
+    c->force_evals++;
+}
+
+// One force evaluation of the resident state; partial sums land in
+// part[0..S) (or directly in acc when S == 1).
+hipError_t force_eval(nb_ctx* c, int kform, double eps, P4*& part, int& S) {
+    const long n = long(c->n);
+    nbk::AccelLaunch L;
+    L.form = kform;
+    L.ipt = choose_ipt(c, n);
+    S = choose_splits(c, n, n, L.ipt);
+    long jchunk = (n + S - 1) / S;
+    jchunk = (jchunk + 63) & ~63L;
+    S = int((n + jchunk - 1) / jchunk);
+    if (S <= 1) {
+        S = 1;
+        jchunk = n;
+        part = (P4*)c->acc.p;
+    } else {
+        hipError_t e = c->part.ensure(size_t(S) * n * sizeof(P4));
+        if (e != hipSuccess) return e;
+        part = (P4*)c->part.p;
+    }
+    L.pi = (const P4*)c->pm.p;
+    L.ni = int(n);
+    L.pj = (const P4*)c->pm.p;
+    L.jbeg = 0;
+    L.jend = int(n);
+    L.jchunk = int(jchunk);
+    L.splits = S;
+    L.eps2 = eps * eps;
+    L.G = 1.0;
+    L.out = part;
+    L.out_stride = n;
+    const long ev = timing_begin(c);
+    hipError_t e = nbk::launch_accel(L, c->stream);
+    timing_end(c, ev);
+    return e;
+}
+
+int kernel_form(int form) {
+    switch (form) {
+    case NB_FORM_FP64: return nbk::FORM_FP64;
+    case NB_FORM_MIXED: return nbk::FORM_MIXED_NR;
+    case NB_FORM_MIXED_RAW: return nbk::FORM_MIXED;
+    default: return -1;
+    }
+}
+
+hipError_t accel_to_acc(nb_ctx* c, int kform, double eps) {
+    P4* part;
+    int S;
+    hipError_t e = force_eval(c, kform, eps, part, S);
+    if (e != hipSuccess || S == 1) return e;
+    return nbk::launch_reduce(part, S, long(c->n), int(c->n), (P4*)c->acc.p, c->stream);
+}
+
+hipStream_t ctx_stream(nb_ctx* c) { return c->stream; }
+size_t ctx_n(nb_ctx* c) { return c->n; }
+P4* ctx_pm(nb_ctx* c) { return (P4*)c->pm.p; }
+P4* ctx_vel(nb_ctx* c) { return (P4*)c->vel.p; }
+P4* ctx_acc(nb_ctx* c) { return (P4*)c->acc.p; }
+int ctx_num_cus(nb_ctx* c) { return c->num_cus; }
+nbl::Learned*& ctx_learned(nb_ctx* c) { return c->learned; }
+
+}  // namespace nbh
+
+using namespace nbh;
+
+extern "C" {
+
+const char* nb_last_error(void) { return g_err.c_str(); }
+
+const char* nb_version(void) { return "nbody-mi355x 0.1 (gfx950)"; }
+
+nb_status nb_create(nb_ctx** out, int device) {
+    if (!out) return fail(NB_ERR_ARG, "nb_create: out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(NB_ERR_DEVICE, "nb_create: no HIP device available");
+    if (device < 0 || device >= count) return fail(NB_ERR_ARG, "nb_create: device index out of range");
+    NB_HIP(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    NB_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(NB_ERR_DEVICE, std::string("nb_create: device is ") + prop.gcnArchName +
+                                       ", this library is built for gfx950 only");
+    nb_ctx* c = new (std::nothrow) nb_ctx;
+    if (!c) return fail(NB_ERR_ALLOC, "nb_create: out of host memory");
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    for (auto& ev : c->step_ev) (void)hipEventCreate(&ev);
+    *out = c;
+    g_err.clear();
+    return NB_OK;
+}
+
+nb_status nb_destroy(nb_ctx* c) {
+    if (!c) return NB_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    nbl::destroy(c->learned);
+    c->learned = nullptr;
+    for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout})
+        b->release();
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->step_ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NB_OK;
+}
+
+nb_status nb_init_system(int kind, size_t n, uint64_t seed, const double* params, double* mass,
+                         double* const pos[3], double* const vel[3], double* const force[3]) {
+    if (!mass || !pos || !vel || !pos[0] || !pos[1] || !pos[2] || !vel[0] || !vel[1] || !vel[2])
+        return fail(NB_ERR_ARG, "nb_init_system: null output array");
+    if (n > kMaxN) return fail(NB_ERR_ARG, "nb_init_system: N must stay below 1e8");
+    switch (kind) {
+    case NB_IC_RING: {
+        if (n < 2) return fail(NB_ERR_ARG, "nb_init_system: the ring needs N >= 2");
+        const double R = params ? params[0] : 1.0;
+        const double omega = params ? params[1] : 2.0 * M_PI;
+        const double ratio = params ? params[2] : 1e5;
+        if (force && (!force[0] || !force[1] || !force[2]))
+            return fail(NB_ERR_ARG, "nb_init_system: null force plane");
+        ic_ring(n, R, omega, ratio, mass, pos, vel, force);
+        break;
+    }
+    case NB_IC_SPHERE:
+        ic_sphere(n, seed, mass, pos, vel);
+        break;
+    case NB_IC_PLUMMER: {
+        const double a = params ? params[0] : 1.0;
+        const double rmax = params ? params[1] : 20.0;
+        if (!(a > 0.0) || !(rmax > 0.0)) return fail(NB_ERR_ARG, "nb_init_system: Plummer a, rmax must be > 0");
+        ic_plummer(n, seed, a, rmax, mass, pos, vel);
+        break;
+    }
+    default:
+        return fail(NB_ERR_ARG, "nb_init_system: unknown initial-condition kind");
+    }
+    return NB_OK;
+}
+
+nb_status nb_yoshida(int order, double* c, double* d, size_t* lf_steps) {
+    if (!c || !d || !lf_steps) return fail(NB_ERR_ARG, "nb_yoshida: null argument");
+    *lf_steps = yoshida(order, c, d);
+    return NB_OK;
+}
+
+nb_status nb_set_state(nb_ctx* c, size_t n, const double* mass, const double* const pos[3],
+                       const double* const vel[3]) {
+    if (!c) return fail(NB_ERR_ARG, "nb_set_state: null context");
+    if (n > kMaxN) return fail(NB_ERR_ARG, "nb_set_state: N must stay below 1e8");
+    if (n > 0 && (!mass || !pos || !pos[0] || !pos[1] || !pos[2]))
+        return fail(NB_ERR_ARG, "nb_set_state: null mass or position array");
+    if (n > 0 && vel && (!vel[0] || !vel[1] || !vel[2]))
+        return fail(NB_ERR_ARG, "nb_set_state: null velocity plane");
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const size_t cap = std::max<size_t>(n, 1);
+    NB_HIP(c->pm.ensure(cap * sizeof(P4)), "allocate positions");
+    NB_HIP(c->vel.ensure(cap * sizeof(P4)), "allocate velocities");
+    NB_HIP(c->acc.ensure(cap * sizeof(P4)), "allocate accelerations");
+    std::vector<P4> hp(n), hv(n);
+    for (size_t i = 0; i < n; i++) {
+        hp[i] = P4{pos[0][i], pos[1][i], pos[2][i], mass[i]};
+        hv[i] = vel ? P4{vel[0][i], vel[1][i], vel[2][i], 0.0} : P4{0.0, 0.0, 0.0, 0.0};
+    }
+    if (n) {
+        NB_HIP(hipMemcpyAsync(c->pm.p, hp.data(), n * sizeof(P4), hipMemcpyHostToDevice, c->stream), "upload positions");
+        NB_HIP(hipMemcpyAsync(c->vel.p, hv.data(), n * sizeof(P4), hipMemcpyHostToDevice, c->stream), "upload velocities");
+        NB_HIP(hipMemsetAsync(c->acc.p, 0, n * sizeof(P4), c->stream), "clear accelerations");
+        NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    }
+    c->mass_h.assign(mass, mass + n);
+    c->n = n;
+    nbl::on_new_state(c->learned);
+    return NB_OK;
+}
+
+nb_status nb_get_state(nb_ctx* c, double* const pos[3], double* const vel[3]) {
+    if (!c) return fail(NB_ERR_ARG, "nb_get_state: null context");
+    if ((pos && (!pos[0] || !pos[1] || !pos[2])) || (vel && (!vel[0] || !vel[1] || !vel[2])))
+        return fail(NB_ERR_ARG, "nb_get_state: null plane");
+    const size_t n = c->n;
+    if (n == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    std::vector<P4> h(n);
+    if (pos) {
+        NB_HIP(hipMemcpyAsync(h.data(), c->pm.p, n * sizeof(P4), hipMemcpyDeviceToHost, c->stream), "download positions");
+        NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        for (size_t i = 0; i < n; i++) {
+            pos[0][i] = h[i].x; pos[1][i] = h[i].y; pos[2][i] = h[i].z;
+        }
+    }
+    if (vel) {
+        NB_HIP(hipMemcpyAsync(h.data(), c->vel.p, n * sizeof(P4), hipMemcpyDeviceToHost, c->stream), "download velocities");
+        NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        for (size_t i = 0; i < n; i++) {
+            vel[0][i] = h[i].x; vel[1][i] = h[i].y; vel[2][i] = h[i].z;
+        }
+    }
+    return NB_OK;
+}
+
+static nb_status check_eps(double eps, const char* fn) {
+    if (!(eps >= 0.0) || !std::isfinite(eps)) return fail(NB_ERR_ARG, std::string(fn) + ": eps must be finite and >= 0");
+    return NB_OK;
+}
+
+static nb_status download_acc(nb_ctx* c, double* const out[3], bool times_mass) {
+    const size_t n = c->n;
+    std::vector<P4> h(n);
+    NB_HIP(hipMemcpyAsync(h.data(), c->acc.p, n * sizeof(P4), hipMemcpyDeviceToHost, c->stream), "download accelerations");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < n; i++) {
+        const double s = times_mass ? c->mass_h[i] : 1.0;
+        out[0][i] = s * h[i].x;
+        out[1][i] = s * h[i].y;
+        out[2][i] = s * h[i].z;
+    }
+    return NB_OK;
+}
+
+static nb_status accel_common(nb_ctx* c, int form, double eps, double* const out[3], bool times_mass,
+                              const char* fn) {
+    if (!c) return fail(NB_ERR_ARG, std::string(fn) + ": null context");
+    if (!out || !out[0] || !out[1] || !out[2]) return fail(NB_ERR_ARG, std::string(fn) + ": null output plane");
+    if (nb_status s = check_eps(eps, fn)) return s;
+    const int kf = kernel_form(form);
+    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
+    if (c->n == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    NB_HIP(accel_to_acc(c, kf, eps), "force kernel");
+    return download_acc(c, out, times_mass);
+}
+
+nb_status nb_accel(nb_ctx* c, int form, double eps, double* const acc[3]) {
+    return accel_common(c, form, eps, acc, false, "nb_accel");
+}
+
+nb_status nb_compute_forces(nb_ctx* c, int form, double eps, double* const force[3]) {
+    return accel_common(c, form, eps, force, true, "nb_compute_forces");
+}
+
+nb_status nb_accel_device(nb_ctx* c, int form, double eps) {
+    if (!c) return fail(NB_ERR_ARG, "nb_accel_device: null context");
+    if (nb_status s = check_eps(eps, "nb_accel_device")) return s;
+    const int kf = kernel_form(form);
+    if (kf < 0) return fail(NB_ERR_ARG, "nb_accel_device: unknown form");
+    if (c->n == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    NB_HIP(accel_to_acc(c, kf, eps), "force kernel");
+    return NB_OK;
+}
+
+// Shared body of the ring-share entry points: d_pi (ni records) against d_pj
+// (nj records) in j-blocks of jb, partials then one ordered reduction.
+static nb_status block_eval(nb_ctx* c, int kf, double eps, long ni, const P4* d_pi, long nj,
+                            const P4* d_pj, long jb, P4* d_out) {
+    const int ipt = choose_ipt(c, ni);
+    const long nb = (nj + jb - 1) / jb;
+    int S = choose_splits(c, ni, jb, ipt);
+    long jchunk = (jb + S - 1) / S;
+    jchunk = (jchunk + 63) & ~63L;
+    S = int((jb + jchunk - 1) / jchunk);
+    const long parts = nb * S;
+    P4* part = d_out;
+    if (parts > 1) {
+        NB_HIP(c->part.ensure(size_t(parts) * ni * sizeof(P4)), "allocate partial sums");
+        part = (P4*)c->part.p;
+    }
+    for (long b = 0; b < nb; b++) {
+        nbk::AccelLaunch L;
+        L.form = kf;
+        L.ipt = ipt;
+        L.pi = d_pi;
+        L.ni = int(ni);
+        L.pj = d_pj;
+        L.jbeg = int(b * jb);
+        L.jend = int(std::min(nj, (b + 1) * jb));
+        L.jchunk = int(jchunk);
+        L.splits = S;
+        L.eps2 = eps * eps;
+        L.G = 1.0;
+        L.out = part;
+        L.out_stride = ni;
+        L.split_base = int(b * S);
+        const long ev = timing_begin(c);
+        hipError_t e = nbk::launch_accel(L, c->stream);
+        timing_end(c, ev);
+        NB_HIP(e, "force kernel");
+    }
+    if (parts > 1) NB_HIP(nbk::launch_reduce(part, int(parts), ni, int(ni), d_out, c->stream), "reduce kernel");
+    return NB_OK;
+}
+
+static nb_status block_args(nb_ctx* c, int form, double eps, size_t ni, const double* pi, size_t nj,
+                            const double* pj, const void* out, const char* fn, int& kf) {
+    if (!c) return fail(NB_ERR_ARG, std::string(fn) + ": null context");
+    if ((ni && (!pi || !out)) || (nj && !pj)) return fail(NB_ERR_ARG, std::string(fn) + ": null buffer");
+    if (ni > kMaxN || nj > kMaxN) return fail(NB_ERR_ARG, std::string(fn) + ": sizes must stay below 1e8");
+    if (nb_status s = check_eps(eps, fn)) return s;
+    kf = kernel_form(form);
+    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": unknown form");
+    return NB_OK;
+}
+
+nb_status nb_accel_block(nb_ctx* c, int form, double eps, size_t ni, const double* pi, size_t nj,
+                         const double* pj, size_t j_block, double* acc) {
+    int kf;
+    if (nb_status s = block_args(c, form, eps, ni, pi, nj, pj, acc, "nb_accel_block", kf)) return s;
+    if (ni == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    const long jb = (j_block == 0 || j_block > nj) ? long(std::max<size_t>(nj, 1)) : long(j_block);
+    NB_HIP(c->bi.ensure(ni * sizeof(P4)), "allocate i block");
+    NB_HIP(c->bj.ensure(std::max<size_t>(nj, 1) * sizeof(P4)), "allocate j set");
+    NB_HIP(c->bout.ensure(ni * sizeof(P4)), "allocate output");
+    NB_HIP(hipMemcpyAsync(c->bi.p, pi, ni * sizeof(P4), hipMemcpyHostToDevice, c->stream), "upload i block");
+    if (nj) NB_HIP(hipMemcpyAsync(c->bj.p, pj, nj * sizeof(P4), hipMemcpyHostToDevice, c->stream), "upload j set");
+    if (nj == 0) {
+        NB_HIP(hipMemsetAsync(c->bout.p, 0, ni * sizeof(P4), c->stream), "clear output");
+    } else if (nb_status s = block_eval(c, kf, eps, long(ni), (const P4*)c->bi.p, long(nj), (const P4*)c->bj.p, jb,
+                                        (P4*)c->bout.p)) {
+        return s;
+    }
+    std::vector<P4> h(ni);
+    NB_HIP(hipMemcpyAsync(h.data(), c->bout.p, ni * sizeof(P4), hipMemcpyDeviceToHost, c->stream), "download output");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < ni; i++) {
+        acc[3 * i] = h[i].x;
+        acc[3 * i + 1] = h[i].y;
+        acc[3 * i + 2] = h[i].z;
+    }
+    return NB_OK;
+}
+
+nb_status nb_accel_block_device(nb_ctx* c, int form, double eps, size_t ni, const double* d_pi, size_t nj,
+                                const double* d_pj, size_t j_block, double* d_acc4) {
+    int kf;
+    if (nb_status s = block_args(c, form, eps, ni, d_pi, nj, d_pj, d_acc4, "nb_accel_block_device", kf)) return s;
+    if (ni == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    if (nj == 0) {
+        NB_HIP(hipMemsetAsync(d_acc4, 0, ni * sizeof(P4), c->stream), "clear output");
+        return NB_OK;
+    }
+    const long jb = (j_block == 0 || j_block > nj) ? long(nj) : long(j_block);
+    return block_eval(c, kf, eps, long(ni), (const P4*)d_pi, long(nj), (const P4*)d_pj, jb, (P4*)d_acc4);
+}
+
+nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, const double* dd, size_t lf_steps,
+                  int form, double eps) {
+    if (!c) return fail(NB_ERR_ARG, "nb_step: null context");
+    if (!cc || !dd) return fail(NB_ERR_ARG, "nb_step: null coefficient array");
+    if (lf_steps < 2 || lf_steps > 16) return fail(NB_ERR_ARG, "nb_step: lf_steps must be in [2, 16]");
+    if (!std::isfinite(dt)) return fail(NB_ERR_ARG, "nb_step: dt must be finite");
+    if (nb_status s = check_eps(eps, "nb_step")) return s;
+    if (form == NB_FORM_LEARNED) return nbl::step_learned(c, time_steps, dt, cc, dd, lf_steps, eps);
+    const int kf = kernel_form(form);
+    if (kf < 0) return fail(NB_ERR_ARG, "nb_step: unknown form");
+    if (c->n == 0 || time_steps == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    const int n = int(c->n);
+    P4* pm = (P4*)c->pm.p;
+    P4* vel = (P4*)c->vel.p;
+    P4* acc = (P4*)c->acc.p;
+    if (c->timing) (void)hipEventRecord(c->step_ev[0], c->stream);
+    NB_HIP(nbk::launch_drift(n, pm, vel, cc[0] * dt, c->stream), "drift kernel");
+    for (size_t t = 0; t < time_steps; t++) {
+        for (size_t s = 0; s + 1 < lf_steps; s++) {
+            P4* part;
+            int S;
+            NB_HIP(force_eval(c, kf, eps, part, S), "force kernel");
+            double c1, c2 = 0.0;
+            if (s + 2 < lf_steps) {
+                c1 = cc[s + 1] * dt;
+            } else {
+                c1 = cc[lf_steps - 1] * dt;
+                if (t + 1 < time_steps) c2 = cc[0] * dt;
+            }
+            NB_HIP(nbk::launch_kick_drift(part, S, long(n), n, pm, vel, acc, dd[s] * dt, c1, c2, c->stream),
+                   "kick/drift kernel");
+        }
+    }
+    if (c->timing) {
+        (void)hipEventRecord(c->step_ev[1], c->stream);
+        NB_HIP(hipEventSynchronize(c->step_ev[1]), "hipEventSynchronize");
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->step_ev[0], c->step_ev[1]) == hipSuccess) c->step_ms += ms;
+    }
+    return NB_OK;
+}
+
+nb_status nb_energy(nb_ctx* c, double eps, double* T, double* U) {
+    if (!c) return fail(NB_ERR_ARG, "nb_energy: null context");
+    if (!T || !U) return fail(NB_ERR_ARG, "nb_energy: null output");
+    if (nb_status s = check_eps(eps, "nb_energy")) return s;
+    *T = 0.0;
+    *U = 0.0;
+    const long n = long(c->n);
+    if (n == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    int S = choose_splits(c, n, n, 1);
+    long jchunk = (n + S - 1) / S;
+    jchunk = (jchunk + 63) & ~63L;
+    S = int((n + jchunk - 1) / jchunk);
+    const long nblk = (n + nbk::BLOCK - 1) / nbk::BLOCK;
+    NB_HIP(c->rows.ensure(size_t(S) * n * 16), "allocate energy rows");
+    NB_HIP(c->eblk.ensure(size_t(nblk) * 4 * sizeof(double)), "allocate energy blocks");
+    NB_HIP(nbk::launch_energy((const P4*)c->pm.p, (const P4*)c->vel.p, int(n), S, int(jchunk), eps * eps, c->rows.p,
+                              (double*)c->eblk.p, c->stream),
+           "energy kernels");
+    std::vector<double> h(size_t(nblk) * 4);
+    NB_HIP(hipMemcpyAsync(h.data(), c->eblk.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+           "download energy blocks");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    // Neumaier sum of the block partials in block order.
+    auto nsum = [&](int off) {
+        double s = 0.0, comp = 0.0;
+        for (long b = 0; b < nblk; b++)
+            for (int k = 0; k < 2; k++) {
+                const double v = h[4 * b + off + k];
+                const double t = s + v;
+                comp += (std::fabs(s) >= std::fabs(v)) ? (s - t) + v : (v - t) + s;
+                s = t;
+            }
+        return s + comp;
+    };
+    *U = -0.5 * nsum(0);   // sum_i m_i row_i counts every pair twice; G = 1
+    *T = 0.5 * nsum(2);
+    return NB_OK;
+}
+
+nb_status nb_timing_enable(nb_ctx* c, int enable) {
+    if (!c) return fail(NB_ERR_ARG, "nb_timing_enable: null context");
+    c->timing = enable != 0;
+    return NB_OK;
+}
+
+nb_status nb_timing_read(nb_ctx* c, double* force_ms, uint64_t* evals, double* step_ms) {
+    if (!c) return fail(NB_ERR_ARG, "nb_timing_read: null context");
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]) == hipSuccess) c->force_ms += ms;
+    }
+    c->ev_used = 0;
+    if (force_ms) *force_ms = c->force_ms;
+    if (evals) *evals = c->force_evals;
+    if (step_ms) *step_ms = c->step_ms;
+    return NB_OK;
+}
+
+nb_status nb_timing_reset(nb_ctx* c) {
+    if (!c) return fail(NB_ERR_ARG, "nb_timing_reset: null context");
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    c->ev_used = 0;
+    c->force_ms = 0.0;
+    c->force_evals = 0;
+    c->step_ms = 0.0;
+    return NB_OK;
+}
+
+nb_status nb_synchronize(nb_ctx* c) {
+    if (!c) return fail(NB_ERR_ARG, "nb_synchronize: null context");
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return NB_OK;
+}
+
+}  // extern "C"
+
+namespace nbh {
+nb_status set_error(nb_status s, const std::string& msg) { return fail(s, msg); }
+}  // namespace nbh

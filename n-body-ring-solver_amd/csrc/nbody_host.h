@@ -1,0 +1,47 @@
+// nbody_host.h -- host-side helpers shared inside the library (not public).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "nbody.h"
+#include "nbody_kernels.h"
+
+namespace nbl {
+struct Learned;
+}
+
+namespace nbh {
+
+// Initial conditions and integrator coefficients (nbody_ic.cpp).
+void ic_sphere(size_t n, uint64_t seed, double* m, double* const pos[3], double* const vel[3]);
+void ic_plummer(size_t n, uint64_t seed, double a, double rmax, double* m, double* const pos[3],
+                double* const vel[3]);
+void ic_ring(size_t n_total, double R, double omega, double ratio, double* m, double* const pos[3],
+             double* const vel[3], double* const force[3]);
+size_t yoshida(int order, double* c, double* d);
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b);
+    void release();
+};
+
+// Context internals used by the learned-form module.
+nb_status set_error(nb_status s, const std::string& msg);
+int kernel_form(int form);
+hipError_t force_eval(nb_ctx* c, int kform, double eps, nbk::P4*& part, int& S);
+hipStream_t ctx_stream(nb_ctx* c);
+size_t ctx_n(nb_ctx* c);
+nbk::P4* ctx_pm(nb_ctx* c);
+nbk::P4* ctx_vel(nb_ctx* c);
+nbk::P4* ctx_acc(nb_ctx* c);
+int ctx_num_cus(nb_ctx* c);
+nbl::Learned*& ctx_learned(nb_ctx* c);
+
+}  // namespace nbh

@@ -1,0 +1,43 @@
+// nbody_kernels.h -- internal interface between the C ABI layer and the
+// gfx950 kernels in nbody_kernels.hip.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nbk {
+
+constexpr int BLOCK = 256;   // threads per workgroup (4 wave64s)
+
+// One particle record in HBM: (x, y, z, m) / (vx, vy, vz, 0) / (ax, ay, az, 0).
+struct __attribute__((aligned(32))) P4 {
+    double x, y, z, m;
+};
+
+enum Form { FORM_FP64 = 0, FORM_MIXED = 1, FORM_MIXED_NR = 2 };
+
+// Launch description of the force kernel: accelerations of pi[0..ni) from
+// pj[jbeg..jend), the j-range cut into `splits` chunks of `jchunk` (one per
+// blockIdx.y); chunk s writes out[(split_base + s) * out_stride + i].
+struct AccelLaunch {
+    int form = FORM_FP64;
+    int ipt = 1;              // i-particles per lane (1 or 2)
+    const P4* pi = nullptr;
+    int ni = 0;
+    const P4* pj = nullptr;
+    int jbeg = 0, jend = 0, jchunk = 0, splits = 1;
+    double eps2 = 0.0, G = 1.0;
+    P4* out = nullptr;
+    long out_stride = 0;
+    int split_base = 0;
+};
+
+hipError_t launch_accel(const AccelLaunch& L, hipStream_t st);
+hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st);
+hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
+                             double kdt, double c1dt, double c2dt, hipStream_t st);
+hipError_t launch_drift(int n, P4* pm, const P4* vel, double cdt, hipStream_t st);
+// rows: splits * n * 16 bytes; blocks_out: 4 doubles per 256-particle block.
+hipError_t launch_energy(const P4* pm, const P4* vel, int n, int splits, int jchunk, double eps2,
+                         void* rows, double* blocks_out, hipStream_t st);
+
+}  // namespace nbk

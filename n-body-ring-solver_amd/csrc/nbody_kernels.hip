@@ -1,0 +1,403 @@
+// nbody_kernels.hip -- gfx950 kernels for the softened all-pairs force,
+// the fused leapfrog kick/drift and the energy diagnostic.
+//
+// Data layout in HBM: one 32-byte record per particle, (x, y, z, m) for
+// position+mass (the FPGA particle_t = double4, FPGA/host/solver.cpp:721),
+// (vx, vy, vz, 0) for velocity and (ax, ay, az, 0) for acceleration.
+//
+// Force kernel scheme: a 256-thread workgroup owns 256*IPT i-particles held in
+// registers (IPT per lane).  It streams its j-range through LDS in tiles of
+// 256 records (one coalesced 32-byte load per lane, double-buffered, one
+// barrier per tile); every lane of a wave then reads the same LDS record
+// (broadcast, no bank conflicts) and evaluates IPT pairs.  For small N the
+// j-range is split over blockIdx.y so the grid still fills 256 CUs; the
+// partial sums are added in a fixed order by the kick/drift or reduce kernel,
+// so results are bitwise reproducible.
+//
+// Pair term (reference CPU/force_calculation.cpp:78-82, softened):
+//   a_i += m_j (x_j - x_i) / (|x_j - x_i|^2 + eps^2)^(3/2)
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "nbody_kernels.h"
+
+namespace nbk {
+
+// LDS record of the mixed form: FP64 position (differences stay FP64), FP32 mass.
+struct __attribute__((aligned(32))) P4m {
+    double x, y, z;
+    float m, pad;
+};
+
+__device__ __forceinline__ P4 load_p4(const P4* __restrict__ p, int j, int jend) {
+    if (j < jend) return p[j];
+    return P4{0.0, 0.0, 0.0, 0.0};
+}
+
+// ---------------------------------------------------------------------------
+// Pair kernels
+// ---------------------------------------------------------------------------
+
+// FP64: v_rsq_f64 gives y ~ r2^(-1/2) to about 2^-22 relative; with
+// e = 1 - r2 y^2 the exact y_true^3 = y^3 (1-e)^(-3/2) = y^3 (1 + 3/2 e + 15/8 e^2 + O(e^3)),
+// so two FMAs restore full FP64 accuracy (|e|^3 ~ 2^-66).
+template <bool EPS0>
+__device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, double xj, double yj,
+                                          double zj, double mj, double eps2, double& ax, double& ay,
+                                          double& az) {
+    const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double t = y * y;
+    const double e = __builtin_fma(-r2, t, 1.0);
+    const double y3 = t * y;
+    const double p = __builtin_fma(e, __builtin_fma(e, 1.875, 1.5), 1.0);
+    double s = (mj * y3) * p;
+    if (EPS0) s = (r2 > 0.0) ? s : 0.0;   // zero separation (self pair) contributes 0
+    ax = __builtin_fma(s, dx, ax);
+    ay = __builtin_fma(s, dy, ay);
+    az = __builtin_fma(s, dz, az);
+}
+
+// Mixed: FP64 differences rounded to FP32, FP32 rsqrt (1 ulp) plus one
+// Newton-Raphson step y <- y (3/2 - r2/2 y^2), FP32 pair arithmetic.
+template <bool EPS0, bool NR>
+__device__ __forceinline__ void pair_mixed(double xi, double yi, double zi, double xj, double yj,
+                                           double zj, float mj, float eps2, float& ax, float& ay,
+                                           float& az) {
+    const float dx = (float)(xj - xi), dy = (float)(yj - yi), dz = (float)(zj - zi);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    float y = __builtin_amdgcn_rsqf(r2);
+    if (NR) {
+        const float h = 0.5f * r2;
+        y = y * __builtin_fmaf(-h, y * y, 1.5f);
+    }
+    const float y2 = y * y;
+    float s = (mj * y) * y2;
+    if (EPS0) s = (r2 > 0.0f) ? s : 0.0f;
+    ax = __builtin_fmaf(s, dx, ax);
+    ay = __builtin_fmaf(s, dy, ay);
+    az = __builtin_fmaf(s, dz, az);
+}
+
+// ---------------------------------------------------------------------------
+// Force kernel (asymmetric: all ordered pairs of i-set x j-range)
+// ---------------------------------------------------------------------------
+template <int FORM, bool EPS0, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int ni,
+                                                 const P4* __restrict__ pj, int jbeg_all,
+                                                 int jend_all, int jchunk, double eps2, double G,
+                                                 P4* __restrict__ out, long out_stride,
+                                                 int split_base) {
+    constexpr bool MIXED = (FORM != FORM_FP64);
+    constexpr bool NR = (FORM == FORM_MIXED_NR);
+    using Rec = typename std::conditional<MIXED, P4m, P4>::type;
+    __shared__ Rec tile[2][BLOCK];
+
+    const int t = threadIdx.x;
+    const int split = blockIdx.y;
+    const int jbeg = jbeg_all + split * jchunk;
+    const int jend = min(jend_all, jbeg + jchunk);
+    const int ibase = blockIdx.x * BLOCK * IPT;
+
+    double xi[IPT], yi[IPT], zi[IPT];
+    double ax[IPT], ay[IPT], az[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int i = ibase + k * BLOCK + t;
+        const P4 p = (i < ni) ? pi[i] : P4{0.0, 0.0, 0.0, 0.0};
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+        ax[k] = ay[k] = az[k] = 0.0;
+    }
+
+    const int ntiles = (jend > jbeg) ? (jend - jbeg + BLOCK - 1) / BLOCK : 0;
+    P4 nxt = load_p4(pj, jbeg + t, jend);
+    const float eps2f = (float)eps2;
+
+    for (int tt = 0; tt < ntiles; tt++) {
+        const int buf = tt & 1;
+        if constexpr (MIXED) {
+            tile[buf][t] = P4m{nxt.x, nxt.y, nxt.z, (float)nxt.m, 0.0f};
+        } else {
+            tile[buf][t] = nxt;
+        }
+        __syncthreads();
+        const int j0 = jbeg + tt * BLOCK;
+        if (tt + 1 < ntiles) nxt = load_p4(pj, j0 + BLOCK + t, jend);
+        const int cnt = min(BLOCK, jend - j0);
+        const int cnt8 = (cnt + 7) & ~7;   // entries past jend are zero-mass records
+        const Rec* tl = tile[buf];
+        if constexpr (MIXED) {
+            float fx[IPT], fy[IPT], fz[IPT];
+#pragma unroll
+            for (int k = 0; k < IPT; k++) fx[k] = fy[k] = fz[k] = 0.0f;
+            for (int jj = 0; jj < cnt8; jj += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const P4m q = tl[jj + u];
+#pragma unroll
+                    for (int k = 0; k < IPT; k++)
+                        pair_mixed<EPS0, NR>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2f,
+                                             fx[k], fy[k], fz[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < IPT; k++) {   // flush the FP32 tile sums into FP64
+                ax[k] += (double)fx[k];
+                ay[k] += (double)fy[k];
+                az[k] += (double)fz[k];
+            }
+        } else {
+            for (int jj = 0; jj < cnt8; jj += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const P4 q = tl[jj + u];
+#pragma unroll
+                    for (int k = 0; k < IPT; k++)
+                        pair_fp64<EPS0>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2, ax[k], ay[k],
+                                        az[k]);
+                }
+            }
+        }
+    }
+
+    P4* o = out + (long)(split_base + split) * out_stride;
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        const int i = ibase + k * BLOCK + t;
+        if (i < ni) o[i] = P4{G * ax[k], G * ay[k], G * az[k], 0.0};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Partial-sum reduction, kick and drift
+// ---------------------------------------------------------------------------
+
+// acc_i = sum_s part[s][i] in order s = 0..S-1.
+__device__ __forceinline__ void sum_parts(const P4* __restrict__ part, int S, long stride, int i,
+                                          double& ax, double& ay, double& az) {
+    ax = ay = az = 0.0;
+    for (int s = 0; s < S; s++) {
+        const P4 p = part[(long)s * stride + i];
+        ax += p.x; ay += p.y; az += p.z;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reduce(const P4* __restrict__ part, int S, long stride,
+                                                  int n, P4* __restrict__ acc) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double ax, ay, az;
+    sum_parts(part, S, stride, i, ax, ay, az);
+    acc[i] = P4{ax, ay, az, 0.0};
+}
+
+// v += kdt a; x += c1dt v; x += c2dt v  (the kick d[s] dt followed by the drift
+// of the next substep; c2dt carries the first drift of the next step when the
+// last drift of this step is merged with it, FPGA integrator :853).
+__global__ __launch_bounds__(BLOCK) void k_kick_drift(const P4* __restrict__ part, int S, long stride,
+                                                      int n, P4* __restrict__ pm, P4* __restrict__ vel,
+                                                      P4* __restrict__ acc, double kdt, double c1dt,
+                                                      double c2dt) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double ax, ay, az;
+    sum_parts(part, S, stride, i, ax, ay, az);
+    if (acc != part) acc[i] = P4{ax, ay, az, 0.0};
+    P4 v = vel[i];
+    v.x = __builtin_fma(kdt, ax, v.x);
+    v.y = __builtin_fma(kdt, ay, v.y);
+    v.z = __builtin_fma(kdt, az, v.z);
+    vel[i] = v;
+    P4 p = pm[i];
+    p.x = __builtin_fma(c1dt, v.x, p.x);
+    p.y = __builtin_fma(c1dt, v.y, p.y);
+    p.z = __builtin_fma(c1dt, v.z, p.z);
+    p.x = __builtin_fma(c2dt, v.x, p.x);
+    p.y = __builtin_fma(c2dt, v.y, p.y);
+    p.z = __builtin_fma(c2dt, v.z, p.z);
+    pm[i] = p;
+}
+
+// x += cdt v
+__global__ __launch_bounds__(BLOCK) void k_drift(int n, P4* __restrict__ pm, const P4* __restrict__ vel,
+                                                 double cdt) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const P4 v = vel[i];
+    P4 p = pm[i];
+    p.x = __builtin_fma(cdt, v.x, p.x);
+    p.y = __builtin_fma(cdt, v.y, p.y);
+    p.z = __builtin_fma(cdt, v.z, p.z);
+    pm[i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// Energy diagnostic
+// ---------------------------------------------------------------------------
+
+// Error-free sum (Knuth TwoSum): s + c == a + b exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& c) {
+    s = a + b;
+    const double bb = s - a;
+    c = (a - (s - bb)) + (b - bb);
+}
+
+struct __attribute__((aligned(16))) DD { double s, c; };
+
+__device__ __forceinline__ void dd_add(DD& acc, double v) {
+    double s, e;
+    two_sum(acc.s, v, s, e);
+    acc.s = s;
+    acc.c += e;
+}
+
+__device__ __forceinline__ void dd_merge(DD& a, const DD& b) {
+    double s, e;
+    two_sum(a.s, b.s, s, e);
+    a.s = s;
+    a.c += e + b.c;
+}
+
+// row_i (split part) = sum_j m_j / sqrt(|x_j - x_i|^2 + eps^2), zero-separation
+// pairs skipped; FP64 with a compensated per-lane sum.
+__global__ __launch_bounds__(BLOCK) void k_potential(const P4* __restrict__ pm, int n, int jchunk,
+                                                     double eps2, DD* __restrict__ out) {
+    __shared__ P4 tile[2][BLOCK];
+    const int t = threadIdx.x;
+    const int split = blockIdx.y;
+    const int jbeg = split * jchunk;
+    const int jend = min(n, jbeg + jchunk);
+    const int i = blockIdx.x * BLOCK + t;
+    const P4 me = (i < n) ? pm[i] : P4{0.0, 0.0, 0.0, 0.0};
+    DD acc{0.0, 0.0};
+    const int ntiles = (jend > jbeg) ? (jend - jbeg + BLOCK - 1) / BLOCK : 0;
+    P4 nxt = load_p4(pm, jbeg + t, jend);
+    for (int tt = 0; tt < ntiles; tt++) {
+        const int buf = tt & 1;
+        tile[buf][t] = nxt;
+        __syncthreads();
+        const int j0 = jbeg + tt * BLOCK;
+        if (tt + 1 < ntiles) nxt = load_p4(pm, j0 + BLOCK + t, jend);
+        const int cnt = min(BLOCK, jend - j0);
+        for (int jj = 0; jj < cnt; jj++) {
+            const P4 q = tile[buf][jj];
+            const double dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+            const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+            if (d2 == 0.0) continue;
+            const double s2 = d2 + eps2;
+            const double y = __builtin_amdgcn_rsq(s2);
+            const double e = __builtin_fma(-s2, y * y, 1.0);
+            // (1-e)^(-1/2) = 1 + e/2 + 3/8 e^2 + O(e^3)
+            const double inv = y * __builtin_fma(e, __builtin_fma(e, 0.375, 0.5), 1.0);
+            dd_add(acc, q.m * inv);
+        }
+    }
+    if (i < n) out[(long)split * n + i] = acc;
+}
+
+// Per block: U_b = sum_i m_i row_i (row over all splits), T_b = sum_i m_i |v_i|^2,
+// compensated, fixed tree order.  out[b] = {U.s, U.c, T.s, T.c}.
+__global__ __launch_bounds__(BLOCK) void k_energy_reduce(const DD* __restrict__ rows, int S, int n,
+                                                         const P4* __restrict__ pm,
+                                                         const P4* __restrict__ vel,
+                                                         double* __restrict__ out) {
+    __shared__ DD su[BLOCK], st[BLOCK];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * BLOCK + t;
+    DD u{0.0, 0.0}, k{0.0, 0.0};
+    if (i < n) {
+        DD row{0.0, 0.0};
+        for (int s = 0; s < S; s++) dd_merge(row, rows[(long)s * n + i]);
+        const double m = pm[i].m;
+        const double r = row.s + row.c;
+        u.s = m * r;
+        u.c = __builtin_fma(m, r, -u.s);   // exact product error
+        const P4 v = vel[i];
+        const double v2 = __builtin_fma(v.z, v.z, __builtin_fma(v.y, v.y, v.x * v.x));
+        k.s = m * v2;
+        k.c = __builtin_fma(m, v2, -k.s);
+    }
+    su[t] = u;
+    st[t] = k;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            dd_merge(su[t], su[t + w]);
+            dd_merge(st[t], st[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[4 * blockIdx.x + 0] = su[0].s;
+        out[4 * blockIdx.x + 1] = su[0].c;
+        out[4 * blockIdx.x + 2] = st[0].s;
+        out[4 * blockIdx.x + 3] = st[0].c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+template <int FORM, bool EPS0, int IPT>
+static hipError_t launch_accel_t(const AccelLaunch& L, hipStream_t st) {
+    const dim3 grid((L.ni + BLOCK * IPT - 1) / (BLOCK * IPT), L.splits);
+    hipLaunchKernelGGL((k_accel<FORM, EPS0, IPT>), grid, dim3(BLOCK), 0, st, L.pi, L.ni, L.pj,
+                       L.jbeg, L.jend, L.jchunk, L.eps2, L.G, L.out, L.out_stride, L.split_base);
+    return hipGetLastError();
+}
+
+template <int FORM, bool EPS0>
+static hipError_t launch_accel_ipt(const AccelLaunch& L, hipStream_t st) {
+    return (L.ipt == 2) ? launch_accel_t<FORM, EPS0, 2>(L, st) : launch_accel_t<FORM, EPS0, 1>(L, st);
+}
+
+template <int FORM>
+static hipError_t launch_accel_eps(const AccelLaunch& L, hipStream_t st) {
+    return (L.eps2 == 0.0) ? launch_accel_ipt<FORM, true>(L, st) : launch_accel_ipt<FORM, false>(L, st);
+}
+
+hipError_t launch_accel(const AccelLaunch& L, hipStream_t st) {
+    if (L.ni <= 0 || L.splits <= 0) return hipSuccess;
+    switch (L.form) {
+    case FORM_FP64: return launch_accel_eps<FORM_FP64>(L, st);
+    case FORM_MIXED: return launch_accel_eps<FORM_MIXED>(L, st);
+    case FORM_MIXED_NR: return launch_accel_eps<FORM_MIXED_NR>(L, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, S, stride, n, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
+                             double kdt, double c1dt, double c2dt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kick_drift, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, S, stride,
+                       n, pm, vel, acc, kdt, c1dt, c2dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_drift(int n, P4* pm, const P4* vel, double cdt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_drift, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, n, pm, vel, cdt);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(const P4* pm, const P4* vel, int n, int splits, int jchunk, double eps2,
+                         void* rows, double* blocks_out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((n + BLOCK - 1) / BLOCK, splits);
+    hipLaunchKernelGGL(k_potential, grid, dim3(BLOCK), 0, st, pm, n, jchunk, eps2, (DD*)rows);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_energy_reduce, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const DD*)rows,
+                       splits, n, pm, vel, blocks_out);
+    return hipGetLastError();
+}
+
+}  // namespace nbk

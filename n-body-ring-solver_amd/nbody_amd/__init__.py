@@ -1,0 +1,304 @@
+"""Python view of the gfx950 N-body solver's C ABI (include/nbody.h).
+
+A thin ctypes layer: every call goes to lib/libnbody_hip.so, which holds the
+HIP kernels.  There is no Python or CPU fallback -- if the library is missing
+or no gfx950 device is present, the calls raise.
+
+Arrays follow the reference's struct-of-arrays layout (CPU/nbody_system.h:41-74):
+positions, velocities, forces and accelerations are (3, N) float64 arrays and
+masses are (N,).  Ring-share particle sets are (n, 4) rows of (x, y, z, m), the
+FPGA particle_t layout (FPGA/host/solver.cpp:721).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "LIB_PATH", "NBodyError", "Solver", "load_library", "init_system", "yoshida", "FORMS", "IC_KINDS",
+    "mlp_param_count",
+]
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libnbody_hip.so")
+
+FORMS = {"fp64": 0, "mixed": 1, "learned": 2, "mixed-raw": 3}
+IC_KINDS = {"ring": 0, "sphere": 1, "plummer": 2}
+STATUS = {0: "NB_OK", 1: "NB_ERR_ARG", 2: "NB_ERR_DEVICE", 3: "NB_ERR_ALLOC", 4: "NB_ERR_STATE",
+          5: "NB_ERR_LAUNCH"}
+
+# Every symbol include/nbody.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "nb_last_error", "nb_version", "nb_create", "nb_destroy", "nb_init_system", "nb_yoshida",
+    "nb_set_state", "nb_get_state", "nb_accel", "nb_compute_forces", "nb_accel_device",
+    "nb_accel_block", "nb_accel_block_device", "nb_step", "nb_energy", "nb_timing_enable",
+    "nb_timing_read", "nb_timing_reset", "nb_synchronize", "nb_learned_configure",
+    "nb_learned_set_mode", "nb_learned_collect", "nb_learned_train", "nb_learned_stats_read",
+    "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
+)
+
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_P3 = C.POINTER(_D)
+_SZ = C.c_size_t
+_VP = C.c_void_p
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [("exact_evals", C.c_uint64), ("predicted_evals", C.c_uint64), ("fallbacks", C.c_uint64),
+                ("last_check_err", C.c_double), ("max_check_err", C.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load (once) and prototype libnbody_hip.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(path)
+    L.nb_last_error.restype = C.c_char_p
+    L.nb_version.restype = C.c_char_p
+    L.nb_create.argtypes = [C.POINTER(_VP), C.c_int]
+    L.nb_destroy.argtypes = [_VP]
+    L.nb_init_system.argtypes = [C.c_int, _SZ, C.c_uint64, _D, _D, _P3, _P3, _P3]
+    L.nb_yoshida.argtypes = [C.c_int, _D, _D, C.POINTER(_SZ)]
+    L.nb_set_state.argtypes = [_VP, _SZ, _D, _P3, _P3]
+    L.nb_get_state.argtypes = [_VP, _P3, _P3]
+    L.nb_accel.argtypes = [_VP, C.c_int, C.c_double, _P3]
+    L.nb_compute_forces.argtypes = [_VP, C.c_int, C.c_double, _P3]
+    L.nb_accel_device.argtypes = [_VP, C.c_int, C.c_double]
+    L.nb_accel_block.argtypes = [_VP, C.c_int, C.c_double, _SZ, _D, _SZ, _D, _SZ, _D]
+    L.nb_accel_block_device.argtypes = [_VP, C.c_int, C.c_double, _SZ, _VP, _SZ, _VP, _SZ, _VP]
+    L.nb_step.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
+    L.nb_energy.argtypes = [_VP, C.c_double, _D, _D]
+    L.nb_timing_enable.argtypes = [_VP, C.c_int]
+    L.nb_timing_read.argtypes = [_VP, _D, C.POINTER(C.c_uint64), _D]
+    L.nb_timing_reset.argtypes = [_VP]
+    L.nb_synchronize.argtypes = [_VP]
+    L.nb_learned_configure.argtypes = [_VP, C.c_int, C.c_double, C.c_int]
+    L.nb_learned_set_mode.argtypes = [_VP, C.c_int]
+    L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
+    L.nb_learned_train.argtypes = [_VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]
+    L.nb_learned_stats_read.argtypes = [_VP, C.POINTER(LearnedStats)]
+    L.nb_mlp_param_count.argtypes = [C.POINTER(_SZ)]
+    L.nb_mlp_get_params.argtypes = [_VP, _F]
+    L.nb_mlp_set_params.argtypes = [_VP, _F]
+    L.nb_mlp_forward.argtypes = [_VP, _SZ, _F, _F]
+    L.nb_mlp_grad.argtypes = [_VP, _SZ, _F, _F, _F, _D]
+    for name in EXPORTS:
+        if name not in ("nb_last_error", "nb_version"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(status: int, where: str):
+    if status != 0:
+        raise NBodyError(status, where, _lib.nb_last_error().decode())
+
+
+def _p(a: np.ndarray, dtype=np.float64):
+    assert a.dtype == dtype and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_D if dtype == np.float64 else _F)
+
+
+def _planes(a: np.ndarray):
+    """(3, N) C-contiguous float64 -> double*[3]."""
+    assert a.ndim == 2 and a.shape[0] == 3 and a.dtype == np.float64 and a.flags.c_contiguous
+    return (_D * 3)(*[a[k].ctypes.data_as(_D) for k in range(3)])
+
+
+def _form(form) -> int:
+    if isinstance(form, str):
+        return FORMS[form]
+    return int(form)
+
+
+def init_system(kind: str, n: int, seed: int = 42, params=None):
+    """Seeded initial conditions: returns mass (N,), pos (3,N), vel (3,N), force (3,N) or None."""
+    L = load_library()
+    m = np.empty(n)
+    pos, vel = np.empty((3, n)), np.empty((3, n))
+    force = np.empty((3, n)) if kind == "ring" else None
+    par = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    _check(L.nb_init_system(IC_KINDS[kind], n, seed, None if par is None else _p(par), _p(m), _planes(pos),
+                            _planes(vel), None if force is None else _planes(force)), "nb_init_system")
+    return m, pos, vel, force
+
+
+def yoshida(order: int):
+    L = load_library()
+    c, d = np.zeros(16), np.zeros(16)
+    lf = _SZ()
+    _check(L.nb_yoshida(order, _p(c), _p(d), C.byref(lf)), "nb_yoshida")
+    return c[:lf.value].copy(), d[:lf.value - 1].copy(), lf.value
+
+
+def mlp_param_count() -> int:
+    L = load_library()
+    n = _SZ()
+    _check(L.nb_mlp_param_count(C.byref(n)), "nb_mlp_param_count")
+    return n.value
+
+
+class Solver:
+    """One context on one GPU: the device-resident system and its stream."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = _VP()
+        _check(self.L.nb_create(C.byref(h), device), "nb_create")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- state ---------------------------------------------------------------
+    def set_state(self, mass, pos, vel=None):
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = mass.shape[0]
+        assert pos.shape == (3, n)
+        vp = None
+        if vel is not None:
+            vel = np.ascontiguousarray(vel, dtype=np.float64)
+            assert vel.shape == (3, n)
+            vp = _planes(vel)
+        _check(self.L.nb_set_state(self.h, n, _p(mass), _planes(pos), vp), "nb_set_state")
+        self.n = n
+
+    def get_state(self):
+        pos, vel = np.empty((3, self.n)), np.empty((3, self.n))
+        _check(self.L.nb_get_state(self.h, _planes(pos), _planes(vel)), "nb_get_state")
+        return pos, vel
+
+    # --- forces --------------------------------------------------------------
+    def accel(self, form="fp64", eps: float = 0.0) -> np.ndarray:
+        a = np.empty((3, self.n))
+        _check(self.L.nb_accel(self.h, _form(form), eps, _planes(a)), "nb_accel")
+        return a
+
+    def compute_forces(self, form="fp64", eps: float = 0.0) -> np.ndarray:
+        f = np.empty((3, self.n))
+        _check(self.L.nb_compute_forces(self.h, _form(form), eps, _planes(f)), "nb_compute_forces")
+        return f
+
+    def accel_device(self, form="fp64", eps: float = 0.0):
+        _check(self.L.nb_accel_device(self.h, _form(form), eps), "nb_accel_device")
+
+    def accel_block(self, pi, pj, form="fp64", eps: float = 0.0, j_block: int = 0) -> np.ndarray:
+        pi = np.ascontiguousarray(pi, dtype=np.float64)
+        pj = np.ascontiguousarray(pj, dtype=np.float64)
+        assert pi.ndim == 2 and pi.shape[1] == 4 and pj.ndim == 2 and pj.shape[1] == 4
+        out = np.empty((pi.shape[0], 3))
+        _check(self.L.nb_accel_block(self.h, _form(form), eps, pi.shape[0], _p(pi), pj.shape[0], _p(pj), j_block,
+                                     _p(out)), "nb_accel_block")
+        return out
+
+    def accel_block_device(self, form, eps, ni, d_pi, nj, d_pj, j_block, d_out):
+        """Device pointers (ints, e.g. torch tensor.data_ptr()); queued, not synchronised."""
+        _check(self.L.nb_accel_block_device(self.h, _form(form), eps, ni, d_pi, nj, d_pj, j_block, d_out),
+               "nb_accel_block_device")
+
+    # --- integration ---------------------------------------------------------
+    def step(self, steps: int, dt: float, order: int = 2, form="fp64", eps: float = 0.0, coeffs=None):
+        c, d, lf = coeffs if coeffs is not None else yoshida(order)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        _check(self.L.nb_step(self.h, steps, dt, _p(c), _p(d), lf, _form(form), eps), "nb_step")
+
+    def energy(self, eps: float = 0.0):
+        T, U = C.c_double(), C.c_double()
+        _check(self.L.nb_energy(self.h, eps, C.byref(T), C.byref(U)), "nb_energy")
+        return T.value, U.value
+
+    # --- timing --------------------------------------------------------------
+    def timing(self, enable: bool = True):
+        _check(self.L.nb_timing_enable(self.h, int(enable)), "nb_timing_enable")
+
+    def timing_read(self):
+        ms, st = C.c_double(), C.c_double()
+        ev = C.c_uint64()
+        _check(self.L.nb_timing_read(self.h, C.byref(ms), C.byref(ev), C.byref(st)), "nb_timing_read")
+        return ms.value, ev.value, st.value
+
+    def timing_reset(self):
+        _check(self.L.nb_timing_reset(self.h), "nb_timing_reset")
+
+    def synchronize(self):
+        _check(self.L.nb_synchronize(self.h), "nb_synchronize")
+
+    # --- learned extrapolator -----------------------------------------------
+    def learned_configure(self, period: int = 4, threshold: float = 1e-3, exact_form="mixed"):
+        _check(self.L.nb_learned_configure(self.h, period, threshold, _form(exact_form)), "nb_learned_configure")
+
+    def learned_set_mode(self, mode: int):
+        _check(self.L.nb_learned_set_mode(self.h, mode), "nb_learned_set_mode")
+
+    def learned_collect(self, steps: int, dt: float, order: int = 2, form="mixed", eps: float = 0.0):
+        c, d, lf = yoshida(order)
+        _check(self.L.nb_learned_collect(self.h, steps, dt, _p(c), _p(d), lf, _form(form), eps),
+               "nb_learned_collect")
+
+    def learned_train(self, iters: int, batch: int = 4096, lr: float = 1e-3, seed: int = 1) -> float:
+        loss = C.c_double()
+        _check(self.L.nb_learned_train(self.h, iters, batch, lr, seed, C.byref(loss)), "nb_learned_train")
+        return loss.value
+
+    def learned_stats(self) -> dict:
+        s = LearnedStats()
+        _check(self.L.nb_learned_stats_read(self.h, C.byref(s)), "nb_learned_stats_read")
+        return {k: getattr(s, k) for k, _ in LearnedStats._fields_}
+
+    def mlp_get_params(self) -> np.ndarray:
+        out = np.empty(mlp_param_count(), dtype=np.float32)
+        _check(self.L.nb_mlp_get_params(self.h, _p(out, np.float32)), "nb_mlp_get_params")
+        return out
+
+    def mlp_set_params(self, params: np.ndarray):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.shape == (mlp_param_count(),)
+        _check(self.L.nb_mlp_set_params(self.h, _p(params, np.float32)), "nb_mlp_set_params")
+
+    def mlp_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        assert x.ndim == 2 and x.shape[1] == 15
+        y = np.empty((x.shape[0], 3), dtype=np.float32)
+        _check(self.L.nb_mlp_forward(self.h, x.shape[0], _p(x, np.float32), _p(y, np.float32)), "nb_mlp_forward")
+        return y
+
+    def mlp_grad(self, x: np.ndarray, target: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.empty(mlp_param_count(), dtype=np.float32)
+        loss = C.c_double()
+        _check(self.L.nb_mlp_grad(self.h, x.shape[0], _p(x, np.float32), _p(target, np.float32), _p(g, np.float32),
+                                  C.byref(loss)), "nb_mlp_grad")
+        return g, loss.value

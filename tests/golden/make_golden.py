@@ -1,0 +1,71 @@
+"""Write the golden fixtures from the reference's own CPU routines.
+
+Runs the reference's force kernels, leapfrog driver and energy functions
+(compiled from its sources by oracle/Makefile into oracle/_ref/libnbref.so)
+on small seeded inputs, and stores inputs and outputs as .npz files next to
+this script.  Re-run with `python tests/golden/make_golden.py` after
+`make -C oracle ref`; the fixtures are data (inputs and expected outputs)
+and contain no reference code.
+
+The reference has no softening, so every fixture is at eps = 0.
+
+Fixtures
+  ring_<N>.npz      reference ring + centre IC (CPU/nbody_system.cpp:235):
+                    m, pos, vel, analytic force; forces of
+                    compute_forces_avx2_blocked (force_avx2),
+                    compute_forces_avx512_rsqrt_4i_blocked (force_avx512),
+                    compute_forces_avx2_blocked_sp (force_sp)
+  jitter_<N>.npz    the ring with seeded 3-D jitter (so no symmetry hides a
+                    wrong sign or index), same forces, energies
+                    (T, U_avx2, U_scalar), and 10 leapfrog steps of
+                    reduced_solver_omp_lf at orders 2, 4, 6, 8
+                    (pos_lf{k}, vel_lf{k}) with dt = 1e-3 and the Yoshida
+                    coefficients the reference builds (c{k}, d{k}).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+
+from oracle import Oracle, Reference  # noqa: E402
+
+
+def main():
+    ref = Reference()
+    orc = Oracle()
+    for n in (64, 257, 2049):
+        m, pos, vel, f = ref.init_ring(n, 1.0, 2 * np.pi, 1e5)
+        np.savez_compressed(
+            os.path.join(HERE, f"ring_{n}.npz"), m=m, pos=pos, vel=vel, force_analytic=f,
+            force_avx2=ref.forces(Reference.AVX2_BLOCKED, m, pos),
+            force_avx512=ref.forces(Reference.AVX512_RSQRT_4I, m, pos),
+            force_sp=ref.forces(Reference.AVX2_BLOCKED_SP, m, pos))
+    for n in (64, 300):
+        m, pos, vel, _ = ref.init_ring(n, 1.0, 2 * np.pi, 1e3)
+        rng = np.random.default_rng(1234 + n)
+        pos = np.ascontiguousarray(pos + 0.05 * rng.standard_normal(pos.shape))
+        vel = np.ascontiguousarray(vel + 0.05 * rng.standard_normal(vel.shape))
+        # masses spread over two decades so a swapped m_i / m_j shows up
+        m = np.ascontiguousarray(m * np.exp(rng.uniform(-2.3, 2.3, n)))
+        out = dict(m=m, pos=pos, vel=vel,
+                   force_avx2=ref.forces(Reference.AVX2_BLOCKED, m, pos),
+                   force_avx512=ref.forces(Reference.AVX512_RSQRT_4I, m, pos),
+                   force_sp=ref.forces(Reference.AVX2_BLOCKED_SP, m, pos))
+        T, Ua, Us = ref.energy(m, pos, vel)
+        out.update(T=T, U_avx2=Ua, U_scalar=Us)
+        for order in (2, 4, 6, 8):
+            c, d, lf = orc.yoshida(order)
+            p1, v1 = ref.leapfrog(m, pos, vel, 10, 1e-3, c, d)
+            out[f"pos_lf{order}"], out[f"vel_lf{order}"] = p1, v1
+            out[f"c{order}"], out[f"d{order}"] = c, d
+        np.savez_compressed(os.path.join(HERE, f"jitter_{n}.npz"), **out)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
