@@ -131,17 +131,21 @@ void ic_ring(size_t n_total, double R, double omega, double ratio, double* m, do
     sum += ratio;
     total_mass /= sum;
     for (size_t i = 0; i < n; i++) {
+        // one sincos per angle, as the reference's compiled code evaluates it
+        // (its sin/cos pairs are merged by the compiler): identical bits
         const double th = double(i) * 2 * M_PI / double(n);
+        double sn, cs;
+        ::sincos(th, &sn, &cs);
         m[i] = total_mass;
-        pos[0][i] = R * std::cos(th);
-        pos[1][i] = R * std::sin(th);
+        pos[0][i] = R * cs;
+        pos[1][i] = R * sn;
         pos[2][i] = 0.0;
-        vel[0][i] = -R * omega * std::sin(th);
-        vel[1][i] = R * omega * std::cos(th);
+        vel[0][i] = -R * omega * sn;
+        vel[1][i] = R * omega * cs;
         vel[2][i] = 0.0;
         if (force) {
-            force[0][i] = -G * m[i] * R * omega * omega * std::cos(th);
-            force[1][i] = -G * m[i] * R * omega * omega * std::sin(th);
+            force[0][i] = -G * m[i] * R * omega * omega * cs;
+            force[1][i] = -G * m[i] * R * omega * omega * sn;
             force[2][i] = 0.0;
         }
     }

@@ -29,6 +29,7 @@
  * FP64 MLP) are pinned by the tests against closed forms, long-double brute
  * force and distribution statistics; each says how in its comment.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -182,12 +183,15 @@ ORC_API void orc_ic_ring(size_t n_total, double R, double omega, double ratio, d
     sum += ratio;
     total_mass /= sum;
     for (size_t i = 0; i < n; i++) {
-        double th = (double)i * 2 * M_PI / (double)n;
+        /* one sincos per angle: what the reference's compiled code evaluates
+           (gcc merges its sin/cos pairs), so the ICs agree bit for bit */
+        double th = (double)i * 2 * M_PI / (double)n, sn, cs;
+        sincos(th, &sn, &cs);
         m[i] = total_mass;
-        x[i] = R * cos(th); y[i] = R * sin(th); z[i] = 0.0;
-        vx[i] = -R * omega * sin(th); vy[i] = R * omega * cos(th); vz[i] = 0.0;
-        fx[i] = -G * m[i] * R * omega * omega * cos(th);
-        fy[i] = -G * m[i] * R * omega * omega * sin(th);
+        x[i] = R * cs; y[i] = R * sn; z[i] = 0.0;
+        vx[i] = -R * omega * sn; vy[i] = R * omega * cs; vz[i] = 0.0;
+        fx[i] = -G * m[i] * R * omega * omega * cs;
+        fy[i] = -G * m[i] * R * omega * omega * sn;
         fz[i] = 0.0;
     }
     m[n] = ratio * total_mass;

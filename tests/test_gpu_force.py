@@ -1,0 +1,165 @@
+"""GPU force parity: the HIP kernels through the C ABI against the oracle and
+the reference's golden fixtures.
+
+Bars: FP64 form within 1e-12 relative per particle of the compensated oracle
+(the kernel's rsq + cubic correction is accurate to a few ulp; the sums run in
+a different order); mixed form within 2e-5 relative per particle (the
+reference's own FP32 form reaches ~1e-5 on the jittered fixture), bitwise
+reproducible across runs.
+"""
+import numpy as np
+import pytest
+
+import nbody_amd
+from conftest import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 2e-5
+
+
+def upload(solver, m, pos, vel=None):
+    solver.set_state(m, pos, vel)
+
+
+@pytest.mark.parametrize("n", [64, 257, 2049])
+def test_ring_fixture_forces(solver, n):
+    g = golden(f"ring_{n}.npz")
+    upload(solver, g["m"], g["pos"])
+    f = solver.compute_forces("fp64", 0.0)
+    scale = np.max(np.abs(g["force_avx2"]))
+    assert np.max(np.abs(f - g["force_avx2"])) < 1e-12 * scale
+    assert np.max(np.abs(f - g["force_avx512"])) < 1e-12 * scale
+    assert np.max(np.abs(f - g["force_analytic"])) < 1e-12 * scale
+
+
+@pytest.mark.parametrize("n", [64, 300])
+def test_jitter_fixture_forces(solver, n):
+    g = golden(f"jitter_{n}.npz")
+    upload(solver, g["m"], g["pos"])
+    f = solver.compute_forces("fp64", 0.0)
+    assert rel_err(f, g["force_avx2"]) < FP64_TOL
+    fm = solver.compute_forces("mixed", 0.0)
+    err_mixed = rel_err(fm, g["force_avx2"])
+    err_ref_sp = rel_err(g["force_sp"], g["force_avx2"])
+    assert err_mixed < MIXED_TOL
+    assert err_mixed < 3 * err_ref_sp + 1e-6   # no worse than the reference's FP32 form
+
+
+@pytest.mark.parametrize("kind,n,eps", [("sphere", 2048, 1e-3), ("plummer", 4096, 0.01), ("plummer", 1000, 0.0),
+                                        ("sphere", 777, 0.05)])
+def test_forces_vs_oracle(solver, orc, kind, n, eps):
+    m, pos, vel, _ = nbody_amd.init_system(kind, n, 17)
+    upload(solver, m, pos, vel)
+    ref = orc.accel(m, pos, eps)
+    a = solver.accel("fp64", eps)
+    assert rel_err(a, ref) < FP64_TOL
+    for form in ("mixed", "mixed-raw"):
+        assert rel_err(solver.accel(form, eps), ref) < MIXED_TOL, form
+
+
+@pytest.mark.parametrize("eps", [0.0, 1e-3, 0.05])
+def test_softened_ring_closed_form(solver, orc, eps):
+    n = 513
+    m, pos, vel, _ = nbody_amd.init_system("ring", n, 0, [1.0, 2 * np.pi, 1e2])
+    upload(solver, m, pos)
+    exact = orc.ring_accel_exact(n, 1.0, m[0], m[-1], eps)
+    a = solver.accel("fp64", eps)
+    assert rel_err(a[:, :-1], exact[:, :-1]) < FP64_TOL
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 63, 64, 65, 255, 256, 257, 1023, 1025])
+def test_ragged_sizes(solver, orc, n):
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 5)
+    m = m * np.linspace(0.5, 2.0, n)
+    upload(solver, m, pos)
+    for eps in (0.0, 0.02):
+        a = solver.accel("fp64", eps)
+        ref = orc.accel(m, pos, eps)
+        if n == 1:
+            assert np.all(a == 0.0)
+        else:
+            assert rel_err(a, ref) < FP64_TOL
+
+
+def test_empty_system(solver):
+    upload(solver, np.zeros(0), np.zeros((3, 0)))
+    assert solver.accel("fp64", 0.0).shape == (3, 0)
+    solver.step(3, 1e-3, 2, "fp64", 0.0)
+    assert solver.energy(0.0) == (0.0, 0.0)
+
+
+def test_coincident_particles_and_zero_mass(solver, orc):
+    # duplicated positions and massless tracers: zero-separation pairs add nothing
+    m, pos, _, _ = nbody_amd.init_system("sphere", 300, 8)
+    pos[:, 150:160] = pos[:, 10:20]
+    m[200:210] = 0.0
+    upload(solver, m, np.ascontiguousarray(pos))
+    for eps in (0.0, 0.01):
+        a = solver.accel("fp64", eps)
+        assert np.all(np.isfinite(a))
+        assert rel_err(a, orc.accel(m, pos, eps)) < FP64_TOL
+        am = solver.accel("mixed", eps)
+        assert np.all(np.isfinite(am))
+
+
+def test_bitwise_reproducible(solver):
+    m, pos, _, _ = nbody_amd.init_system("plummer", 50000, 3)
+    upload(solver, m, pos)
+    for form in ("fp64", "mixed"):
+        a1 = solver.accel(form, 0.01)
+        a2 = solver.accel(form, 0.01)
+        assert np.array_equal(a1, a2), form
+
+
+def test_unequal_masses_momentum(solver):
+    # full-size property: sum_i m_i a_i = 0 (Newton's third law) at N = 262144
+    n = 262144
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 99)
+    upload(solver, m, pos)
+    for form, tol in (("fp64", 1e-13), ("mixed", 1e-6)):
+        a = solver.accel(form, 0.005)
+        tot = np.abs((m * a).sum(axis=1))
+        scale = (m * np.abs(a)).sum(axis=1)
+        assert np.all(tot < tol * scale), (form, tot / scale)
+
+
+def test_full_size_mixed_vs_fp64(solver):
+    n = 262144
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 99)
+    upload(solver, m, pos)
+    a64 = solver.accel("fp64", 0.005)
+    am = solver.accel("mixed", 0.005)
+    assert rel_err(am, a64) < MIXED_TOL
+
+
+def test_accel_block_vs_oracle(solver, orc):
+    m, pos, _, _ = nbody_amd.init_system("plummer", 6000, 12)
+    pj = np.stack([pos[0], pos[1], pos[2], m * np.linspace(0.5, 1.5, 6000)], axis=1)
+    pi = pj[1000:2500]                       # i is a subset of j: self pairs present
+    ref = orc.accel_block(pi, pj, 0.01)
+    for jb in (0, 1000, 2048):
+        acc = solver.accel_block(pi, pj, "fp64", 0.01, j_block=jb)
+        assert rel_err(acc.T, ref.T) < FP64_TOL, jb
+    accm = solver.accel_block(pi, pj, "mixed", 0.01, j_block=1500)
+    assert rel_err(accm.T, ref.T) < MIXED_TOL
+    # eps = 0 with i inside j: the self pair must be masked
+    acc0 = solver.accel_block(pi, pj, "fp64", 0.0)
+    assert rel_err(acc0.T, orc.accel_block(pi, pj, 0.0).T) < FP64_TOL
+
+
+def test_accel_block_empty(solver):
+    pj = np.zeros((0, 4))
+    pi = np.ones((5, 4))
+    assert np.all(solver.accel_block(pi, pj, "fp64", 0.0) == 0.0)
+    assert solver.accel_block(np.zeros((0, 4)), pi, "fp64", 0.0).shape == (0, 3)
+
+
+def test_argument_errors_on_device(solver):
+    with pytest.raises(nbody_amd.NBodyError):
+        solver.accel("fp64", -1.0)
+    with pytest.raises(nbody_amd.NBodyError):
+        solver.accel(9, 0.0)
+    with pytest.raises(nbody_amd.NBodyError):
+        solver.step(1, 1e-3, coeffs=(np.array([0.5]), np.array([]), 1))

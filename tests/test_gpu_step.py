@@ -1,0 +1,104 @@
+"""GPU leapfrog and energy parity against the oracle and the reference's
+leapfrog fixtures; integrator invariants on the GPU."""
+import numpy as np
+import pytest
+
+import nbody_amd
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def run(solver, m, pos, vel, steps, dt, order, form="fp64", eps=0.0):
+    solver.set_state(m, pos, vel)
+    solver.step(steps, dt, order, form, eps)
+    return solver.get_state()
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_reference_leapfrog_fixture(solver, order):
+    g = golden("jitter_64.npz")
+    p, v = run(solver, g["m"], g["pos"], g["vel"], 10, 1e-3, order)
+    assert np.max(np.abs(p - g[f"pos_lf{order}"])) < 1e-12 * (1 + np.max(np.abs(g[f"pos_lf{order}"])))
+    assert np.max(np.abs(v - g[f"vel_lf{order}"])) < 1e-11 * np.max(np.abs(g[f"vel_lf{order}"]))
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_leapfrog_vs_oracle(solver, orc, order):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 512, 4)
+    p, v = run(solver, m, pos, vel, 10, 1.0 / 128, order, "fp64", 0.01)
+    po, vo = orc.leapfrog(m, pos, vel, 10, 1.0 / 128, order, 0.01)
+    assert np.max(np.abs(p - po)) < 1e-12 * np.max(np.abs(po))
+    assert np.max(np.abs(v - vo)) < 1e-11 * np.max(np.abs(vo))
+
+
+def test_cold_collapse_config(solver, orc):
+    # N = 2048 uniform sphere, eps = 1e-3, dt = 1e-3, 100 steps (-T 0.1 -s 100)
+    m, pos, vel, _ = nbody_amd.init_system("sphere", 2048, 42)
+    p, v = run(solver, m, pos, vel, 100, 1e-3, 2, "fp64", 1e-3)
+    po, vo = orc.leapfrog(m, pos, vel, 100, 1e-3, 2, 1e-3)
+    assert np.max(np.abs(p - po)) < 1e-10
+    assert np.max(np.abs(v - vo)) < 1e-8 * np.max(np.abs(vo))
+
+
+def test_ring_quarter_revolution(solver, orc):
+    n = 129
+    m, pos, vel, _ = nbody_amd.init_system("ring", n, 0, [1.0, 2 * np.pi, 1e5])
+    p, v = run(solver, m, pos, vel, 400, 0.25 / 400, 4)
+    pp, vp = orc.ring_predict(n, 1.0, 2 * np.pi, 0.25)
+    assert np.max(np.abs(p - pp)) < 1e-7 and np.max(np.abs(v - vp)) < 1e-6
+
+
+def test_time_reversibility(solver):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 1024, 21)
+    p1, v1 = run(solver, m, pos, vel, 40, 1e-3, 4, "fp64", 0.05)
+    p2, v2 = run(solver, m, p1, np.ascontiguousarray(-v1), 40, 1e-3, 4, "fp64", 0.05)
+    assert np.max(np.abs(p2 - pos)) < 1e-12
+    assert np.max(np.abs(-v2 - vel)) < 1e-11
+
+
+def test_mixed_trajectory_tracks_fp64(solver):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 4096, 8)
+    p64, v64 = run(solver, m, pos, vel, 20, 1.0 / 128, 2, "fp64", 0.01)
+    pm, vm = run(solver, m, pos, vel, 20, 1.0 / 128, 2, "mixed", 0.01)
+    assert np.max(np.abs(pm - p64)) < 1e-7
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_energy_vs_oracle(solver, orc, eps):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 3000, 6)
+    m = m * np.linspace(0.5, 1.5, 3000)
+    solver.set_state(m, pos, vel)
+    T, U = solver.energy(eps)
+    To, Uo = orc.energy(m, pos, vel, eps)
+    assert abs(T - To) <= 1e-15 * abs(To)
+    assert abs(U - Uo) <= 1e-14 * abs(Uo)
+    assert solver.energy(eps) == (T, U)   # bitwise reproducible
+
+
+def test_energy_fixture(solver):
+    g = golden("jitter_300.npz")
+    solver.set_state(g["m"], g["pos"], g["vel"])
+    T, U = solver.energy(0.0)
+    assert abs(T - g["T"]) <= 1e-14 * abs(g["T"])
+    assert abs(U - g["U_scalar"]) <= 1e-14 * abs(g["U_scalar"])
+
+
+def test_energy_conservation_short(solver):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 8192, 2)
+    solver.set_state(m, pos, vel)
+    e0 = sum(solver.energy(0.01))
+    solver.step(64, 1.0 / 128, 2, "fp64", 0.01)
+    e1 = sum(solver.energy(0.01))
+    assert abs((e1 - e0) / e0) < 1e-4
+
+
+def test_step_timing_counters(solver):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 4096, 2)
+    solver.set_state(m, pos, vel)
+    solver.timing(True)
+    solver.timing_reset()
+    solver.step(7, 1e-3, 4, "fp64", 0.01)     # order 4: 3 force evaluations per step
+    ms, evals, step_ms = solver.timing_read()
+    solver.timing(False)
+    assert evals == 21 and ms > 0 and step_ms >= ms
