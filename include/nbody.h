@@ -151,6 +151,19 @@ NB_API nb_status nb_timing_enable(nb_ctx* ctx, int enable);
 NB_API nb_status nb_timing_read(nb_ctx* ctx, double* force_ms, uint64_t* force_evals, double* step_ms);
 NB_API nb_status nb_timing_reset(nb_ctx* ctx);
 
+/* Force-kernel selection for the resident system (nb_accel, nb_step):
+ * NB_KERNEL_AUTO picks by size; NB_KERNEL_TILE evaluates every ordered pair
+ * (i-particles in registers, j-tiles through LDS); NB_KERNEL_SYMMETRIC
+ * evaluates each unordered pair once and applies it to both particles (the
+ * reference's reduced scheme, CPU/force_calculation.cpp:530-531). */
+typedef enum nb_kernel {
+    NB_KERNEL_AUTO = 0,
+    NB_KERNEL_TILE = 1,
+    NB_KERNEL_SYMMETRIC = 2
+} nb_kernel;
+
+NB_API nb_status nb_set_kernel(nb_ctx* ctx, int kernel);
+
 /* Wait for all work queued on the context's stream. */
 NB_API nb_status nb_synchronize(nb_ctx* ctx);
 

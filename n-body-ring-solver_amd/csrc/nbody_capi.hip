@@ -82,6 +82,13 @@ struct nb_ctx {
     double step_ms = 0.0;
     hipEvent_t step_ev[2] = {nullptr, nullptr};
     nbl::Learned* learned = nullptr;
+    int kernel = NB_KERNEL_AUTO;
+    // symmetric-kernel plan for the resident size
+    struct {
+        size_t n = 0;
+        int K = 0, nB = 0, L = 0, nitems = 0;
+        DevBuf items;
+    } sym;
 };
 
 namespace nbh {
@@ -132,17 +139,89 @@ static void timing_end(nb_ctx* c, long idx) {
     c->force_evals++;
 }
 
-// One force evaluation of the resident state; partial sums land in
-// part[0..S) (or directly in acc when S == 1).
-hipError_t force_eval(nb_ctx* c, int kform, double eps, P4*& part, int& S) {
+// Symmetric plan: blocks of S = 64*K, items of L tiles along each row.
+static hipError_t sym_plan(nb_ctx* c, int K) {
+    auto& P = c->sym;
+    if (P.n == c->n && P.K == K) return hipSuccess;
+    const long S = 64L * K;
+    const long nB = (long(c->n) + S - 1) / S;
+    const long tiles = nB * (nB + 1) / 2;
+    const long target = long(c->num_cus) * 16;   // items: ~16 per CU
+    const long L = std::max(1L, tiles / target);
+    std::vector<int4> items;
+    for (long A = 0; A < nB; A++)
+        for (long b0 = A; b0 < nB; b0 += L) items.push_back(make_int4(int(A), int(b0), int(std::min(nB, b0 + L)), 0));
+    // longest items first so the short row tails fill in at the end
+    std::stable_sort(items.begin(), items.end(), [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+    hipError_t e = P.items.ensure(items.size() * sizeof(int4));
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(P.items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    P.n = c->n;
+    P.K = K;
+    P.nB = int(nB);
+    P.L = int(L);
+    P.nitems = int(items.size());
+    return hipSuccess;
+}
+
+// Which kernel evaluates the resident system: the symmetric one from 32768
+// particles up (it does half the pair work) while its partial-sum buffer
+// stays within 16 GiB, the tile kernel otherwise.
+static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     const long n = long(c->n);
+    // 8 particles per lane for FP64 at large N; the mixed form keeps FP32 and
+    // FP64 sums per particle, so 8 would not fit two waves per SIMD.
+    K = (n >= 131072 && kform == nbk::FORM_FP64) ? 8 : 4;
+    const long S = 64L * K;
+    const long nB = (n + S - 1) / S;
+    const bool fits = double(nB) * nB * S * sizeof(P4) <= 16.0 * (1L << 30);
+    if (c->kernel == NB_KERNEL_TILE) return false;
+    if (c->kernel == NB_KERNEL_SYMMETRIC) return fits && n >= 2;
+    return fits && n >= 32768;
+}
+
+// One force evaluation of the resident state; `out` says where the partial
+// sums are and how to add them.
+hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
+    const long n = long(c->n);
+    int K;
+    if (use_symmetric(c, kform, K)) {
+        hipError_t e = sym_plan(c, K);
+        if (e != hipSuccess) return e;
+        const auto& P = c->sym;
+        e = c->part.ensure(size_t(P.nB) * P.nB * 64 * K * sizeof(P4));
+        if (e != hipSuccess) return e;
+        nbk::SymLaunch L;
+        L.form = kform;
+        L.K = K;
+        L.pm = (const P4*)c->pm.p;
+        L.n = int(n);
+        L.items = (const int4*)P.items.p;
+        L.nitems = P.nitems;
+        L.nB = P.nB;
+        L.eps2 = eps * eps;
+        L.part = (P4*)c->part.p;
+        out.sym = true;
+        out.part = L.part;
+        out.nB = P.nB;
+        out.S = 64 * K;
+        out.L = P.L;
+        const long ev = timing_begin(c);
+        e = nbk::launch_accel_sym(L, c->stream);
+        timing_end(c, ev);
+        return e;
+    }
     nbk::AccelLaunch L;
     L.form = kform;
     L.ipt = choose_ipt(c, n);
-    S = choose_splits(c, n, n, L.ipt);
+    int S = choose_splits(c, n, n, L.ipt);
     long jchunk = (n + S - 1) / S;
     jchunk = (jchunk + 63) & ~63L;
     S = int((n + jchunk - 1) / jchunk);
+    P4* part;
     if (S <= 1) {
         S = 1;
         jchunk = n;
@@ -163,10 +242,31 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, P4*& part, int& S) {
     L.G = 1.0;
     L.out = part;
     L.out_stride = n;
+    out.sym = false;
+    out.part = part;
+    out.S = S;
     const long ev = timing_begin(c);
     hipError_t e = nbk::launch_accel(L, c->stream);
     timing_end(c, ev);
     return e;
+}
+
+// acc = sum of the partials of `out`.
+hipError_t finish_accel(nb_ctx* c, const ForceOut& out) {
+    const int n = int(c->n);
+    if (out.sym) return nbk::launch_sym_reduce(out.part, out.nB, out.S, out.L, n, (P4*)c->acc.p, c->stream);
+    if (out.S == 1) return hipSuccess;
+    return nbk::launch_reduce(out.part, out.S, long(n), n, (P4*)c->acc.p, c->stream);
+}
+
+// acc = sum of the partials, then v += kdt a; x += c1dt v; x += c2dt v.
+hipError_t finish_kick_drift(nb_ctx* c, const ForceOut& out, double kdt, double c1dt, double c2dt) {
+    const int n = int(c->n);
+    P4* pm = (P4*)c->pm.p;
+    P4* vel = (P4*)c->vel.p;
+    P4* acc = (P4*)c->acc.p;
+    if (out.sym) return nbk::launch_sym_kick_drift(out.part, out.nB, out.S, out.L, n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
+    return nbk::launch_kick_drift(out.part, out.S, long(n), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
 }
 
 int kernel_form(int form) {
@@ -179,11 +279,10 @@ int kernel_form(int form) {
 }
 
 hipError_t accel_to_acc(nb_ctx* c, int kform, double eps) {
-    P4* part;
-    int S;
-    hipError_t e = force_eval(c, kform, eps, part, S);
-    if (e != hipSuccess || S == 1) return e;
-    return nbk::launch_reduce(part, S, long(c->n), int(c->n), (P4*)c->acc.p, c->stream);
+    ForceOut out;
+    hipError_t e = force_eval(c, kform, eps, out);
+    if (e != hipSuccess) return e;
+    return finish_accel(c, out);
 }
 
 hipStream_t ctx_stream(nb_ctx* c) { return c->stream; }
@@ -238,7 +337,7 @@ nb_status nb_destroy(nb_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     nbl::destroy(c->learned);
     c->learned = nullptr;
-    for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout})
+    for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->sym.items})
         b->release();
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->step_ev)
@@ -508,9 +607,8 @@ nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, con
     NB_HIP(nbk::launch_drift(n, pm, vel, cc[0] * dt, c->stream), "drift kernel");
     for (size_t t = 0; t < time_steps; t++) {
         for (size_t s = 0; s + 1 < lf_steps; s++) {
-            P4* part;
-            int S;
-            NB_HIP(force_eval(c, kf, eps, part, S), "force kernel");
+            ForceOut out;
+            NB_HIP(force_eval(c, kf, eps, out), "force kernel");
             double c1, c2 = 0.0;
             if (s + 2 < lf_steps) {
                 c1 = cc[s + 1] * dt;
@@ -518,8 +616,7 @@ nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, con
                 c1 = cc[lf_steps - 1] * dt;
                 if (t + 1 < time_steps) c2 = cc[0] * dt;
             }
-            NB_HIP(nbk::launch_kick_drift(part, S, long(n), n, pm, vel, acc, dd[s] * dt, c1, c2, c->stream),
-                   "kick/drift kernel");
+            NB_HIP(finish_kick_drift(c, out, dd[s] * dt, c1, c2), "kick/drift kernel");
         }
     }
     if (c->timing) {
@@ -600,6 +697,14 @@ nb_status nb_timing_reset(nb_ctx* c) {
     c->force_ms = 0.0;
     c->force_evals = 0;
     c->step_ms = 0.0;
+    return NB_OK;
+}
+
+nb_status nb_set_kernel(nb_ctx* c, int kernel) {
+    if (!c) return fail(NB_ERR_ARG, "nb_set_kernel: null context");
+    if (kernel != NB_KERNEL_AUTO && kernel != NB_KERNEL_TILE && kernel != NB_KERNEL_SYMMETRIC)
+        return fail(NB_ERR_ARG, "nb_set_kernel: unknown kernel");
+    c->kernel = kernel;
     return NB_OK;
 }
 

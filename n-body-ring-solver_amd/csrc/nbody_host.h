@@ -32,10 +32,20 @@ struct DevBuf {
     void release();
 };
 
+// Where one force evaluation left its partial sums.
+struct ForceOut {
+    bool sym = false;
+    nbk::P4* part = nullptr;
+    int S = 1;          // tile kernel: number of j-splits; symmetric: block size
+    int nB = 0, L = 1;  // symmetric: blocks, tiles per work item
+};
+
 // Context internals used by the learned-form module.
 nb_status set_error(nb_status s, const std::string& msg);
 int kernel_form(int form);
-hipError_t force_eval(nb_ctx* c, int kform, double eps, nbk::P4*& part, int& S);
+hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out);
+hipError_t finish_accel(nb_ctx* c, const ForceOut& out);
+hipError_t finish_kick_drift(nb_ctx* c, const ForceOut& out, double kdt, double c1dt, double c2dt);
 hipStream_t ctx_stream(nb_ctx* c);
 size_t ctx_n(nb_ctx* c);
 nbk::P4* ctx_pm(nb_ctx* c);

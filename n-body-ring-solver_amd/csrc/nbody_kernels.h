@@ -32,6 +32,26 @@ struct AccelLaunch {
 };
 
 hipError_t launch_accel(const AccelLaunch& L, hipStream_t st);
+
+// Symmetric kernel (nbody_sym.hip): blocks of S = 64*K particles, work items
+// (A, b0, b1) = one wave sweeping j-blocks [b0, b1) against i-block A;
+// part holds nB*nB*S records (slot layout described in nbody_sym.hip).
+struct SymLaunch {
+    int form = FORM_FP64;
+    int K = 8;
+    const P4* pm = nullptr;
+    int n = 0;
+    const int4* items = nullptr;
+    int nitems = 0;
+    int nB = 0;
+    double eps2 = 0.0;
+    P4* part = nullptr;
+};
+
+hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st);
+hipError_t launch_sym_reduce(const P4* part, int nB, int S, int L, int n, P4* acc, hipStream_t st);
+hipError_t launch_sym_kick_drift(const P4* part, int nB, int S, int L, int n, P4* pm, P4* vel, P4* acc,
+                                 double kdt, double c1dt, double c2dt, hipStream_t st);
 hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st);
 hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
                              double kdt, double c1dt, double c2dt, hipStream_t st);

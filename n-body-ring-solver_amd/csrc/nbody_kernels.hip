@@ -41,18 +41,20 @@ __device__ __forceinline__ P4 load_p4(const P4* __restrict__ p, int j, int jend)
 
 // FP64: v_rsq_f64 gives y ~ r2^(-1/2) to about 2^-22 relative; with
 // e = 1 - r2 y^2 the exact y_true^3 = y^3 (1-e)^(-3/2) = y^3 (1 + 3/2 e + 15/8 e^2 + O(e^3)),
-// so two FMAs restore full FP64 accuracy (|e|^3 ~ 2^-66).
+// so two FMAs restore full FP64 accuracy (|e|^3 ~ 2^-66).  The constants
+// c15 = 1.5 and c1875 = 1.875 arrive as kernel arguments so they stay in
+// registers (as literals the compiler rebuilds them with two moves per pair).
 template <bool EPS0>
 __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, double xj, double yj,
-                                          double zj, double mj, double eps2, double& ax, double& ay,
-                                          double& az) {
+                                          double zj, double mj, double eps2, double c15, double c1875,
+                                          double& ax, double& ay, double& az) {
     const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
     const double y = __builtin_amdgcn_rsq(r2);
     const double t = y * y;
     const double e = __builtin_fma(-r2, t, 1.0);
     const double y3 = t * y;
-    const double p = __builtin_fma(e, __builtin_fma(e, 1.875, 1.5), 1.0);
+    const double p = __builtin_fma(e, __builtin_fma(e, c1875, c15), 1.0);
     double s = (mj * y3) * p;
     if (EPS0) s = (r2 > 0.0) ? s : 0.0;   // zero separation (self pair) contributes 0
     ax = __builtin_fma(s, dx, ax);
@@ -88,8 +90,8 @@ template <int FORM, bool EPS0, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int ni,
                                                  const P4* __restrict__ pj, int jbeg_all,
                                                  int jend_all, int jchunk, double eps2, double G,
-                                                 P4* __restrict__ out, long out_stride,
-                                                 int split_base) {
+                                                 double c15, double c1875, P4* __restrict__ out,
+                                                 long out_stride, int split_base) {
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR);
     using Rec = typename std::conditional<MIXED, P4m, P4>::type;
@@ -155,8 +157,8 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
                     const P4 q = tl[jj + u];
 #pragma unroll
                     for (int k = 0; k < IPT; k++)
-                        pair_fp64<EPS0>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2, ax[k], ay[k],
-                                        az[k]);
+                        pair_fp64<EPS0>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2, c15, c1875,
+                                        ax[k], ay[k], az[k]);
                 }
             }
         }
@@ -344,7 +346,8 @@ template <int FORM, bool EPS0, int IPT>
 static hipError_t launch_accel_t(const AccelLaunch& L, hipStream_t st) {
     const dim3 grid((L.ni + BLOCK * IPT - 1) / (BLOCK * IPT), L.splits);
     hipLaunchKernelGGL((k_accel<FORM, EPS0, IPT>), grid, dim3(BLOCK), 0, st, L.pi, L.ni, L.pj,
-                       L.jbeg, L.jend, L.jchunk, L.eps2, L.G, L.out, L.out_stride, L.split_base);
+                       L.jbeg, L.jend, L.jchunk, L.eps2, L.G, 1.5, 1.875, L.out, L.out_stride,
+                       L.split_base);
     return hipGetLastError();
 }
 

@@ -1,0 +1,320 @@
+// nbody_sym.hip -- symmetric (Newton's third law) force kernel for gfx950.
+//
+// Every unordered pair is evaluated once and its force applied to both
+// particles, the GPU analogue of the reference's reduced solver
+// (CPU/force_calculation.cpp:530-531, 617-630) and of the FPGA PE_2x2
+// (FPGA/device/ring_lrbd_lf_no_sync_local.cl:161-184).
+//
+// Blocks of S = 64*K particles.  A work item (A, b0, b1) is one wave64 that
+// holds the K particles of block A it owns per lane in registers (lane l owns
+// A*S + k*64 + l) with their action accumulators, and sweeps the j-blocks
+// B in [b0, b1), B >= A.  Each j-block is taken in K chunks of 64: the chunk
+// is staged (twice, so indices never wrap) in the wave's LDS slot, and over 64
+// steps lane l pairs its K particles with chunk particle (l + t) mod 64 read
+// from LDS while the chunk's reaction accumulators travel with their particle
+// through the lanes: after every step they rotate by one lane (DPP wave_rol:1,
+// lane l receives lane l+1), so after 64 steps each sits in its home lane
+// again.  Diagonal tiles (B == A) mask pairs with i >= j.
+//
+// Outputs are deterministic partial sums, no atomics: reactions of block B
+// from block A go to part[B][A] (slot A < B), the actions of item (A, b0, ..)
+// to part[A][b0] (slot >= A), diagonal reactions fold into the actions.
+// k_sym_kick_drift / k_sym_reduce add a block's slots in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "nbody_kernels.h"
+
+namespace nbk {
+
+namespace {
+
+constexpr int kWavesPerGroup = 4;
+
+// Lane l receives the value held by lane l+1 (mod 64): DPP wave_rol:1, every
+// lane enabled, so no lane keeps an old value.
+constexpr int kDppWaveRol1 = 0x134;
+
+__device__ __forceinline__ double rotate_down(double v) {
+    const int2 b = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_mov_dpp(b.x, kDppWaveRol1, 0xF, 0xF, false);
+    r.y = __builtin_amdgcn_mov_dpp(b.y, kDppWaveRol1, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, r);
+}
+
+__device__ __forceinline__ float rotate_down(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), kDppWaveRol1, 0xF, 0xF,
+                                                              false));
+}
+
+// One unordered pair, FP64.  d = x_j - x_i; w = 1/(|d|^2+eps^2)^(3/2) from
+// v_rsq_f64 and the cubic correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2;
+// a_i += m_j w d, b_j -= m_i w d.  `keep` zeroes masked pairs.
+template <bool EPS0, bool MASK>
+__device__ __forceinline__ void sym64(double xi, double yi, double zi, double mi, double xj, double yj,
+                                      double zj, double mj, double eps2, double c15, double c1875, bool keep,
+                                      double& ax, double& ay, double& az, double& bx, double& by,
+                                      double& bz) {
+    const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double t = y * y;
+    const double e = __builtin_fma(-r2, t, 1.0);
+    const double p = __builtin_fma(e, __builtin_fma(e, c1875, c15), 1.0);
+    double w = (t * y) * p;
+    if (EPS0) w = (r2 > 0.0) ? w : 0.0;
+    if (MASK) w = keep ? w : 0.0;
+    const double si = mj * w, sj = mi * w;
+    ax = __builtin_fma(si, dx, ax);
+    ay = __builtin_fma(si, dy, ay);
+    az = __builtin_fma(si, dz, az);
+    bx = __builtin_fma(-sj, dx, bx);
+    by = __builtin_fma(-sj, dy, by);
+    bz = __builtin_fma(-sj, dz, bz);
+}
+
+// Mixed: FP64 difference rounded to FP32, FP32 rsqrt (+ one Newton step when
+// NR) and pair arithmetic, FP32 sums (flushed to FP64 per 64-particle chunk).
+template <bool EPS0, bool MASK, bool NR>
+__device__ __forceinline__ void sym32(double xi, double yi, double zi, float mi, double xj, double yj,
+                                      double zj, float mj, float eps2, bool keep, float& ax, float& ay,
+                                      float& az, float& bx, float& by, float& bz) {
+    const float dx = (float)(xj - xi), dy = (float)(yj - yi), dz = (float)(zj - zi);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    float y = __builtin_amdgcn_rsqf(r2);
+    if (NR) y = y * __builtin_fmaf(-0.5f * r2, y * y, 1.5f);
+    float w = (y * y) * y;
+    if (EPS0) w = (r2 > 0.0f) ? w : 0.0f;
+    if (MASK) w = keep ? w : 0.0f;
+    const float si = mj * w, sj = mi * w;
+    ax = __builtin_fmaf(si, dx, ax);
+    ay = __builtin_fmaf(si, dy, ay);
+    az = __builtin_fmaf(si, dz, az);
+    bx = __builtin_fmaf(-sj, dx, bx);
+    by = __builtin_fmaf(-sj, dy, by);
+    bz = __builtin_fmaf(-sj, dz, bz);
+}
+
+// Order this lane's LDS accesses against the other lanes' of the same wave
+// (LDS instructions of one wave execute in order; the fences stop the compiler
+// from moving loads across the point where the chunk is (re)written).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ P4 load_or_zero(const P4* __restrict__ p, int j, int n) {
+    return (j < n) ? p[j] : P4{0.0, 0.0, 0.0, 0.0};
+}
+
+}  // namespace
+
+template <int FORM, bool EPS0, int K>
+__global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __restrict__ pm, int n,
+                                                                   const int4* __restrict__ items, int nitems,
+                                                                   int nB, double eps2, double c15,
+                                                                   double c1875, P4* __restrict__ part) {
+    constexpr bool MIXED = (FORM != FORM_FP64);
+    constexpr bool NR = (FORM == FORM_MIXED_NR);
+    constexpr int S = 64 * K;
+    __shared__ P4 jt[kWavesPerGroup][128];   // each wave's chunk, stored twice
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int item = blockIdx.x * kWavesPerGroup + wave;
+    if (item >= nitems) return;   // the whole wave leaves; no workgroup barrier is used below
+    const int4 it = items[item];
+    const int A = it.x, b0 = it.y, b1 = it.z;
+    P4* __restrict__ lds = jt[wave];
+
+    double xi[K], yi[K], zi[K], mi[K], ax[K], ay[K], az[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const P4 p = load_or_zero(pm, A * S + k * 64 + lane, n);
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.m;
+        ax[k] = ay[k] = az[k] = 0.0;
+    }
+    const float eps2f = (float)eps2;
+    float mif[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) mif[k] = (float)mi[k];
+
+    for (int B = b0; B < b1; B++) {
+        const bool diag = (B == A);
+        for (int c = 0; c < K; c++) {
+            const P4 q = load_or_zero(pm, B * S + c * 64 + lane, n);
+            lds[lane] = q;
+            lds[lane + 64] = q;
+            wave_sync();
+            double bx, by, bz;
+            if constexpr (!MIXED) {
+                bx = by = bz = 0.0;
+                if (!diag) {
+#pragma unroll 2
+                    for (int t = 0; t < 64; t++) {
+                        const P4 J = lds[lane + t];
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+                            sym64<EPS0, false>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, c15, c1875,
+                                               true, ax[k], ay[k], az[k], bx, by, bz);
+                        bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
+                    }
+                } else {
+                    for (int t = 0; t < 64; t++) {
+                        const P4 J = lds[lane + t];
+                        const bool same_ok = (t >= 1) && (lane + t < 64);   // i = c*64+lane < j = c*64+lane+t
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+                            sym64<EPS0, true>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, c15, c1875,
+                                              (k < c) || (k == c && same_ok), ax[k], ay[k], az[k], bx, by, bz);
+                        bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
+                    }
+                }
+            } else {
+                float fbx = 0.f, fby = 0.f, fbz = 0.f;
+                float fax[K], fay[K], faz[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) fax[k] = fay[k] = faz[k] = 0.f;
+                if (!diag) {
+#pragma unroll 2
+                    for (int t = 0; t < 64; t++) {
+                        const P4 J = lds[lane + t];
+                        const float mj = (float)J.m;
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+                            sym32<EPS0, false, NR>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f, true,
+                                                   fax[k], fay[k], faz[k], fbx, fby, fbz);
+                        fbx = rotate_down(fbx); fby = rotate_down(fby); fbz = rotate_down(fbz);
+                    }
+                } else {
+                    for (int t = 0; t < 64; t++) {
+                        const P4 J = lds[lane + t];
+                        const float mj = (float)J.m;
+                        const bool same_ok = (t >= 1) && (lane + t < 64);
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+                            sym32<EPS0, true, NR>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f,
+                                                  (k < c) || (k == c && same_ok), fax[k], fay[k], faz[k], fbx,
+                                                  fby, fbz);
+                        fbx = rotate_down(fbx); fby = rotate_down(fby); fbz = rotate_down(fbz);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    ax[k] += (double)fax[k];
+                    ay[k] += (double)fay[k];
+                    az[k] += (double)faz[k];
+                }
+                bx = (double)fbx; by = (double)fby; bz = (double)fbz;
+            }
+            if (diag) {
+                // the chunk is this wave's own particles k == c: fold the reactions in
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    if (k == c) {
+                        ax[k] += bx; ay[k] += by; az[k] += bz;
+                    }
+            } else {
+                part[((long)B * nB + A) * S + c * 64 + lane] = P4{bx, by, bz, 0.0};
+            }
+            wave_sync();
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) part[((long)A * nB + b0) * S + k * 64 + lane] = P4{ax[k], ay[k], az[k], 0.0};
+}
+
+// a_i for i in block X = sum of reaction slots Y < X, then action slots
+// b0 = X, X+L, ... < nB, in that order.
+__device__ __forceinline__ void sym_sum(const P4* __restrict__ part, int nB, int S, int L, int i, double& ax,
+                                        double& ay, double& az) {
+    const int X = i / S, r = i - X * S;
+    ax = ay = az = 0.0;
+    const P4* row = part + (long)X * nB * S + r;
+    for (int Y = 0; Y < X; Y++) {
+        const P4 p = row[(long)Y * S];
+        ax += p.x; ay += p.y; az += p.z;
+    }
+    for (int Y = X; Y < nB; Y += L) {
+        const P4 p = row[(long)Y * S];
+        ax += p.x; ay += p.y; az += p.z;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sym_reduce(const P4* __restrict__ part, int nB, int S, int L, int n,
+                                                      P4* __restrict__ acc) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double ax, ay, az;
+    sym_sum(part, nB, S, L, i, ax, ay, az);
+    acc[i] = P4{ax, ay, az, 0.0};
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sym_kick_drift(const P4* __restrict__ part, int nB, int S, int L, int n,
+                                                          P4* __restrict__ pm, P4* __restrict__ vel,
+                                                          P4* __restrict__ acc, double kdt, double c1dt,
+                                                          double c2dt) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double ax, ay, az;
+    sym_sum(part, nB, S, L, i, ax, ay, az);
+    acc[i] = P4{ax, ay, az, 0.0};
+    P4 v = vel[i];
+    v.x = __builtin_fma(kdt, ax, v.x);
+    v.y = __builtin_fma(kdt, ay, v.y);
+    v.z = __builtin_fma(kdt, az, v.z);
+    vel[i] = v;
+    P4 p = pm[i];
+    p.x = __builtin_fma(c1dt, v.x, p.x);
+    p.y = __builtin_fma(c1dt, v.y, p.y);
+    p.z = __builtin_fma(c1dt, v.z, p.z);
+    p.x = __builtin_fma(c2dt, v.x, p.x);
+    p.y = __builtin_fma(c2dt, v.y, p.y);
+    p.z = __builtin_fma(c2dt, v.z, p.z);
+    pm[i] = p;
+}
+
+template <int FORM, bool EPS0, int K>
+static hipError_t launch_sym_t(const SymLaunch& L, hipStream_t st) {
+    const int groups = (L.nitems + kWavesPerGroup - 1) / kWavesPerGroup;
+    hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm, L.n,
+                       L.items, L.nitems, L.nB, L.eps2, 1.5, 1.875, L.part);
+    return hipGetLastError();
+}
+
+template <int FORM, bool EPS0>
+static hipError_t launch_sym_k(const SymLaunch& L, hipStream_t st) {
+    return (L.K == 8) ? launch_sym_t<FORM, EPS0, 8>(L, st) : launch_sym_t<FORM, EPS0, 4>(L, st);
+}
+
+template <int FORM>
+static hipError_t launch_sym_eps(const SymLaunch& L, hipStream_t st) {
+    return (L.eps2 == 0.0) ? launch_sym_k<FORM, true>(L, st) : launch_sym_k<FORM, false>(L, st);
+}
+
+hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st) {
+    if (L.nitems <= 0) return hipSuccess;
+    switch (L.form) {
+    case FORM_FP64: return launch_sym_eps<FORM_FP64>(L, st);
+    case FORM_MIXED: return launch_sym_eps<FORM_MIXED>(L, st);
+    case FORM_MIXED_NR: return launch_sym_eps<FORM_MIXED_NR>(L, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sym_reduce(const P4* part, int nB, int S, int L, int n, P4* acc, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sym_reduce, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, nB, S, L, n, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_kick_drift(const P4* part, int nB, int S, int L, int n, P4* pm, P4* vel, P4* acc,
+                                 double kdt, double c1dt, double c2dt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sym_kick_drift, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, nB, S, L, n, pm,
+                       vel, acc, kdt, c1dt, c2dt);
+    return hipGetLastError();
+}
+
+}  // namespace nbk

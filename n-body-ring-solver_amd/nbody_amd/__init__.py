@@ -37,7 +37,9 @@ EXPORTS = (
     "nb_timing_read", "nb_timing_reset", "nb_synchronize", "nb_learned_configure",
     "nb_learned_set_mode", "nb_learned_collect", "nb_learned_train", "nb_learned_stats_read",
     "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
+    "nb_set_kernel",
 )
+KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
 _D = C.POINTER(C.c_double)
 _F = C.POINTER(C.c_float)
@@ -87,6 +89,7 @@ def load_library(path: str = LIB_PATH):
     L.nb_timing_read.argtypes = [_VP, _D, C.POINTER(C.c_uint64), _D]
     L.nb_timing_reset.argtypes = [_VP]
     L.nb_synchronize.argtypes = [_VP]
+    L.nb_set_kernel.argtypes = [_VP, C.c_int]
     L.nb_learned_configure.argtypes = [_VP, C.c_int, C.c_double, C.c_int]
     L.nb_learned_set_mode.argtypes = [_VP, C.c_int]
     L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
@@ -251,6 +254,10 @@ class Solver:
 
     def timing_reset(self):
         _check(self.L.nb_timing_reset(self.h), "nb_timing_reset")
+
+    def set_kernel(self, kernel: str):
+        """Force kernel for nb_accel / nb_step: "auto", "tile" or "symmetric"."""
+        _check(self.L.nb_set_kernel(self.h, KERNELS[kernel]), "nb_set_kernel")
 
     def synchronize(self):
         _check(self.L.nb_synchronize(self.h), "nb_synchronize")

@@ -3,13 +3,158 @@
  *
  * TEST INFRASTRUCTURE ONLY (see nbody_oracle.c).  No reference counterpart:
  * the reference has no learned component.  Pinned by the tests against
- * closed forms (polynomial extrapolation is exact on cubics) and against
- * central finite differences of the FP64 loss (gradients).
+ * closed forms (the cubic extrapolation is exact on cubics; the features are
+ * invariant under scaling), and the backward pass against central finite
+ * differences of the FP64 loss.
+ *
+ * Network: 15 -> 128 -> 128 -> 128 -> 3, SiLU hidden activations, linear
+ * output.  Parameters are packed W1 b1 W2 b2 W3 b3 W4 b4 with W row-major
+ * [out][in].  With `bf16` set, every operand of a matrix product (weights,
+ * inputs, hidden activations, back-propagated deltas) is first rounded to
+ * bfloat16 (round to nearest even), which is what the GPU's MFMA kernels
+ * consume; sums stay in FP64.
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
 
 #define ORC_API __attribute__((visibility("default")))
+
+enum { NIN = 15, NH = 128, NOUT = 3 };
+
+static const size_t layer_in[4] = {NIN, NH, NH, NH};
+static const size_t layer_out[4] = {NH, NH, NH, NOUT};
+
+ORC_API size_t orc_mlp_param_count(void)
+{
+    size_t n = 0;
+    for (int l = 0; l < 4; l++) n += layer_out[l] * layer_in[l] + layer_out[l];
+    return n;
+}
+
+/* float -> bfloat16 -> float, round to nearest even (NaN kept quiet). */
+static double bf16r(double v)
+{
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) {
+        if (u & 0x007fffffu) u |= 0x00400000u;
+        u &= 0xffff0000u;
+    } else {
+        u += 0x7fffu + ((u >> 16) & 1u);
+        u &= 0xffff0000u;
+    }
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+ORC_API double orc_bf16_round(double v) { return bf16r(v); }
+
+static double silu(double z) { return z / (1.0 + exp(-z)); }
+
+static double silu_d(double z)
+{
+    double s = 1.0 / (1.0 + exp(-z));
+    return s * (1.0 + z * (1.0 - s));
+}
+
+static const double *layer_w(const double *p, int l)
+{
+    for (int k = 0; k < l; k++) p += layer_out[k] * layer_in[k] + layer_out[k];
+    return p;
+}
+
+/* Forward pass for one row; keeps pre-activations z[l][] and inputs h[l][]. */
+static void fwd_row(const double *p, const double *x, int bf16, double h[4][NH], double z[4][NH],
+                    double *y)
+{
+    for (size_t k = 0; k < NIN; k++) h[0][k] = bf16 ? bf16r(x[k]) : x[k];
+    for (int l = 0; l < 4; l++) {
+        const double *W = layer_w(p, l);
+        const double *b = W + layer_out[l] * layer_in[l];
+        for (size_t o = 0; o < layer_out[l]; o++) {
+            double s = 0.0;
+            for (size_t i = 0; i < layer_in[l]; i++) {
+                double w = bf16 ? bf16r(W[o * layer_in[l] + i]) : W[o * layer_in[l] + i];
+                s += w * h[l][i];
+            }
+            s += b[o];
+            z[l][o] = s;
+            if (l < 3) {
+                double a = silu(s);
+                h[l + 1][o] = bf16 ? bf16r(a) : a;
+            } else {
+                y[o] = s;
+            }
+        }
+    }
+}
+
+ORC_API void orc_mlp_forward(const double *params, size_t rows, const double *x, int bf16, double *y)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t r = 0; r < rows; r++) {
+        double h[4][NH], z[4][NH];
+        fwd_row(params, x + r * NIN, bf16, h, z, y + r * NOUT);
+    }
+}
+
+/* Loss L = sum_rows sum_k (y_k - t_k)^2 / (2 rows) and its gradient with
+ * respect to every parameter (packed like the parameters).  Returns L. */
+ORC_API double orc_mlp_grad(const double *params, size_t rows, const double *x, const double *t,
+                            int bf16, double *grad)
+{
+    const size_t np = orc_mlp_param_count();
+    memset(grad, 0, np * sizeof(double));
+    double loss = 0.0;
+    for (size_t r = 0; r < rows; r++) {
+        double h[4][NH], z[4][NH], y[NOUT];
+        fwd_row(params, x + r * NIN, bf16, h, z, y);
+        double delta[NH], prev[NH];
+        for (size_t k = 0; k < NOUT; k++) {
+            double e = y[k] - t[r * NOUT + k];
+            loss += 0.5 * e * e / (double)rows;
+            delta[k] = e / (double)rows;
+        }
+        for (int l = 3; l >= 0; l--) {
+            const double *W = layer_w(params, l);
+            double *gW = grad + (layer_w(params, l) - params);
+            double *gb = gW + layer_out[l] * layer_in[l];
+            double dl[NH];
+            for (size_t o = 0; o < layer_out[l]; o++) dl[o] = bf16 ? bf16r(delta[o]) : delta[o];
+            for (size_t o = 0; o < layer_out[l]; o++) {
+                gb[o] += delta[o];
+                for (size_t i = 0; i < layer_in[l]; i++) gW[o * layer_in[l] + i] += dl[o] * h[l][i];
+            }
+            if (l == 0) break;
+            for (size_t i = 0; i < layer_in[l]; i++) {
+                double s = 0.0;
+                for (size_t o = 0; o < layer_out[l]; o++) {
+                    double w = bf16 ? bf16r(W[o * layer_in[l] + i]) : W[o * layer_in[l] + i];
+                    s += w * dl[o];
+                }
+                prev[i] = s * silu_d(z[l - 1][i]);
+            }
+            memcpy(delta, prev, layer_in[l] * sizeof(double));
+        }
+    }
+    return loss;
+}
+
+/* One Adam step (Kingma & Ba 2015) with bias correction, step count t >= 1. */
+ORC_API void orc_adam(size_t n, double *p, const double *g, double *m, double *v, double lr,
+                      double b1, double b2, double eps, int t)
+{
+    double c1 = 1.0 - pow(b1, t), c2 = 1.0 - pow(b2, t);
+    for (size_t i = 0; i < n; i++) {
+        m[i] = b1 * m[i] + (1.0 - b1) * g[i];
+        v[i] = b2 * v[i] + (1.0 - b2) * g[i] * g[i];
+        p[i] -= lr * (m[i] / c1) / (sqrt(v[i] / c2) + eps);
+    }
+}
 
 /* Cubic extrapolation from the last four equally spaced samples:
  * a_{n+1} = 4 a_n - 6 a_{n-1} + 4 a_{n-2} - a_{n-3} (exact for cubics). */
@@ -17,4 +162,45 @@ ORC_API void orc_poly_extrap(size_t n3, const double *a0, const double *a1, cons
                              const double *a3, double *out)
 {
     for (size_t i = 0; i < n3; i++) out[i] = 4.0 * a0[i] - 6.0 * a1[i] + 4.0 * a2[i] - a3[i];
+}
+
+/* Extrapolation features of particle i from its last four accelerations
+ * a0 = a_n, a1 = a_{n-1}, a2, a3 (each 3 doubles) and velocity v:
+ *   D1 = a0 - a1, D2 = a0 - 2a1 + a2, D3 = a0 - 3a1 + 3a2 - a3 (backward
+ *   differences; the cubic extrapolation is a0 + D1 + D2 + D3),
+ *   S = |D1| + |D2| + |D3|,
+ *   x = (a0/|a0|, D1/S, D2/S, D3/S, v/|v|)   (15 numbers, each <= 1),
+ * and the output scale |D3|: the network predicts the extrapolation residual
+ * (a_{n+1} - cubic) / |D3|.  Zero norms give zero features. */
+ORC_API void orc_features(size_t n, const double *a0, const double *a1, const double *a2,
+                          const double *a3, const double *v, double *x, double *scale,
+                          double *cubic)
+{
+    for (size_t i = 0; i < n; i++) {
+        double d1[3], d2[3], d3[3], c[3];
+        for (int k = 0; k < 3; k++) {
+            double p0 = a0[3 * i + k], p1 = a1[3 * i + k], p2 = a2[3 * i + k], p3 = a3[3 * i + k];
+            d1[k] = p0 - p1;
+            d2[k] = p0 - 2.0 * p1 + p2;
+            d3[k] = p0 - 3.0 * p1 + 3.0 * p2 - p3;
+            c[k] = p0 + d1[k] + d2[k] + d3[k];
+        }
+        double n0 = sqrt(a0[3 * i] * a0[3 * i] + a0[3 * i + 1] * a0[3 * i + 1] + a0[3 * i + 2] * a0[3 * i + 2]);
+        double n1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
+        double n2 = sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2]);
+        double n3 = sqrt(d3[0] * d3[0] + d3[1] * d3[1] + d3[2] * d3[2]);
+        double nv = sqrt(v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+        double S = n1 + n2 + n3;
+        double i0 = n0 > 0 ? 1.0 / n0 : 0.0, iS = S > 0 ? 1.0 / S : 0.0, iv = nv > 0 ? 1.0 / nv : 0.0;
+        double *xr = x + 15 * i;
+        for (int k = 0; k < 3; k++) {
+            xr[k] = a0[3 * i + k] * i0;
+            xr[3 + k] = d1[k] * iS;
+            xr[6 + k] = d2[k] * iS;
+            xr[9 + k] = d3[k] * iS;
+            xr[12 + k] = v[3 * i + k] * iv;
+            cubic[3 * i + k] = c[k];
+        }
+        scale[i] = n3;
+    }
 }

@@ -163,3 +163,54 @@ def test_argument_errors_on_device(solver):
         solver.accel(9, 0.0)
     with pytest.raises(nbody_amd.NBodyError):
         solver.step(1, 1e-3, coeffs=(np.array([0.5]), np.array([]), 1))
+
+
+# --- both force kernels on the resident system ---------------------------------
+
+@pytest.mark.parametrize("kernel", ["tile", "symmetric"])
+@pytest.mark.parametrize("n", [2, 3, 100, 257, 513, 4097, 20000])
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_kernels_vs_oracle(solver, orc, kernel, n, eps):
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 31)
+    m = m * np.linspace(0.3, 3.0, n)   # unequal masses: a swapped m_i / m_j shows up
+    solver.set_kernel(kernel)
+    try:
+        solver.set_state(m, pos)
+        ref = orc.accel(m, pos, eps)
+        assert rel_err(solver.accel("fp64", eps), ref) < FP64_TOL
+        assert rel_err(solver.accel("mixed", eps), ref) < MIXED_TOL
+        assert rel_err(solver.accel("mixed-raw", eps), ref) < MIXED_TOL
+    finally:
+        solver.set_kernel("auto")
+
+
+@pytest.mark.parametrize("n", [40000, 131072 + 77])
+def test_symmetric_matches_tile_large(solver, n):
+    # K = 4 and K = 8 blocks with a ragged last block; both kernels agree to the FP64 bar
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 8)
+    solver.set_state(m, pos)
+    solver.set_kernel("tile")
+    a_tile = solver.accel("fp64", 0.005)
+    solver.set_kernel("symmetric")
+    a_sym = solver.accel("fp64", 0.005)
+    a_sym2 = solver.accel("fp64", 0.005)
+    solver.set_kernel("auto")
+    assert rel_err(a_sym, a_tile) < FP64_TOL
+    assert np.array_equal(a_sym, a_sym2)
+
+
+def test_symmetric_coincident_eps0(solver, orc):
+    # zero-separation pairs inside and across blocks must not produce NaN at eps = 0
+    m, pos, _, _ = nbody_amd.init_system("sphere", 1500, 8)
+    pos[:, 700:720] = pos[:, 10:30]      # same block for K = 4 (256) and a far block
+    pos[:, 1400:1402] = pos[:, 1398:1400]
+    pos = np.ascontiguousarray(pos)
+    solver.set_kernel("symmetric")
+    try:
+        solver.set_state(m, pos)
+        for form in ("fp64", "mixed"):
+            a = solver.accel(form, 0.0)
+            assert np.all(np.isfinite(a)), form
+        assert rel_err(solver.accel("fp64", 0.0), orc.accel(m, pos, 0.0)) < FP64_TOL
+    finally:
+        solver.set_kernel("auto")

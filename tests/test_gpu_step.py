@@ -102,3 +102,16 @@ def test_step_timing_counters(solver):
     ms, evals, step_ms = solver.timing_read()
     solver.timing(False)
     assert evals == 21 and ms > 0 and step_ms >= ms
+
+
+@pytest.mark.parametrize("kernel", ["tile", "symmetric"])
+def test_leapfrog_both_kernels(solver, orc, kernel):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 3000, 14)
+    solver.set_kernel(kernel)
+    try:
+        p, v = run(solver, m, pos, vel, 8, 1.0 / 128, 4, "fp64", 0.01)
+    finally:
+        solver.set_kernel("auto")
+    po, vo = orc.leapfrog(m, pos, vel, 8, 1.0 / 128, 4, 0.01)
+    assert np.max(np.abs(p - po)) < 1e-12 * np.max(np.abs(po))
+    assert np.max(np.abs(v - vo)) < 1e-11 * np.max(np.abs(vo))
