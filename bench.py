@@ -130,7 +130,19 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--forms", default="fp64,mixed,learned")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (a known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU ring is a design "
               f"note, not part of this build", file=sys.stderr)
@@ -151,6 +163,7 @@ def main():
         try:
             if form == "learned":
                 s.learned_configure(period=4, threshold=1e-3, exact_form="mixed")
+                s.learned_reset(1)
                 s.set_state(m, pos, vel)
                 s.learned_collect(200, args.dt, 2, "mixed", args.eps)
                 loss = s.learned_train(2000, 4096, 1e-3, 1)

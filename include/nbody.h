@@ -180,6 +180,7 @@ typedef struct nb_learned_stats {
     uint64_t exact_evals;      /* direct sums run by learned-form steps */
     uint64_t predicted_evals;  /* substeps served by the extrapolator */
     uint64_t fallbacks;        /* exact checks that exceeded the threshold */
+    uint64_t subset_rows;      /* flagged particles summed exactly on extrapolated substeps */
     double last_check_err;     /* max relative acceleration error, last check */
     double max_check_err;      /* largest such error since configure */
 } nb_learned_stats;
@@ -196,6 +197,11 @@ NB_API nb_status nb_learned_set_mode(nb_ctx* ctx, int mode);
  * record training pairs (features, target correction) in device memory. */
 NB_API nb_status nb_learned_collect(nb_ctx* ctx, size_t time_steps, double dt, const double* c,
                              const double* d, size_t lf_steps, int form, double eps);
+
+/* Forget the network (re-initialise it from `seed`: hidden layers uniform in
+ * +-1/sqrt(fan_in), output layer zero), its optimiser state and the collected
+ * training pairs. */
+NB_API nb_status nb_learned_reset(nb_ctx* ctx, uint64_t seed);
 
 /* Train the network on the collected pairs with the library's own MFMA
  * forward/backward kernels and Adam: `iters` minibatches of `batch`. */

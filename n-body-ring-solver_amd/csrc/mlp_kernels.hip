@@ -1,0 +1,339 @@
+// mlp_kernels.hip -- BF16 MFMA kernels of the learned acceleration
+// extrapolator: a 15 -> 128 -> 128 -> 128 -> 3 MLP with SiLU hidden
+// activations, FP32 accumulation, trained with Adam on FP32 master weights.
+//
+// Every matrix product runs on v_mfma_f32_16x16x16_bf16 in "transposed" form:
+// a layer computes D^T = W X^T with the weights as the A operand (rows =
+// output features) and 16 particles as the columns of the B operand.  The
+// accumulator layout of one layer -- lane (r, g), r = lane % 16, g = lane / 16,
+// holds features 4g..4g+3 of particle r -- is exactly the B-operand layout the
+// next layer needs, so activations never leave registers inside the forward or
+// the backward sweep.  Weight gradients dW = delta^T H are a GEMM over the
+// batch; the sweeps store delta and H feature-major (bf16) for it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mlp_kernels.h"
+
+namespace mlp {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short to_bf16(float f) {   // round to nearest even
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0x007fffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x4 v;
+    v[0] = (short)to_bf16(a);
+    v[1] = (short)to_bf16(b);
+    v[2] = (short)to_bf16(c);
+    v[3] = (short)to_bf16(d);
+    return v;
+}
+
+__device__ __forceinline__ bf16x4 load4(const unsigned short* __restrict__ p) {
+    return __builtin_bit_cast(bf16x4, *(const uint2*)p);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+__device__ __forceinline__ float silu_d(float z) {
+    const float s = sigmoid(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// Feature-major bf16 store of one accumulator tile: rows f0 + 4g + j, column p.
+__device__ __forceinline__ void store_fm(unsigned short* __restrict__ T, long ld, int f0, int g, long p, bf16x4 v) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) T[(long)(f0 + 4 * g + j) * ld + p] = (unsigned short)v[j];
+}
+
+// One hidden layer: out[m] (8 tiles of 16 features) = W (128 x 16*KT) times in[KT].
+template <int KT>
+__device__ __forceinline__ void layer(const unsigned short* __restrict__ W, const bf16x4* in, f32x4* out, int r,
+                                      int g) {
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < KT; kt++) acc = mfma(load4(W + (long)(16 * m + r) * (16 * KT) + 16 * kt + 4 * g), in[kt], acc);
+        out[m] = acc;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Forward: Y = f(X) for rows [0, rows); with `save` the pre-activations Z_l
+// (row-major [rows][128] fp32) and the layer inputs H_l (feature-major bf16,
+// H0 [16][ld], H1..H3 [128][ld]) are kept for the backward pass.  Rows past
+// `rows` (up to the 16-row tile) read zero inputs and are not written.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_fwd(Net net, const float* __restrict__ X, int rows, float* __restrict__ Y,
+                                                 int save, float* __restrict__ Z, unsigned short* __restrict__ H, long ld) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int tiles = (rows + 15) / 16;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    for (int tile = wave; tile < tiles; tile += nwaves) {
+        const long p = (long)tile * 16 + r;
+        const bool valid = p < rows;
+        float4 xin = valid ? *(const float4*)(X + p * 16 + 4 * g) : make_float4(0.f, 0.f, 0.f, 0.f);
+        bf16x4 h0[1] = {pack4(xin.x, xin.y, xin.z, xin.w)};
+        if (save) store_fm(H, ld, 0, g, p, h0[0]);
+        bf16x4 h[8];
+        f32x4 z[8];
+        layer<1>(net.W1, h0, z, r, g);
+#pragma unroll
+        for (int l = 0; l < 3; l++) {   // bias + SiLU of layer l+1, then layer l+2's product
+            const float* b = (l == 0) ? net.b1 : (l == 1) ? net.b2 : net.b3;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                f32x4 zz = z[m];
+#pragma unroll
+                for (int j = 0; j < 4; j++) zz[j] += b[16 * m + 4 * g + j];
+                if (save && valid) *(f32x4*)(Z + (long)l * ld * 128 + p * 128 + 16 * m + 4 * g) = zz;
+                h[m] = pack4(silu(zz[0]), silu(zz[1]), silu(zz[2]), silu(zz[3]));
+                if (save) store_fm(H + 16 * ld + (long)l * 128 * ld, ld, 16 * m, g, p, h[m]);
+            }
+            if (l < 2) layer<8>(l == 0 ? net.W2 : net.W3, h, z, r, g);
+        }
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 8; kt++) acc = mfma(load4(net.W4 + (long)r * 128 + 16 * kt + 4 * g), h[kt], acc);
+        if (g == 0 && valid) *(float4*)(Y + p * 4) = make_float4(acc[0] + net.b4[0], acc[1] + net.b4[1], acc[2] + net.b4[2], 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backward of L = sum (y - t)^2 / (2 rows): deltas of every layer, stored
+// feature-major bf16 (D4 [16][ld], D3..D1 [128][ld]); per-tile column sums of
+// the deltas (bias gradients, fp32) and of the squared error.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_bwd(Net net, const float* __restrict__ Y, const float* __restrict__ T,
+                                                 int rows, const float* __restrict__ Z, unsigned short* __restrict__ D,
+                                                 long ld, float* __restrict__ dbpart, float* __restrict__ losspart) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int tiles = (rows + 15) / 16;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    const float inv = 1.0f / (float)rows;
+    for (int tile = wave; tile < tiles; tile += nwaves) {
+        const long p = (long)tile * 16 + r;
+        const bool valid = p < rows;
+        // output delta: features 0..2 live in the g == 0 lanes
+        float e[3] = {0.f, 0.f, 0.f};
+        if (g == 0 && valid) {
+            const float4 y = *(const float4*)(Y + p * 4);
+            const float4 t = *(const float4*)(T + p * 4);
+            e[0] = y.x - t.x; e[1] = y.y - t.y; e[2] = y.z - t.z;
+        }
+        f32x4 d4 = {e[0] * inv, e[1] * inv, e[2] * inv, 0.f};
+        float sq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+        bf16x4 dl[8];
+        dl[0] = pack4(d4[0], d4[1], d4[2], d4[3]);
+        store_fm(D, ld, 0, g, p, dl[0]);
+        float* dbt = dbpart + (long)tile * (16 + 3 * 128);
+        // column sums over the 16 particles (lanes with equal g) by xor-shuffles
+        auto colsum = [&](float v) {
+            for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+            return v;
+        };
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float s = colsum(d4[j]);
+            if (r == 0) dbt[4 * g + j] = s;
+        }
+        sq = colsum(sq);
+        if (lane == 0) losspart[tile] = sq;
+        // hidden layers 3, 2, 1
+        f32x4 acc[8];
+#pragma unroll
+        for (int l = 2; l >= 0; l--) {
+            const unsigned short* WT = (l == 2) ? net.W4T : (l == 1) ? net.W3T : net.W2T;
+            const int KT = (l == 2) ? 1 : 8;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                f32x4 a = {0.f, 0.f, 0.f, 0.f};
+                for (int kt = 0; kt < KT; kt++) a = mfma(load4(WT + (long)(16 * m + r) * (16 * KT) + 16 * kt + 4 * g), dl[kt], a);
+                acc[m] = a;
+            }
+            float* dbl = dbt + 16 + (long)l * 128;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                f32x4 zz = valid ? *(const f32x4*)(Z + (long)l * ld * 128 + p * 128 + 16 * m + 4 * g) : f32x4{0.f, 0.f, 0.f, 0.f};
+                f32x4 d;
+#pragma unroll
+                for (int j = 0; j < 4; j++) d[j] = valid ? acc[m][j] * silu_d(zz[j]) : 0.f;
+                dl[m] = pack4(d[0], d[1], d[2], d[3]);
+                store_fm(D + 16 * ld + (long)(2 - l) * 128 * ld, ld, 16 * m, g, p, dl[m]);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float s = colsum(d[j]);
+                    if (r == 0) dbl[16 * m + 4 * g + j] = s;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dW_l = delta_l^T H_{l-1}, K = ld (the padded batch): one wave per 16x16
+// output tile.  Tiles: layer 1 (8 x 1), layer 2 (8 x 8), layer 3 (8 x 8),
+// layer 4 (1 x 8).  Writes fp32 into the packed gradient (W row-major
+// [out][in], W1 with in = 15, W4 with out = 3).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_dw(const unsigned short* __restrict__ H, const unsigned short* __restrict__ D,
+                                                long ld, float* __restrict__ grad) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // layer, tile (mt, nt), operand bases
+    int layer_id, mt, nt;
+    if (t < 8) { layer_id = 0; mt = t; nt = 0; }
+    else if (t < 72) { layer_id = 1; mt = (t - 8) / 8; nt = (t - 8) % 8; }
+    else if (t < 136) { layer_id = 2; mt = (t - 72) / 8; nt = (t - 72) % 8; }
+    else if (t < 144) { layer_id = 3; mt = 0; nt = t - 136; }
+    else return;
+    // delta of layer l: D layout [D4 16 rows][D3 128][D2 128][D1 128]
+    const long drow = (layer_id == 3) ? 0 : 16 + (long)(2 - layer_id) * 128;
+    const long hrow = (layer_id == 0) ? 0 : 16 + (long)(layer_id - 1) * 128;
+    const unsigned short* A = D + (drow + 16 * mt + r) * ld;   // A[m][k] = delta[k][out]
+    const unsigned short* B = H + (hrow + 16 * nt + r) * ld;   // B[k][n] = H[k][in]
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (long k = 0; k < ld; k += 16) acc = mfma(load4(A + k + 4 * g), load4(B + k + 4 * g), acc);
+    const int in_dim = (layer_id == 0) ? 15 : 128;
+    const int out_dim = (layer_id == 3) ? 3 : 128;
+    const long off = (layer_id == 0) ? 0 : (layer_id == 1) ? OFF_W2 : (layer_id == 2) ? OFF_W3 : OFF_W4;
+    const int col = 16 * nt + r;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int row = 16 * mt + 4 * g + j;
+        if (row < out_dim && col < in_dim) grad[off + (long)row * in_dim + col] = acc[j];
+    }
+}
+
+// Bias gradients and loss: sums over tiles in tile order.
+__global__ __launch_bounds__(256) void k_mlp_db(const float* __restrict__ dbpart, const float* __restrict__ losspart,
+                                                int tiles, float* __restrict__ grad, float* __restrict__ loss) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < 16 + 3 * 128) {
+        float s = 0.f;
+        for (int t = 0; t < tiles; t++) s += dbpart[(long)t * (16 + 3 * 128) + i];
+        // dbpart rows: [0,16) output, [16,144) layer 1, [144,272) layer 2, [272,400) layer 3
+        if (i < 3) grad[OFF_B4 + i] = s;
+        else if (i >= 16 && i < 144) grad[OFF_B1 + (i - 16)] = s;
+        else if (i >= 144 && i < 272) grad[OFF_B2 + (i - 144)] = s;
+        else if (i >= 272) grad[OFF_B3 + (i - 272)] = s;
+    } else if (i == 16 + 3 * 128) {
+        float s = 0.f;
+        for (int t = 0; t < tiles; t++) s += losspart[t];
+        *loss = s;
+    }
+}
+
+// Adam (Kingma & Ba 2015) with bias correction on the FP32 master weights.
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                                              float* __restrict__ v, int n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copies for the kernels: W1 [128][16] (column 15 zero), W2, W3 [128][128],
+// W4 [16][128] (rows 3..15 zero) and the transposes W2T, W3T [128][128],
+// W4T [128][16] (columns 3..15 zero); fp32 biases (b4 padded to 16).
+__global__ __launch_bounds__(256) void k_publish(const float* __restrict__ p, Net net) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    unsigned short* W1 = (unsigned short*)net.W1;
+    unsigned short* W2 = (unsigned short*)net.W2;
+    unsigned short* W3 = (unsigned short*)net.W3;
+    unsigned short* W4 = (unsigned short*)net.W4;
+    unsigned short* W2T = (unsigned short*)net.W2T;
+    unsigned short* W3T = (unsigned short*)net.W3T;
+    unsigned short* W4T = (unsigned short*)net.W4T;
+    if (i < 128 * 16) {
+        const int o = i / 16, c = i % 16;
+        W1[i] = (c < 15) ? to_bf16(p[OFF_W1 + o * 15 + c]) : 0;
+        const int row = i / 16, k = i % 16;   // W4T[row = in][k = out]
+        W4T[i] = (k < 3) ? to_bf16(p[OFF_W4 + k * 128 + row]) : 0;
+        const int o4 = i / 128, c4 = i % 128;   // W4[o4][c4]
+        W4[i] = (o4 < 3) ? to_bf16(p[OFF_W4 + o4 * 128 + c4]) : 0;
+    }
+    if (i < 128 * 128) {
+        const int o = i / 128, c = i % 128;
+        W2[i] = to_bf16(p[OFF_W2 + i]);
+        W3[i] = to_bf16(p[OFF_W3 + i]);
+        W2T[i] = to_bf16(p[OFF_W2 + c * 128 + o]);
+        W3T[i] = to_bf16(p[OFF_W3 + c * 128 + o]);
+    }
+    if (i < 128) {
+        ((float*)net.b1)[i] = p[OFF_B1 + i];
+        ((float*)net.b2)[i] = p[OFF_B2 + i];
+        ((float*)net.b3)[i] = p[OFF_B3 + i];
+    }
+    if (i < 16) ((float*)net.b4)[i] = (i < 3) ? p[OFF_B4 + i] : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static int grid_for(int rows) {
+    const int tiles = (rows + 15) / 16;
+    return std::max(1, std::min(4096, (tiles + 3) / 4));
+}
+
+hipError_t forward(const Net& net, const float* X, int rows, float* Y, bool save, float* Z, unsigned short* H, long ld,
+                   hipStream_t st) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(grid_for(rows)), dim3(256), 0, st, net, X, rows, Y, save ? 1 : 0, Z, H, ld);
+    return hipGetLastError();
+}
+
+hipError_t backward(const Net& net, const float* Y, const float* T, int rows, const float* Z, const unsigned short* H,
+                    unsigned short* D, long ld, float* dbpart, float* losspart, float* grad, float* loss,
+                    hipStream_t st) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_bwd, dim3(grid_for(rows)), dim3(256), 0, st, net, Y, T, rows, Z, D, ld, dbpart, losspart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_dw, dim3(36), dim3(256), 0, st, H, (const unsigned short*)D, ld, grad);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_db, dim3(2), dim3(256), 0, st, dbpart, losspart, (rows + 15) / 16, grad, loss);
+    return hipGetLastError();
+}
+
+hipError_t adam(float* p, const float* g, float* m, float* v, int n, float lr, int t, hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t publish(const float* p, const Net& net, hipStream_t st) {
+    hipLaunchKernelGGL(k_publish, dim3(64), dim3(256), 0, st, p, net);
+    return hipGetLastError();
+}
+
+}  // namespace mlp

@@ -602,7 +602,6 @@ nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, con
     const int n = int(c->n);
     P4* pm = (P4*)c->pm.p;
     P4* vel = (P4*)c->vel.p;
-    P4* acc = (P4*)c->acc.p;
     if (c->timing) (void)hipEventRecord(c->step_ev[0], c->stream);
     NB_HIP(nbk::launch_drift(n, pm, vel, cc[0] * dt, c->stream), "drift kernel");
     for (size_t t = 0; t < time_steps; t++) {

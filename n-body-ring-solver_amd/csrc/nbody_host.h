@@ -42,6 +42,7 @@ struct ForceOut {
 
 // Context internals used by the learned-form module.
 nb_status set_error(nb_status s, const std::string& msg);
+int choose_splits(const nb_ctx* c, long ni, long nj, int ipt);
 int kernel_form(int form);
 hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out);
 hipError_t finish_accel(nb_ctx* c, const ForceOut& out);

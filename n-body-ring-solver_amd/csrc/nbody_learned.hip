@@ -1,38 +1,782 @@
-// nbody_learned.hip -- learned acceleration extrapolator (placeholder until
-// the MLP kernels land): every entry point reports that it is unavailable.
+// nbody_learned.hip -- the learned acceleration extrapolator (NB_FORM_LEARNED).
+//
+// No counterpart in the reference.  Every force evaluation of the leapfrog is
+// a "substep".  Each particle keeps its last four accelerations a_n..a_{n-3}
+// (the ones the integrator used).  On three of every `period` substeps the
+// acceleration is extrapolated,
+//     a_{n+1} ~ cubic + |D3| tau f(x),   cubic = 4a_n - 6a_{n-1} + 4a_{n-2} - a_{n-3},
+// where D1..D3 are the backward differences, x = (a_n/|a_n|, D1/S, D2/S,
+// D3/S, v/|v|) with S = |D1|+|D2|+|D3| (15 numbers) and f is the MLP of
+// mlp_kernels.hip; the network predicts the residual of the cubic in units of
+// |D3|, so it cannot disturb smooth orbits (where |D3| is tiny).  On the
+// remaining substep the exact direct sum runs, the extrapolation made for the
+// same substep is compared with it, and (the first extrapolation is used only
+// after the first passed check)
+//   * while the RMS relative error sqrt(sum|da|^2 / sum|a|^2) exceeds
+//     `threshold` (or is NaN), every substep is exact (fallback);
+//   * particles whose own relative error exceeds flag_factor * threshold get
+//     exact sums (their rows only) on the following extrapolated substeps.
+// Mode 1 drops the network (f = 0): the method without it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "mlp_kernels.h"
 #include "nbody_host.h"
 #include "nbody_learned.h"
 
+using nbk::P4;
+using nbh::DevBuf;
+
 namespace nbl {
 
-struct Learned {};
+namespace {
 
-void destroy(Learned* l) { delete l; }
-void on_new_state(Learned*) {}
+constexpr int BLK = 256;
 
-nb_status step_learned(nb_ctx*, size_t, double, const double*, const double*, size_t, double) {
-    return nbh::set_error(NB_ERR_STATE, "nb_step: the learned form needs a trained extrapolator");
+// history slot k steps back from head (k = 0 newest)
+__device__ __forceinline__ const P4& hist_at(const P4* __restrict__ hist, int head, int k, long n, long i) {
+    return hist[(long)((head - k + 4) & 3) * n + i];
+}
+
+// Features, cubic extrapolation and output scale of particle i.
+__device__ __forceinline__ void features(const P4* __restrict__ hist, int head, long n, long i, const P4& v,
+                                         float* x, double* cubic, double& scale) {
+    const P4& a0 = hist_at(hist, head, 0, n, i);
+    const P4& a1 = hist_at(hist, head, 1, n, i);
+    const P4& a2 = hist_at(hist, head, 2, n, i);
+    const P4& a3 = hist_at(hist, head, 3, n, i);
+    const double A0[3] = {a0.x, a0.y, a0.z}, A1[3] = {a1.x, a1.y, a1.z}, A2[3] = {a2.x, a2.y, a2.z},
+                 A3[3] = {a3.x, a3.y, a3.z};
+    double d1[3], d2[3], d3[3];
+    for (int k = 0; k < 3; k++) {
+        d1[k] = A0[k] - A1[k];
+        d2[k] = A0[k] - 2.0 * A1[k] + A2[k];
+        d3[k] = A0[k] - 3.0 * A1[k] + 3.0 * A2[k] - A3[k];
+        cubic[k] = A0[k] + d1[k] + d2[k] + d3[k];
+    }
+    const double n0 = sqrt(A0[0] * A0[0] + A0[1] * A0[1] + A0[2] * A0[2]);
+    const double n1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
+    const double n2 = sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2]);
+    const double n3 = sqrt(d3[0] * d3[0] + d3[1] * d3[1] + d3[2] * d3[2]);
+    const double nv = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    const double S = n1 + n2 + n3;
+    const double i0 = n0 > 0 ? 1.0 / n0 : 0.0, iS = S > 0 ? 1.0 / S : 0.0, iv = nv > 0 ? 1.0 / nv : 0.0;
+    const double V[3] = {v.x, v.y, v.z};
+    for (int k = 0; k < 3; k++) {
+        x[k] = (float)(A0[k] * i0);
+        x[3 + k] = (float)(d1[k] * iS);
+        x[6 + k] = (float)(d2[k] * iS);
+        x[9 + k] = (float)(d3[k] * iS);
+        x[12 + k] = (float)(V[k] * iv);
+    }
+    x[15] = 0.f;
+    scale = n3;
+}
+
+// X[i] = features, cub[i] = (cubic, |D3|)
+__global__ __launch_bounds__(BLK) void k_features(const P4* __restrict__ hist, int head, long n, const P4* __restrict__ vel,
+                                                  float* __restrict__ X, P4* __restrict__ cub) {
+    const long i = (long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    float x[16];
+    double c[3], s;
+    features(hist, head, n, i, vel[i], x, c, s);
+    float4* xo = (float4*)(X + i * 16);
+    for (int q = 0; q < 4; q++) xo[q] = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+    cub[i] = P4{c[0], c[1], c[2], s};
+}
+
+// out = cubic + |D3| tau y  (use_net) or cubic
+__global__ __launch_bounds__(BLK) void k_combine(const P4* __restrict__ cub, const float* __restrict__ Y, float tau,
+                                                 int use_net, long n, P4* __restrict__ out) {
+    const long i = (long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    P4 c = cub[i];
+    if (use_net) {
+        const float4 y = *(const float4*)(Y + i * 4);
+        const double f = c.m * (double)tau;
+        c.x += f * y.x;
+        c.y += f * y.y;
+        c.z += f * y.z;
+    }
+    out[i] = P4{c.x, c.y, c.z, 0.0};
+}
+
+// Training pair: T[row] = (a_exact - cubic) / |D3|  (0 where |D3| = 0)
+__global__ __launch_bounds__(BLK) void k_targets(const P4* __restrict__ cub, const P4* __restrict__ acc, long n,
+                                                 float* __restrict__ T) {
+    const long i = (long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const P4 c = cub[i], a = acc[i];
+    const double inv = c.m > 0 ? 1.0 / c.m : 0.0;
+    *(float4*)(T + i * 4) = make_float4((float)((a.x - c.x) * inv), (float)((a.y - c.y) * inv),
+                                        (float)((a.z - c.z) * inv), 0.f);
+}
+
+// Per block: sum |pred - exact|^2, sum |exact|^2, max relative error; flag
+// particles above flag_thr (index list + counter).
+__global__ __launch_bounds__(BLK) void k_check(const P4* __restrict__ pred, const P4* __restrict__ acc, long n,
+                                               double flag_thr, double* __restrict__ out, int* __restrict__ flags,
+                                               int* __restrict__ nflags, int max_flags) {
+    __shared__ double s0[BLK], s1[BLK], s2[BLK];
+    const long i = (long)blockIdx.x * BLK + threadIdx.x;
+    double e2 = 0.0, a2 = 0.0, rel = 0.0;
+    if (i < n) {
+        const P4 p = pred[i], a = acc[i];
+        const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
+        e2 = dx * dx + dy * dy + dz * dz;
+        a2 = a.x * a.x + a.y * a.y + a.z * a.z;
+        rel = a2 > 0 ? sqrt(e2 / a2) : 0.0;
+        if (!(rel <= flag_thr)) {   // NaN predictions are flagged too
+            const int k = atomicAdd(nflags, 1);
+            if (k < max_flags) flags[k] = (int)i;
+            if (!(rel == rel)) rel = INFINITY;
+        }
+    }
+    s0[threadIdx.x] = e2;
+    s1[threadIdx.x] = a2;
+    s2[threadIdx.x] = rel;
+    __syncthreads();
+    for (int w = BLK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x] = s0[0];
+        out[3 * blockIdx.x + 1] = s1[0];
+        out[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_gather(const P4* __restrict__ src, const int* __restrict__ idx, int m,
+                                                P4* __restrict__ dst) {
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k < m) dst[k] = src[idx[k]];
+}
+
+__global__ __launch_bounds__(BLK) void k_scatter(const P4* __restrict__ src, const int* __restrict__ idx, int m,
+                                                 P4* __restrict__ dst) {
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k < m) dst[idx[k]] = src[k];
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Minibatch: rows drawn with a counter-based hash of (seed, iteration, slot);
+// targets scaled by 1/tau and clipped to +-clip.
+__global__ __launch_bounds__(BLK) void k_batch(const float* __restrict__ dsX, const float* __restrict__ dsT, long rows,
+                                               unsigned long long seed, int it, int batch, float inv_tau, float clip,
+                                               float* __restrict__ bX, float* __restrict__ bT) {
+    const int k = blockIdx.x * BLK + threadIdx.x;
+    if (k >= batch) return;
+    const unsigned long long h = mix64(seed * 0x9E3779B97F4A7C15ULL + ((unsigned long long)it << 32) + (unsigned)k + 1);
+    const long r = (long)(h % (unsigned long long)rows);
+    const float4* xs = (const float4*)(dsX + r * 16);
+    float4* xd = (float4*)(bX + (long)k * 16);
+    for (int q = 0; q < 4; q++) xd[q] = xs[q];
+    const float4 t = *(const float4*)(dsT + r * 4);
+    auto c = [&](float v) { return fminf(clip, fmaxf(-clip, v * inv_tau)); };
+    *(float4*)(bT + (long)k * 4) = make_float4(c(t.x), c(t.y), c(t.z), 0.f);
+}
+
+// sum of squares of the dataset targets, per block (for tau)
+__global__ __launch_bounds__(BLK) void k_sumsq(const float* __restrict__ dsT, long rows, double* __restrict__ out) {
+    __shared__ double s[BLK];
+    double acc = 0.0;
+    for (long r = (long)blockIdx.x * BLK + threadIdx.x; r < rows; r += (long)gridDim.x * BLK) {
+        const float4 t = *(const float4*)(dsT + r * 4);
+        acc += (double)t.x * t.x + (double)t.y * t.y + (double)t.z * t.z;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = BLK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+inline dim3 blocks(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
+
+}  // namespace
+
+struct Learned {
+    // configuration
+    int period = 4;
+    double threshold = 1e-3;
+    double flag_factor = 10.0;
+    int exact_kform = nbk::FORM_MIXED_NR;
+    int mode = 0;
+    // network: fp32 master parameters, Adam moments, gradient, device copies
+    DevBuf params, am, av, grad, netmem, lossbuf;
+    mlp::Net net{};
+    bool initialised = false, trained = false;
+    int adam_t = 0;
+    float tau = 1.0f;
+    // training set
+    DevBuf dsX, dsT;
+    long ds_rows = 0, ds_cap = 0;
+    // training scratch
+    DevBuf bX, bT, bY, bZ, bH, bD, dbpart, losspart;
+    // per-particle state
+    size_t n = 0;
+    DevBuf hist, X, Y, cub, pred, chk, flags, nflags, sub_p, sub_part, sub_acc;
+    int head = 3, count = 0;
+    long since_exact = 0;
+    bool fallback = true;
+    int nflag = 0;
+    std::vector<double> chk_h;
+    nb_learned_stats st{};
+};
+
+namespace {
+
+nb_status hip_err(hipError_t e, const char* what) {
+    return nbh::set_error(e == hipErrorOutOfMemory ? NB_ERR_ALLOC : NB_ERR_DEVICE,
+                          std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define LH(call, what)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hip_err(e_, what);  \
+    } while (0)
+
+Learned* get(nb_ctx* c) {
+    Learned*& l = nbh::ctx_learned(c);
+    if (!l) l = new (std::nothrow) Learned;
+    return l;
+}
+
+// Device copies of the network live in one allocation.
+nb_status net_alloc(Learned& L) {
+    if (L.netmem.p) return NB_OK;
+    LH(L.netmem.ensure(mlp::NET_BYTES + 256), "allocate network");
+    LH(L.params.ensure(mlp::NPARAM * sizeof(float)), "allocate parameters");
+    LH(L.am.ensure(mlp::NPARAM * sizeof(float)), "allocate Adam moments");
+    LH(L.av.ensure(mlp::NPARAM * sizeof(float)), "allocate Adam moments");
+    LH(L.grad.ensure(mlp::NPARAM * sizeof(float)), "allocate gradient");
+    LH(L.lossbuf.ensure(64), "allocate loss");
+    char* p = (char*)L.netmem.p;
+    auto take = [&](long bytes) {
+        char* q = p;
+        p += (bytes + 63) & ~63L;
+        return q;
+    };
+    L.net.W1 = (const unsigned short*)take(128 * 16 * 2);
+    L.net.W2 = (const unsigned short*)take(128 * 128 * 2);
+    L.net.W3 = (const unsigned short*)take(128 * 128 * 2);
+    L.net.W4 = (const unsigned short*)take(16 * 128 * 2);
+    L.net.W2T = (const unsigned short*)take(128 * 128 * 2);
+    L.net.W3T = (const unsigned short*)take(128 * 128 * 2);
+    L.net.W4T = (const unsigned short*)take(128 * 16 * 2);
+    L.net.b1 = (const float*)take(128 * 4);
+    L.net.b2 = (const float*)take(128 * 4);
+    L.net.b3 = (const float*)take(128 * 4);
+    L.net.b4 = (const float*)take(16 * 4);
+    return NB_OK;
+}
+
+// Upload host parameters (fp32) and refresh the device copies.
+nb_status net_set(Learned& L, const float* host, hipStream_t st) {
+    if (nb_status s = net_alloc(L)) return s;
+    LH(hipMemcpyAsync(L.params.p, host, mlp::NPARAM * sizeof(float), hipMemcpyHostToDevice, st), "upload parameters");
+    LH(hipMemsetAsync(L.am.p, 0, mlp::NPARAM * sizeof(float), st), "clear Adam moments");
+    LH(hipMemsetAsync(L.av.p, 0, mlp::NPARAM * sizeof(float), st), "clear Adam moments");
+    LH(mlp::publish((const float*)L.params.p, L.net, st), "publish kernel");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    L.adam_t = 0;
+    L.initialised = true;
+    return NB_OK;
+}
+
+// Seeded initialisation: W ~ U(-1/sqrt(in), 1/sqrt(in)), b = 0; the output
+// layer starts at zero so the untrained network leaves the cubic unchanged.
+nb_status net_init(Learned& L, uint64_t seed, hipStream_t st) {
+    std::vector<float> h(mlp::NPARAM, 0.f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](long off, int out, int in) {
+        std::uniform_real_distribution<float> u(-1.0f / std::sqrt((float)in), 1.0f / std::sqrt((float)in));
+        for (long k = 0; k < (long)out * in; k++) h[off + k] = u(rng);
+    };
+    fill(mlp::OFF_W1, 128, 15);
+    fill(mlp::OFF_W2, 128, 128);
+    fill(mlp::OFF_W3, 128, 128);
+    return net_set(L, h.data(), st);
+}
+
+nb_status ensure_particles(Learned& L, size_t n) {
+    if (L.n == n && L.hist.p) return NB_OK;
+    const size_t m = std::max<size_t>(n, 1);
+    LH(L.hist.ensure(4 * m * sizeof(P4)), "allocate history");
+    LH(L.X.ensure(m * 16 * sizeof(float)), "allocate features");
+    LH(L.Y.ensure(m * 4 * sizeof(float)), "allocate network output");
+    LH(L.cub.ensure(m * sizeof(P4)), "allocate extrapolation");
+    LH(L.pred.ensure(m * sizeof(P4)), "allocate prediction");
+    LH(L.chk.ensure(((m + BLK - 1) / BLK) * 3 * sizeof(double)), "allocate check sums");
+    LH(L.flags.ensure(m * sizeof(int)), "allocate flags");
+    LH(L.nflags.ensure(sizeof(int)), "allocate flag counter");
+    L.n = n;
+    L.count = 0;
+    L.head = 3;
+    return NB_OK;
+}
+
+// Extrapolated accelerations for every particle into `out`.
+nb_status predict(nb_ctx* c, Learned& L, P4* out) {
+    hipStream_t st = nbh::ctx_stream(c);
+    const long n = (long)L.n;
+    hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, L.head, n,
+                       (const P4*)nbh::ctx_vel(c), (float*)L.X.p, (P4*)L.cub.p);
+    LH(hipGetLastError(), "features kernel");
+    const bool use_net = (L.mode == 0 && L.trained);
+    if (use_net) LH(mlp::forward(L.net, (const float*)L.X.p, (int)n, (float*)L.Y.p, false, nullptr, nullptr, 0, st), "network forward");
+    hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float*)L.Y.p, L.tau,
+                       use_net ? 1 : 0, n, out);
+    LH(hipGetLastError(), "combine kernel");
+    return NB_OK;
+}
+
+// Exact sums for the flagged particles only, written into acc.
+nb_status exact_subset(nb_ctx* c, Learned& L, double eps, int kform) {
+    if (L.nflag <= 0) return NB_OK;
+    hipStream_t st = nbh::ctx_stream(c);
+    const long n = (long)L.n, m = L.nflag;
+    const int ipt = 1;
+    int S = nbh::choose_splits(c, m, n, ipt);
+    long jchunk = (n + S - 1) / S;
+    jchunk = (jchunk + 63) & ~63L;
+    S = int((n + jchunk - 1) / jchunk);
+    LH(L.sub_p.ensure(m * sizeof(P4)), "allocate subset");
+    LH(L.sub_part.ensure((size_t)S * m * sizeof(P4)), "allocate subset partials");
+    LH(L.sub_acc.ensure(m * sizeof(P4)), "allocate subset output");
+    hipLaunchKernelGGL(k_gather, blocks(m), dim3(BLK), 0, st, (const P4*)nbh::ctx_pm(c), (const int*)L.flags.p, (int)m,
+                       (P4*)L.sub_p.p);
+    LH(hipGetLastError(), "gather kernel");
+    nbk::AccelLaunch A;
+    A.form = kform;
+    A.ipt = ipt;
+    A.pi = (const P4*)L.sub_p.p;
+    A.ni = (int)m;
+    A.pj = (const P4*)nbh::ctx_pm(c);
+    A.jbeg = 0;
+    A.jend = (int)n;
+    A.jchunk = (int)jchunk;
+    A.splits = S;
+    A.eps2 = eps * eps;
+    A.G = 1.0;
+    A.out = (P4*)L.sub_part.p;
+    A.out_stride = m;
+    LH(nbk::launch_accel(A, st), "subset force kernel");
+    LH(nbk::launch_reduce((const P4*)L.sub_part.p, S, m, (int)m, (P4*)L.sub_acc.p, st), "subset reduce kernel");
+    hipLaunchKernelGGL(k_scatter, blocks(m), dim3(BLK), 0, st, (const P4*)L.sub_acc.p, (const int*)L.flags.p, (int)m,
+                       nbh::ctx_acc(c));
+    LH(hipGetLastError(), "scatter kernel");
+    return NB_OK;
+}
+
+// Compare the extrapolation in L.pred with the exact acc; update stats, the
+// fallback state and the flag list.  Synchronises the stream.
+nb_status check(nb_ctx* c, Learned& L) {
+    hipStream_t st = nbh::ctx_stream(c);
+    const long n = (long)L.n;
+    const long nb = (n + BLK - 1) / BLK;
+    LH(hipMemsetAsync(L.nflags.p, 0, sizeof(int), st), "clear flag counter");
+    hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, (const P4*)L.pred.p,
+                       (const P4*)nbh::ctx_acc(c), n, L.flag_factor * L.threshold, (double*)L.chk.p, (int*)L.flags.p,
+                       (int*)L.nflags.p, (int)n);
+    LH(hipGetLastError(), "check kernel");
+    L.chk_h.resize(3 * nb);
+    int nf = 0;
+    LH(hipMemcpyAsync(L.chk_h.data(), L.chk.p, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, st), "download check");
+    LH(hipMemcpyAsync(&nf, L.nflags.p, sizeof(int), hipMemcpyDeviceToHost, st), "download flag count");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    double e2 = 0.0, a2 = 0.0, mx = 0.0;
+    for (long b = 0; b < nb; b++) {
+        e2 += L.chk_h[3 * b];
+        a2 += L.chk_h[3 * b + 1];
+        mx = std::max(mx, L.chk_h[3 * b + 2]);
+    }
+    const double err = a2 > 0 ? std::sqrt(e2 / a2) : (e2 > 0 ? INFINITY : 0.0);
+    L.st.last_check_err = err;
+    L.st.max_check_err = std::max(L.st.max_check_err, std::isnan(err) ? INFINITY : err);
+    const bool ok = (err <= L.threshold);   // NaN fails
+    if (!ok) L.st.fallbacks++;
+    L.fallback = !ok;
+    L.nflag = ok ? std::min<int>(nf, (int)n) : 0;
+    return NB_OK;
+}
+
+// Push acc into the history ring.
+nb_status push_history(nb_ctx* c, Learned& L) {
+    L.head = (L.head + 1) & 3;
+    LH(hipMemcpyAsync((P4*)L.hist.p + (long)L.head * L.n, nbh::ctx_acc(c), L.n * sizeof(P4), hipMemcpyDeviceToDevice,
+                      nbh::ctx_stream(c)),
+       "push history");
+    L.count = std::min(L.count + 1, 4);
+    return NB_OK;
+}
+
+// Append this substep's (features, target) pairs: features from the history
+// before the exact acceleration is pushed.
+nb_status record(nb_ctx* c, Learned& L) {
+    hipStream_t st = nbh::ctx_stream(c);
+    const long n = (long)L.n;
+    if (L.ds_rows + n > L.ds_cap) return NB_OK;
+    float* X = (float*)L.dsX.p + L.ds_rows * 16;
+    float* T = (float*)L.dsT.p + L.ds_rows * 4;
+    hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, L.head, n,
+                       (const P4*)nbh::ctx_vel(c), X, (P4*)L.cub.p);
+    LH(hipGetLastError(), "features kernel");
+    hipLaunchKernelGGL(k_targets, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const P4*)nbh::ctx_acc(c), n, T);
+    LH(hipGetLastError(), "targets kernel");
+    L.ds_rows += n;
+    return NB_OK;
+}
+
+struct Schedule {   // drift/kick coefficients of substep s in step t
+    double kdt, c1, c2;
+};
+
+Schedule schedule(size_t t, size_t steps, size_t s, double dt, const double* cc, const double* dd, size_t lf) {
+    Schedule q{dd[s] * dt, 0.0, 0.0};
+    if (s + 2 < lf) {
+        q.c1 = cc[s + 1] * dt;
+    } else {
+        q.c1 = cc[lf - 1] * dt;
+        if (t + 1 < steps) q.c2 = cc[0] * dt;
+    }
+    return q;
+}
+
+nb_status exact_force(nb_ctx* c, int kform, double eps) {
+    nbh::ForceOut out;
+    LH(nbh::force_eval(c, kform, eps, out), "force kernel");
+    LH(nbh::finish_accel(c, out), "reduce kernel");
+    return NB_OK;
+}
+
+nb_status kick_drift(nb_ctx* c, const Schedule& q) {
+    nbh::ForceOut self;
+    self.sym = false;
+    self.part = nbh::ctx_acc(c);
+    self.S = 1;
+    LH(nbh::finish_kick_drift(c, self, q.kdt, q.c1, q.c2), "kick/drift kernel");
+    return NB_OK;
+}
+
+}  // namespace
+
+void destroy(Learned* l) {
+    if (!l) return;
+    for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
+                      &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub,
+                      &l->pred, &l->chk, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc})
+        b->release();
+    delete l;
+}
+
+void on_new_state(Learned* l) {
+    if (!l) return;
+    l->count = 0;
+    l->head = 3;
+    l->since_exact = 0;
+    l->fallback = true;
+    l->nflag = 0;
+}
+
+nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc, const double* dd, size_t lf,
+                       double eps) {
+    Learned* Lp = nbh::ctx_learned(c);
+    if (!Lp) return nbh::set_error(NB_ERR_STATE, "nb_step: configure the learned form first (nb_learned_configure)");
+    Learned& L = *Lp;
+    const size_t n = nbh::ctx_n(c);
+    if (n == 0 || time_steps == 0) return NB_OK;
+    if (nb_status s = ensure_particles(L, n)) return s;
+    hipStream_t st = nbh::ctx_stream(c);
+    LH(nbk::launch_drift((int)n, nbh::ctx_pm(c), nbh::ctx_vel(c), cc[0] * dt, st), "drift kernel");
+    for (size_t t = 0; t < time_steps; t++) {
+        for (size_t s = 0; s + 1 < lf; s++) {
+            const bool full = (L.count == 4);
+            const bool exact = !full || L.fallback || (L.since_exact + 1 >= L.period);
+            if (exact) {
+                if (full) {
+                    if (nb_status e = predict(c, L, (P4*)L.pred.p)) return e;
+                }
+                if (nb_status e = exact_force(c, L.exact_kform, eps)) return e;
+                L.st.exact_evals++;
+                // extrapolation starts only after a check has passed, so a
+                // network that produces garbage never reaches the integrator
+                if (full) {
+                    if (nb_status e = check(c, L)) return e;
+                }
+                L.since_exact = 0;
+            } else {
+                if (nb_status e = predict(c, L, nbh::ctx_acc(c))) return e;
+                if (nb_status e = exact_subset(c, L, eps, L.exact_kform)) return e;
+                L.st.predicted_evals++;
+                L.st.subset_rows += (uint64_t)L.nflag;
+                L.since_exact++;
+            }
+            if (nb_status e = push_history(c, L)) return e;
+            if (nb_status e = kick_drift(c, schedule(t, time_steps, s, dt, cc, dd, lf))) return e;
+        }
+    }
+    return NB_OK;
 }
 
 }  // namespace nbl
 
-static nb_status unavailable(const char* fn) {
-    return nbh::set_error(NB_ERR_STATE, std::string(fn) + ": learned extrapolator not available");
-}
+using nbl::Learned;
+using nbl::hip_err;
 
 extern "C" {
-nb_status nb_learned_configure(nb_ctx*, int, double, int) { return unavailable("nb_learned_configure"); }
-nb_status nb_learned_set_mode(nb_ctx*, int) { return unavailable("nb_learned_set_mode"); }
-nb_status nb_learned_collect(nb_ctx*, size_t, double, const double*, const double*, size_t, int, double) {
-    return unavailable("nb_learned_collect");
+
+nb_status nb_learned_configure(nb_ctx* c, int period, double threshold, int exact_form) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_configure: null context");
+    if (period < 2) return nbh::set_error(NB_ERR_ARG, "nb_learned_configure: period must be >= 2");
+    if (!(threshold >= 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_learned_configure: threshold must be >= 0");
+    const int kf = nbh::kernel_form(exact_form);
+    if (kf < 0) return nbh::set_error(NB_ERR_ARG, "nb_learned_configure: exact_form must be FP64 or a mixed form");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_configure: out of host memory");
+    L->period = period;
+    L->threshold = threshold;
+    L->exact_kform = kf;
+    L->st = nb_learned_stats{};
+    nbl::on_new_state(L);
+    return NB_OK;
 }
-nb_status nb_learned_train(nb_ctx*, int, int, double, uint64_t, double*) { return unavailable("nb_learned_train"); }
-nb_status nb_learned_stats_read(nb_ctx*, nb_learned_stats*) { return unavailable("nb_learned_stats_read"); }
-nb_status nb_mlp_param_count(size_t*) { return unavailable("nb_mlp_param_count"); }
-nb_status nb_mlp_get_params(nb_ctx*, float*) { return unavailable("nb_mlp_get_params"); }
-nb_status nb_mlp_set_params(nb_ctx*, const float*) { return unavailable("nb_mlp_set_params"); }
-nb_status nb_mlp_forward(nb_ctx*, size_t, const float*, float*) { return unavailable("nb_mlp_forward"); }
-nb_status nb_mlp_grad(nb_ctx*, size_t, const float*, const float*, float*, double*) {
-    return unavailable("nb_mlp_grad");
+
+nb_status nb_learned_set_mode(nb_ctx* c, int mode) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_mode: null context");
+    if (mode != 0 && mode != 1) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_mode: mode must be 0 or 1");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_set_mode: out of host memory");
+    L->mode = mode;
+    return NB_OK;
 }
+
+nb_status nb_learned_collect(nb_ctx* c, size_t time_steps, double dt, const double* cc, const double* dd, size_t lf,
+                             int form, double eps) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_collect: null context");
+    if (!cc || !dd || lf < 2 || lf > 16 || !std::isfinite(dt))
+        return nbh::set_error(NB_ERR_ARG, "nb_learned_collect: bad coefficients or dt");
+    if (!(eps >= 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_learned_collect: eps must be >= 0");
+    const int kf = nbh::kernel_form(form);
+    if (kf < 0) return nbh::set_error(NB_ERR_ARG, "nb_learned_collect: form must be FP64 or a mixed form");
+    Learned* Lp = nbl::get(c);
+    if (!Lp) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_collect: out of host memory");
+    Learned& L = *Lp;
+    const size_t n = nbh::ctx_n(c);
+    if (n == 0 || time_steps == 0) return NB_OK;
+    if (nb_status s = nbl::ensure_particles(L, n)) return s;
+    nbl::on_new_state(&L);
+    const long subs = (long)time_steps * (long)(lf - 1);
+    const long want = L.ds_rows + std::max(0L, subs - 4) * (long)n;
+    if (want > L.ds_cap) {   // grow, keeping what was collected before
+        DevBuf x2, t2;
+        LH(x2.ensure(want * 16 * sizeof(float)), "allocate training features");
+        LH(t2.ensure(want * 4 * sizeof(float)), "allocate training targets");
+        hipStream_t st = nbh::ctx_stream(c);
+        if (L.ds_rows) {
+            LH(hipMemcpyAsync(x2.p, L.dsX.p, L.ds_rows * 16 * sizeof(float), hipMemcpyDeviceToDevice, st), "copy");
+            LH(hipMemcpyAsync(t2.p, L.dsT.p, L.ds_rows * 4 * sizeof(float), hipMemcpyDeviceToDevice, st), "copy");
+            LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+        }
+        L.dsX.release();
+        L.dsT.release();
+        L.dsX = x2;
+        L.dsT = t2;
+        x2.p = t2.p = nullptr;
+        L.ds_cap = want;
+    }
+    hipStream_t st = nbh::ctx_stream(c);
+    LH(nbk::launch_drift((int)n, nbh::ctx_pm(c), nbh::ctx_vel(c), cc[0] * dt, st), "drift kernel");
+    for (size_t t = 0; t < time_steps; t++)
+        for (size_t s = 0; s + 1 < lf; s++) {
+            if (nb_status e = nbl::exact_force(c, kf, eps)) return e;
+            if (L.count == 4)
+                if (nb_status e = nbl::record(c, L)) return e;
+            if (nb_status e = nbl::push_history(c, L)) return e;
+            if (nb_status e = nbl::kick_drift(c, nbl::schedule(t, time_steps, s, dt, cc, dd, lf))) return e;
+        }
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    nbl::on_new_state(&L);
+    return NB_OK;
 }
+
+nb_status nb_learned_train(nb_ctx* c, int iters, int batch, double lr, uint64_t seed, double* final_loss) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_train: null context");
+    if (iters < 0 || batch < 16 || batch > (1 << 20) || !(lr > 0.0))
+        return nbh::set_error(NB_ERR_ARG, "nb_learned_train: need iters >= 0, 16 <= batch <= 2^20, lr > 0");
+    Learned* Lp = nbh::ctx_learned(c);
+    if (!Lp || Lp->ds_rows == 0) return nbh::set_error(NB_ERR_STATE, "nb_learned_train: no training data collected");
+    Learned& L = *Lp;
+    hipStream_t st = nbh::ctx_stream(c);
+    if (!L.initialised)
+        if (nb_status s = nbl::net_init(L, seed, st)) return s;
+    const long b = ((long)batch + 15) & ~15L;
+    LH(L.bX.ensure(b * 16 * sizeof(float)), "allocate batch");
+    LH(L.bT.ensure(b * 4 * sizeof(float)), "allocate batch");
+    LH(L.bY.ensure(b * 4 * sizeof(float)), "allocate batch");
+    LH(L.bZ.ensure(3 * b * 128 * sizeof(float)), "allocate activations");
+    LH(L.bH.ensure((16 + 3 * 128) * b * sizeof(unsigned short)), "allocate activations");
+    LH(L.bD.ensure((16 + 3 * 128) * b * sizeof(unsigned short)), "allocate deltas");
+    LH(L.dbpart.ensure((b / 16) * (16 + 3 * 128) * sizeof(float)), "allocate bias partials");
+    LH(L.losspart.ensure((b / 16) * sizeof(float)), "allocate loss partials");
+    // target scale: RMS of the collected residuals
+    {
+        const int nbk_ = 256;
+        nbh::DevBuf ss;
+        LH(ss.ensure(nbk_ * sizeof(double)), "allocate");
+        hipLaunchKernelGGL(nbl::k_sumsq, dim3(nbk_), dim3(nbl::BLK), 0, st, (const float*)L.dsT.p, L.ds_rows, (double*)ss.p);
+        std::vector<double> h(nbk_);
+        LH(hipMemcpyAsync(h.data(), ss.p, nbk_ * sizeof(double), hipMemcpyDeviceToHost, st), "download");
+        LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+        double s = 0.0;
+        for (double v : h) s += v;
+        const double rms = std::sqrt(s / (3.0 * (double)L.ds_rows));
+        L.tau = (rms > 0 && std::isfinite(rms)) ? (float)rms : 1.0f;
+        ss.release();
+    }
+    const float clip = 8.0f;
+    for (int it = 0; it < iters; it++) {
+        hipLaunchKernelGGL(nbl::k_batch, dim3((unsigned)((b + nbl::BLK - 1) / nbl::BLK)), dim3(nbl::BLK), 0, st,
+                           (const float*)L.dsX.p, (const float*)L.dsT.p, L.ds_rows, (unsigned long long)seed, it,
+                           (int)b, 1.0f / L.tau, clip, (float*)L.bX.p, (float*)L.bT.p);
+        LH(hipGetLastError(), "batch kernel");
+        LH(mlp::forward(L.net, (const float*)L.bX.p, (int)b, (float*)L.bY.p, true, (float*)L.bZ.p,
+                        (unsigned short*)L.bH.p, b, st), "network forward");
+        LH(mlp::backward(L.net, (const float*)L.bY.p, (const float*)L.bT.p, (int)b, (const float*)L.bZ.p,
+                         (const unsigned short*)L.bH.p, (unsigned short*)L.bD.p, b, (float*)L.dbpart.p,
+                         (float*)L.losspart.p, (float*)L.grad.p, (float*)L.lossbuf.p, st), "network backward");
+        LH(mlp::adam((float*)L.params.p, (const float*)L.grad.p, (float*)L.am.p, (float*)L.av.p, (int)mlp::NPARAM,
+                     (float)lr, ++L.adam_t, st), "Adam kernel");
+        LH(mlp::publish((const float*)L.params.p, L.net, st), "publish kernel");
+    }
+    float loss = 0.f;
+    LH(hipMemcpyAsync(&loss, L.lossbuf.p, sizeof(float), hipMemcpyDeviceToHost, st), "download loss");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (final_loss) *final_loss = 0.5 * (double)loss / (double)b;   // last minibatch, before its update
+    L.trained = true;
+    return NB_OK;
+}
+
+nb_status nb_learned_reset(nb_ctx* c, uint64_t seed) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_reset: null context");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_reset: out of host memory");
+    if (nb_status s = nbl::net_init(*L, seed, nbh::ctx_stream(c))) return s;
+    L->trained = false;
+    L->ds_rows = 0;
+    L->tau = 1.0f;
+    nbl::on_new_state(L);
+    return NB_OK;
+}
+
+nb_status nb_learned_stats_read(nb_ctx* c, nb_learned_stats* out) {
+    if (!c || !out) return nbh::set_error(NB_ERR_ARG, "nb_learned_stats_read: null argument");
+    Learned* L = nbh::ctx_learned(c);
+    *out = L ? L->st : nb_learned_stats{};
+    return NB_OK;
+}
+
+nb_status nb_mlp_param_count(size_t* count) {
+    if (!count) return nbh::set_error(NB_ERR_ARG, "nb_mlp_param_count: null argument");
+    *count = (size_t)mlp::NPARAM;
+    return NB_OK;
+}
+
+nb_status nb_mlp_get_params(nb_ctx* c, float* params) {
+    if (!c || !params) return nbh::set_error(NB_ERR_ARG, "nb_mlp_get_params: null argument");
+    Learned* L = nbh::ctx_learned(c);
+    if (!L || !L->initialised) return nbh::set_error(NB_ERR_STATE, "nb_mlp_get_params: no network yet");
+    hipStream_t st = nbh::ctx_stream(c);
+    LH(hipMemcpyAsync(params, L->params.p, mlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost, st), "download parameters");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return NB_OK;
+}
+
+nb_status nb_mlp_set_params(nb_ctx* c, const float* params) {
+    if (!c || !params) return nbh::set_error(NB_ERR_ARG, "nb_mlp_set_params: null argument");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_mlp_set_params: out of host memory");
+    if (nb_status s = nbl::net_set(*L, params, nbh::ctx_stream(c))) return s;
+    L->trained = true;
+    return NB_OK;
+}
+
+static nb_status mlp_prepare(nb_ctx* c, size_t batch, const float* x, Learned*& L, long& b, const char* fn) {
+    if (!c || !x || batch == 0 || batch > (1u << 22)) return nbh::set_error(NB_ERR_ARG, std::string(fn) + ": bad arguments");
+    L = nbh::ctx_learned(c);
+    if (!L || !L->initialised) return nbh::set_error(NB_ERR_STATE, std::string(fn) + ": set or train the network first");
+    b = ((long)batch + 15) & ~15L;
+    hipStream_t st = nbh::ctx_stream(c);
+    LH(L->bX.ensure(b * 16 * sizeof(float)), "allocate batch");
+    LH(L->bT.ensure(b * 4 * sizeof(float)), "allocate batch");
+    LH(L->bY.ensure(b * 4 * sizeof(float)), "allocate batch");
+    std::vector<float> xp((size_t)batch * 16, 0.f);
+    for (size_t r = 0; r < batch; r++) std::memcpy(&xp[r * 16], x + r * 15, 15 * sizeof(float));
+    LH(hipMemcpyAsync(L->bX.p, xp.data(), xp.size() * sizeof(float), hipMemcpyHostToDevice, st), "upload inputs");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return NB_OK;
+}
+
+nb_status nb_mlp_forward(nb_ctx* c, size_t batch, const float* x, float* y) {
+    Learned* L;
+    long b;
+    if (!y) return nbh::set_error(NB_ERR_ARG, "nb_mlp_forward: null output");
+    if (nb_status s = mlp_prepare(c, batch, x, L, b, "nb_mlp_forward")) return s;
+    hipStream_t st = nbh::ctx_stream(c);
+    LH(mlp::forward(L->net, (const float*)L->bX.p, (int)batch, (float*)L->bY.p, false, nullptr, nullptr, 0, st), "network forward");
+    std::vector<float> yp((size_t)batch * 4);
+    LH(hipMemcpyAsync(yp.data(), L->bY.p, yp.size() * sizeof(float), hipMemcpyDeviceToHost, st), "download outputs");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (size_t r = 0; r < batch; r++) std::memcpy(y + r * 3, &yp[r * 4], 3 * sizeof(float));
+    return NB_OK;
+}
+
+nb_status nb_mlp_grad(nb_ctx* c, size_t batch, const float* x, const float* target, float* grads, double* loss) {
+    Learned* L;
+    long b;
+    if (!target || !grads) return nbh::set_error(NB_ERR_ARG, "nb_mlp_grad: null argument");
+    if (nb_status s = mlp_prepare(c, batch, x, L, b, "nb_mlp_grad")) return s;
+    hipStream_t st = nbh::ctx_stream(c);
+    LH(L->bZ.ensure(3 * b * 128 * sizeof(float)), "allocate activations");
+    LH(L->bH.ensure((16 + 3 * 128) * b * sizeof(unsigned short)), "allocate activations");
+    LH(L->bD.ensure((16 + 3 * 128) * b * sizeof(unsigned short)), "allocate deltas");
+    LH(L->dbpart.ensure((b / 16) * (16 + 3 * 128) * sizeof(float)), "allocate bias partials");
+    LH(L->losspart.ensure((b / 16) * sizeof(float)), "allocate loss partials");
+    std::vector<float> tp((size_t)b * 4, 0.f);
+    for (size_t r = 0; r < batch; r++) std::memcpy(&tp[r * 4], target + r * 3, 3 * sizeof(float));
+    LH(hipMemcpyAsync(L->bT.p, tp.data(), tp.size() * sizeof(float), hipMemcpyHostToDevice, st), "upload targets");
+    LH(hipMemsetAsync(L->grad.p, 0, mlp::NPARAM * sizeof(float), st), "clear gradient");
+    LH(mlp::forward(L->net, (const float*)L->bX.p, (int)batch, (float*)L->bY.p, true, (float*)L->bZ.p,
+                    (unsigned short*)L->bH.p, b, st), "network forward");
+    LH(mlp::backward(L->net, (const float*)L->bY.p, (const float*)L->bT.p, (int)batch, (const float*)L->bZ.p,
+                     (const unsigned short*)L->bH.p, (unsigned short*)L->bD.p, b, (float*)L->dbpart.p,
+                     (float*)L->losspart.p, (float*)L->grad.p, (float*)L->lossbuf.p, st), "network backward");
+    float lsum = 0.f;
+    LH(hipMemcpyAsync(grads, L->grad.p, mlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost, st), "download gradient");
+    LH(hipMemcpyAsync(&lsum, L->lossbuf.p, sizeof(float), hipMemcpyDeviceToHost, st), "download loss");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (loss) *loss = 0.5 * (double)lsum / (double)batch;
+    return NB_OK;
+}
+
+}  // extern "C"

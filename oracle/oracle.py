@@ -58,6 +58,14 @@ class Oracle:
         L.orc_max_rel_err.argtypes = [_SZ] + [_D] * 6 + [_D]
         L.orc_max_rel_err.restype = C.c_double
         L.orc_poly_extrap.argtypes = [_SZ, _D, _D, _D, _D, _D]
+        L.orc_mlp_param_count.restype = _SZ
+        L.orc_bf16_round.argtypes = [C.c_double]
+        L.orc_bf16_round.restype = C.c_double
+        L.orc_mlp_forward.argtypes = [_D, _SZ, _D, C.c_int, _D]
+        L.orc_mlp_grad.argtypes = [_D, _SZ, _D, _D, C.c_int, _D]
+        L.orc_mlp_grad.restype = C.c_double
+        L.orc_adam.argtypes = [_SZ, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_features.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _D]
 
     # --- initial conditions -------------------------------------------------
     def rng_stream(self, seed: int, n: int) -> np.ndarray:
@@ -147,6 +155,42 @@ class Oracle:
         mx = self.lib.orc_max_rel_err(a.shape[1], *[_p_row(a, k) for k in range(3)],
                                       *[_p_row(ref, k) for k in range(3)], C.byref(mean))
         return mx, mean.value
+
+    # --- learned extrapolator --------------------------------------------------
+    def mlp_param_count(self) -> int:
+        return self.lib.orc_mlp_param_count()
+
+    def bf16(self, v: float) -> float:
+        return self.lib.orc_bf16_round(v)
+
+    def mlp_forward(self, params, x, bf16: bool = True):
+        """x (rows, 15) -> y (rows, 3); params packed W1 b1 W2 b2 W3 b3 W4 b4."""
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty((x.shape[0], 3))
+        self.lib.orc_mlp_forward(_p(params), x.shape[0], _p(x), int(bf16), _p(y))
+        return y
+
+    def mlp_grad(self, params, x, t, bf16: bool = True):
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        g = np.empty_like(params)
+        loss = self.lib.orc_mlp_grad(_p(params), x.shape[0], _p(x), _p(t), int(bf16), _p(g))
+        return g, loss
+
+    def adam(self, p, g, m, v, lr, t, b1=0.9, b2=0.999, eps=1e-8):
+        for a in (p, g, m, v):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        self.lib.orc_adam(p.shape[0], _p(p), _p(g), _p(m), _p(v), lr, b1, b2, eps, t)
+
+    def features(self, a0, a1, a2, a3, v):
+        """(n, 3) arrays -> features (n, 15), scale (n,), cubic (n, 3)."""
+        n = a0.shape[0]
+        x, s, cub = np.empty((n, 15)), np.empty(n), np.empty((n, 3))
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (a0, a1, a2, a3, v)]
+        self.lib.orc_features(n, *[_p(a) for a in args], _p(x), _p(s), _p(cub))
+        return x, s, cub
 
     def poly_extrap(self, a0, a1, a2, a3):
         out = np.empty_like(a0)

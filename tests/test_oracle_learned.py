@@ -1,0 +1,90 @@
+"""The learned-extrapolator oracle pinned on the CPU: backward pass against
+central finite differences, Adam against its closed form, the cubic
+extrapolation against polynomials, the features' scale invariance and the
+bfloat16 rounding against known bit patterns."""
+import numpy as np
+import pytest
+
+
+def random_params(orc, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = np.empty(orc.mlp_param_count())
+    sizes = [(128, 15), (128,), (128, 128), (128,), (128, 128), (128,), (3, 128), (3,)]
+    off = 0
+    for s in sizes:
+        k = int(np.prod(s))
+        fan_in = s[1] if len(s) == 2 else 8
+        p[off:off + k] = rng.uniform(-1, 1, k) * scale / np.sqrt(fan_in)
+        off += k
+    assert off == p.shape[0] == 35459
+    return p
+
+
+def test_bf16_rounding(orc):
+    assert orc.bf16(1.0) == 1.0
+    assert orc.bf16(1.0 + 2 ** -8) == 1.0            # tie -> even
+    assert orc.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6   # tie -> even (up)
+    assert orc.bf16(1.0 + 2 ** -7) == 1.0 + 2 ** -7     # representable
+    assert orc.bf16(-3.14159) == -3.140625
+
+
+def test_backward_matches_finite_differences(orc):
+    p = random_params(orc, 3)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (6, 15))
+    t = rng.uniform(-1, 1, (6, 3))
+    g, loss = orc.mlp_grad(p, x, t, bf16=False)
+    y = orc.mlp_forward(p, x, bf16=False)
+    assert abs(loss - 0.5 * np.sum((y - t) ** 2) / 6) < 1e-14
+    idx = np.concatenate([rng.choice(1920, 6), 1920 + rng.choice(128, 3), 2048 + rng.choice(16384, 6),
+                          35072 + rng.choice(384, 4), [35456, 35457, 35458]])
+    h = 1e-6
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (orc.mlp_grad(pp, x, t, bf16=False)[1] - orc.mlp_grad(pm, x, t, bf16=False)[1]) / (2 * h)
+        assert abs(fd - g[k]) < 1e-7 * max(1.0, abs(g[k])) + 1e-9, (k, fd, g[k])
+
+
+def test_adam_closed_form(orc):
+    n = 5
+    p = np.linspace(-1, 1, n)
+    g = np.array([0.5, -0.25, 1e-3, 0.0, 2.0])
+    m, v = np.zeros(n), np.zeros(n)
+    p0 = p.copy()
+    orc.adam(p, g, m, v, 1e-2, 1)
+    # first step: m_hat = g, v_hat = g^2 -> p -= lr * sign(g) (up to eps)
+    expect = p0 - 1e-2 * g / (np.abs(g) + 1e-8)
+    np.testing.assert_allclose(p, expect, rtol=0, atol=1e-12)
+
+
+def test_cubic_extrapolation_exact_on_cubics(orc):
+    rng = np.random.default_rng(1)
+    coef = rng.standard_normal((4, 30))
+    f = lambda t: coef[0] + coef[1] * t + coef[2] * t ** 2 + coef[3] * t ** 3
+    a0, a1, a2, a3 = f(3.0), f(2.0), f(1.0), f(0.0)
+    np.testing.assert_allclose(orc.poly_extrap(a0, a1, a2, a3), f(4.0), rtol=1e-12, atol=1e-12)
+    # a quartic term is not reproduced: the residual is the 4th difference 24 c4
+    g = lambda t: t ** 4
+    assert abs(orc.poly_extrap(*[np.array([g(t)]) for t in (3.0, 2.0, 1.0, 0.0)])[0] - g(4.0)) == 24.0
+
+
+def test_features_scale_invariance(orc):
+    rng = np.random.default_rng(2)
+    a = [rng.standard_normal((50, 3)) for _ in range(4)]
+    v = rng.standard_normal((50, 3))
+    x, s, cub = orc.features(*a, v)
+    x2, s2, cub2 = orc.features(*[7.5 * ai for ai in a], 0.1 * v)
+    np.testing.assert_allclose(x2, x, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(s2, 7.5 * s, rtol=1e-12)
+    np.testing.assert_allclose(cub, 4 * a[0] - 6 * a[1] + 4 * a[2] - a[3], rtol=1e-12, atol=1e-12)
+    d3 = a[0] - 3 * a[1] + 3 * a[2] - a[3]
+    np.testing.assert_allclose(s, np.linalg.norm(d3, axis=1), rtol=1e-12)
+    assert np.all(np.abs(x) <= 1.0 + 1e-12)
+
+
+def test_features_zero_history(orc):
+    z = np.zeros((3, 3))
+    x, s, cub = orc.features(z, z, z, z, z)
+    assert np.all(x == 0) and np.all(s == 0) and np.all(cub == 0)
