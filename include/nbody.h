@@ -173,21 +173,27 @@ NB_API nb_status nb_synchronize(nb_ctx* ctx);
  * accumulate) corrects the cubic extrapolation a_{n+1} ~ 4a_n - 6a_{n-1} +
  * 4a_{n-2} - a_{n-3} from each particle's last four accelerations and its
  * velocity.  In nb_step with NB_FORM_LEARNED every `period`-th substep runs
- * the exact direct sum, compares it with the extrapolation, and while the
- * max relative acceleration error exceeds `threshold` all substeps are exact.
+ * the exact direct sum and compares it with the extrapolation; while the RMS
+ * relative acceleration error exceeds `threshold` all substeps are exact.
+ * Particles with a rough history (|third difference| > rho |a|, close
+ * encounters) always get their exact rows of the direct sum.
  * ------------------------------------------------------------------------- */
 typedef struct nb_learned_stats {
     uint64_t exact_evals;      /* direct sums run by learned-form steps */
     uint64_t predicted_evals;  /* substeps served by the extrapolator */
     uint64_t fallbacks;        /* exact checks that exceeded the threshold */
-    uint64_t subset_rows;      /* flagged particles summed exactly on extrapolated substeps */
-    double last_check_err;     /* max relative acceleration error, last check */
+    uint64_t subset_rows;      /* rough particles summed exactly on extrapolated substeps */
+    double last_check_err;     /* RMS relative acceleration error at the last check */
     double max_check_err;      /* largest such error since configure */
 } nb_learned_stats;
 
 /* period >= 2 (exact every period-th substep), threshold > 0 (0 = always
  * exact), exact_form NB_FORM_FP64 / NB_FORM_MIXED / NB_FORM_MIXED_RAW. */
 NB_API nb_status nb_learned_configure(nb_ctx* ctx, int period, double threshold, int exact_form);
+
+/* rho >= 0: roughness ratio |D3| / |a_n| above which a particle is summed
+ * exactly instead of extrapolated (default 1e-2; 0 = every particle exact). */
+NB_API nb_status nb_learned_set_flag_ratio(nb_ctx* ctx, double rho);
 
 /* 0: cubic extrapolation corrected by the network (default once trained);
  * 1: cubic extrapolation alone (the method without the network). */

@@ -9,13 +9,14 @@
 // D3/S, v/|v|) with S = |D1|+|D2|+|D3| (15 numbers) and f is the MLP of
 // mlp_kernels.hip; the network predicts the residual of the cubic in units of
 // |D3|, so it cannot disturb smooth orbits (where |D3| is tiny).  On the
-// remaining substep the exact direct sum runs, the extrapolation made for the
-// same substep is compared with it, and (the first extrapolation is used only
-// after the first passed check)
-//   * while the RMS relative error sqrt(sum|da|^2 / sum|a|^2) exceeds
-//     `threshold` (or is NaN), every substep is exact (fallback);
-//   * particles whose own relative error exceeds flag_factor * threshold get
-//     exact sums (their rows only) on the following extrapolated substeps.
+// remaining substep the exact direct sum runs and is compared with the
+// extrapolation made for the same substep.  Particles whose history is rough,
+// |D3| > rho |a_n| (close encounters), are never extrapolated: their rows of
+// the direct sum are computed exactly on every extrapolated substep.  While
+// the RMS relative error sqrt(sum|da|^2 / sum|a|^2) over the other particles
+// exceeds `threshold` (or is NaN) every substep is exact (fallback); the first
+// extrapolation is used only after a passed check, so a network producing
+// garbage never reaches the integrator.
 // Mode 1 drops the network (f = 0): the method without it.
 #include <hip/hip_runtime.h>
 
@@ -46,7 +47,7 @@ __device__ __forceinline__ const P4& hist_at(const P4* __restrict__ hist, int he
 
 // Features, cubic extrapolation and output scale of particle i.
 __device__ __forceinline__ void features(const P4* __restrict__ hist, int head, long n, long i, const P4& v,
-                                         float* x, double* cubic, double& scale) {
+                                         float* x, double* cubic, double& scale, double& anorm) {
     const P4& a0 = hist_at(hist, head, 0, n, i);
     const P4& a1 = hist_at(hist, head, 1, n, i);
     const P4& a2 = hist_at(hist, head, 2, n, i);
@@ -77,19 +78,28 @@ __device__ __forceinline__ void features(const P4* __restrict__ hist, int head, 
     }
     x[15] = 0.f;
     scale = n3;
+    anorm = n0;
 }
 
-// X[i] = features, cub[i] = (cubic, |D3|)
+// X[i] = features, cub[i] = (cubic, |D3|); with `mask`, particles whose
+// history is rough (|D3| > rho |a_n|, or not finite) are marked and listed.
 __global__ __launch_bounds__(BLK) void k_features(const P4* __restrict__ hist, int head, long n, const P4* __restrict__ vel,
-                                                  float* __restrict__ X, P4* __restrict__ cub) {
+                                                  float* __restrict__ X, P4* __restrict__ cub, double rho,
+                                                  unsigned char* __restrict__ mask, int* __restrict__ flags,
+                                                  int* __restrict__ nflags) {
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     float x[16];
-    double c[3], s;
-    features(hist, head, n, i, vel[i], x, c, s);
+    double c[3], s, a0;
+    features(hist, head, n, i, vel[i], x, c, s, a0);
     float4* xo = (float4*)(X + i * 16);
     for (int q = 0; q < 4; q++) xo[q] = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
     cub[i] = P4{c[0], c[1], c[2], s};
+    if (mask) {
+        const bool rough = !(s <= rho * a0);
+        mask[i] = rough ? 1 : 0;
+        if (rough) flags[atomicAdd(nflags, 1)] = (int)i;
+    }
 }
 
 // out = cubic + |D3| tau y  (use_net) or cubic
@@ -119,25 +129,20 @@ __global__ __launch_bounds__(BLK) void k_targets(const P4* __restrict__ cub, con
                                         (float)((a.z - c.z) * inv), 0.f);
 }
 
-// Per block: sum |pred - exact|^2, sum |exact|^2, max relative error; flag
-// particles above flag_thr (index list + counter).
-__global__ __launch_bounds__(BLK) void k_check(const P4* __restrict__ pred, const P4* __restrict__ acc, long n,
-                                               double flag_thr, double* __restrict__ out, int* __restrict__ flags,
-                                               int* __restrict__ nflags, int max_flags) {
+// Per block, over the particles not marked rough: sum |pred - exact|^2,
+// sum |exact|^2 and the max relative error.
+__global__ __launch_bounds__(BLK) void k_check(const P4* __restrict__ pred, const P4* __restrict__ acc,
+                                               const unsigned char* __restrict__ mask, long n, double* __restrict__ out) {
     __shared__ double s0[BLK], s1[BLK], s2[BLK];
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     double e2 = 0.0, a2 = 0.0, rel = 0.0;
-    if (i < n) {
+    if (i < n && !mask[i]) {
         const P4 p = pred[i], a = acc[i];
         const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
         e2 = dx * dx + dy * dy + dz * dz;
         a2 = a.x * a.x + a.y * a.y + a.z * a.z;
         rel = a2 > 0 ? sqrt(e2 / a2) : 0.0;
-        if (!(rel <= flag_thr)) {   // NaN predictions are flagged too
-            const int k = atomicAdd(nflags, 1);
-            if (k < max_flags) flags[k] = (int)i;
-            if (!(rel == rel)) rel = INFINITY;
-        }
+        if (!(rel == rel)) rel = INFINITY;
     }
     s0[threadIdx.x] = e2;
     s1[threadIdx.x] = a2;
@@ -218,7 +223,7 @@ struct Learned {
     // configuration
     int period = 4;
     double threshold = 1e-3;
-    double flag_factor = 10.0;
+    double rho = 1e-2;   // roughness ratio |D3| / |a_n| above which a particle is summed exactly
     int exact_kform = nbk::FORM_MIXED_NR;
     int mode = 0;
     // network: fp32 master parameters, Adam moments, gradient, device copies
@@ -234,7 +239,7 @@ struct Learned {
     DevBuf bX, bT, bY, bZ, bH, bD, dbpart, losspart;
     // per-particle state
     size_t n = 0;
-    DevBuf hist, X, Y, cub, pred, chk, flags, nflags, sub_p, sub_part, sub_acc;
+    DevBuf hist, X, Y, cub, pred, chk, mask, flags, nflags, sub_p, sub_part, sub_acc;
     int head = 3, count = 0;
     long since_exact = 0;
     bool fallback = true;
@@ -328,6 +333,7 @@ nb_status ensure_particles(Learned& L, size_t n) {
     LH(L.cub.ensure(m * sizeof(P4)), "allocate extrapolation");
     LH(L.pred.ensure(m * sizeof(P4)), "allocate prediction");
     LH(L.chk.ensure(((m + BLK - 1) / BLK) * 3 * sizeof(double)), "allocate check sums");
+    LH(L.mask.ensure(m), "allocate roughness mask");
     LH(L.flags.ensure(m * sizeof(int)), "allocate flags");
     LH(L.nflags.ensure(sizeof(int)), "allocate flag counter");
     L.n = n;
@@ -336,18 +342,27 @@ nb_status ensure_particles(Learned& L, size_t n) {
     return NB_OK;
 }
 
-// Extrapolated accelerations for every particle into `out`.
-nb_status predict(nb_ctx* c, Learned& L, P4* out) {
+// Extrapolated accelerations for every particle into `out`; marks and lists
+// the rough particles.  With `count`, waits for and records how many there are.
+nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
+    LH(hipMemsetAsync(L.nflags.p, 0, sizeof(int), st), "clear flag counter");
     hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, L.head, n,
-                       (const P4*)nbh::ctx_vel(c), (float*)L.X.p, (P4*)L.cub.p);
+                       (const P4*)nbh::ctx_vel(c), (float*)L.X.p, (P4*)L.cub.p, L.rho, (unsigned char*)L.mask.p,
+                       (int*)L.flags.p, (int*)L.nflags.p);
     LH(hipGetLastError(), "features kernel");
     const bool use_net = (L.mode == 0 && L.trained);
     if (use_net) LH(mlp::forward(L.net, (const float*)L.X.p, (int)n, (float*)L.Y.p, false, nullptr, nullptr, 0, st), "network forward");
     hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float*)L.Y.p, L.tau,
                        use_net ? 1 : 0, n, out);
     LH(hipGetLastError(), "combine kernel");
+    if (count) {
+        int nf = 0;
+        LH(hipMemcpyAsync(&nf, L.nflags.p, sizeof(int), hipMemcpyDeviceToHost, st), "download flag count");
+        LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+        L.nflag = std::min<int>(std::max(nf, 0), (int)n);
+    }
     return NB_OK;
 }
 
@@ -389,21 +404,17 @@ nb_status exact_subset(nb_ctx* c, Learned& L, double eps, int kform) {
     return NB_OK;
 }
 
-// Compare the extrapolation in L.pred with the exact acc; update stats, the
-// fallback state and the flag list.  Synchronises the stream.
+// Compare the extrapolation in L.pred with the exact acc over the smooth
+// particles; update stats and the fallback state.  Synchronises the stream.
 nb_status check(nb_ctx* c, Learned& L) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
     const long nb = (n + BLK - 1) / BLK;
-    LH(hipMemsetAsync(L.nflags.p, 0, sizeof(int), st), "clear flag counter");
     hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, (const P4*)L.pred.p,
-                       (const P4*)nbh::ctx_acc(c), n, L.flag_factor * L.threshold, (double*)L.chk.p, (int*)L.flags.p,
-                       (int*)L.nflags.p, (int)n);
+                       (const P4*)nbh::ctx_acc(c), (const unsigned char*)L.mask.p, n, (double*)L.chk.p);
     LH(hipGetLastError(), "check kernel");
     L.chk_h.resize(3 * nb);
-    int nf = 0;
     LH(hipMemcpyAsync(L.chk_h.data(), L.chk.p, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, st), "download check");
-    LH(hipMemcpyAsync(&nf, L.nflags.p, sizeof(int), hipMemcpyDeviceToHost, st), "download flag count");
     LH(hipStreamSynchronize(st), "hipStreamSynchronize");
     double e2 = 0.0, a2 = 0.0, mx = 0.0;
     for (long b = 0; b < nb; b++) {
@@ -417,7 +428,6 @@ nb_status check(nb_ctx* c, Learned& L) {
     const bool ok = (err <= L.threshold);   // NaN fails
     if (!ok) L.st.fallbacks++;
     L.fallback = !ok;
-    L.nflag = ok ? std::min<int>(nf, (int)n) : 0;
     return NB_OK;
 }
 
@@ -440,7 +450,7 @@ nb_status record(nb_ctx* c, Learned& L) {
     float* X = (float*)L.dsX.p + L.ds_rows * 16;
     float* T = (float*)L.dsT.p + L.ds_rows * 4;
     hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, L.head, n,
-                       (const P4*)nbh::ctx_vel(c), X, (P4*)L.cub.p);
+                       (const P4*)nbh::ctx_vel(c), X, (P4*)L.cub.p, 0.0, nullptr, nullptr, nullptr);
     LH(hipGetLastError(), "features kernel");
     hipLaunchKernelGGL(k_targets, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const P4*)nbh::ctx_acc(c), n, T);
     LH(hipGetLastError(), "targets kernel");
@@ -485,7 +495,7 @@ void destroy(Learned* l) {
     if (!l) return;
     for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
                       &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub,
-                      &l->pred, &l->chk, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc})
+                      &l->pred, &l->chk, &l->mask, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc})
         b->release();
     delete l;
 }
@@ -515,7 +525,7 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
             const bool exact = !full || L.fallback || (L.since_exact + 1 >= L.period);
             if (exact) {
                 if (full) {
-                    if (nb_status e = predict(c, L, (P4*)L.pred.p)) return e;
+                    if (nb_status e = predict(c, L, (P4*)L.pred.p, false)) return e;
                 }
                 if (nb_status e = exact_force(c, L.exact_kform, eps)) return e;
                 L.st.exact_evals++;
@@ -526,7 +536,7 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 }
                 L.since_exact = 0;
             } else {
-                if (nb_status e = predict(c, L, nbh::ctx_acc(c))) return e;
+                if (nb_status e = predict(c, L, nbh::ctx_acc(c), true)) return e;
                 if (nb_status e = exact_subset(c, L, eps, L.exact_kform)) return e;
                 L.st.predicted_evals++;
                 L.st.subset_rows += (uint64_t)L.nflag;
@@ -559,6 +569,15 @@ nb_status nb_learned_configure(nb_ctx* c, int period, double threshold, int exac
     L->exact_kform = kf;
     L->st = nb_learned_stats{};
     nbl::on_new_state(L);
+    return NB_OK;
+}
+
+nb_status nb_learned_set_flag_ratio(nb_ctx* c, double rho) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_flag_ratio: null context");
+    if (!(rho >= 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_flag_ratio: ratio must be >= 0");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_set_flag_ratio: out of host memory");
+    L->rho = rho;
     return NB_OK;
 }
 

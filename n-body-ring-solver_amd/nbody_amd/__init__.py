@@ -37,7 +37,7 @@ EXPORTS = (
     "nb_timing_read", "nb_timing_reset", "nb_synchronize", "nb_learned_configure",
     "nb_learned_set_mode", "nb_learned_collect", "nb_learned_train", "nb_learned_stats_read",
     "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
-    "nb_set_kernel", "nb_learned_reset",
+    "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -93,6 +93,7 @@ def load_library(path: str = LIB_PATH):
     L.nb_learned_configure.argtypes = [_VP, C.c_int, C.c_double, C.c_int]
     L.nb_learned_set_mode.argtypes = [_VP, C.c_int]
     L.nb_learned_reset.argtypes = [_VP, C.c_uint64]
+    L.nb_learned_set_flag_ratio.argtypes = [_VP, C.c_double]
     L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
     L.nb_learned_train.argtypes = [_VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]
     L.nb_learned_stats_read.argtypes = [_VP, C.POINTER(LearnedStats)]
@@ -269,6 +270,9 @@ class Solver:
 
     def learned_reset(self, seed: int = 1):
         _check(self.L.nb_learned_reset(self.h, seed), "nb_learned_reset")
+
+    def learned_set_flag_ratio(self, rho: float):
+        _check(self.L.nb_learned_set_flag_ratio(self.h, rho), "nb_learned_set_flag_ratio")
 
     def learned_set_mode(self, mode: int):
         _check(self.L.nb_learned_set_mode(self.h, mode), "nb_learned_set_mode")

@@ -66,12 +66,12 @@ def test_mlp_gradient_descends(solver):
 
 # --- learned stepping ----------------------------------------------------------
 
-def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, flag_factor, params=None, tau=1.0):
+def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, params=None, tau=1.0):
     """numpy restatement of the learned form's schedule (order-2 DKD)."""
     pos, vel = pos.copy(), vel.copy()
     hist = []
-    since, fallback, flags = 0, True, np.zeros(0, dtype=int)
-    st = {"exact": 0, "predicted": 0}
+    since, fallback = 0, True
+    st = {"exact": 0, "predicted": 0, "subset": 0}
 
     def predict():
         a0, a1, a2, a3 = (hist[-1].T, hist[-2].T, hist[-3].T, hist[-4].T)
@@ -79,31 +79,30 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, flag_f
         if params is not None:
             y = orc.mlp_forward(params, x, bf16=True)
             cub = cub + (s * tau)[:, None] * y
-        return cub.T
+        rough = ~(s <= rho * np.linalg.norm(a0, axis=1))
+        return cub.T, rough
 
     pos += 0.5 * dt * vel
     for t in range(steps):
         full = len(hist) == 4
         exact = (not full) or fallback or since + 1 >= period
         if exact:
-            pred = predict() if full else None
             acc = orc.accel(m, pos, eps)
             st["exact"] += 1
             if full:
-                e2 = np.sum((pred - acc) ** 2)
-                a2 = np.sum(acc ** 2)
-                err = np.sqrt(e2 / a2)
-                ok = err <= threshold
-                fallback = not ok
-                rel = np.linalg.norm(pred - acc, axis=0) / np.linalg.norm(acc, axis=0)
-                flags = np.nonzero(~(rel <= flag_factor * threshold))[0] if ok else np.zeros(0, dtype=int)
+                pred, rough = predict()
+                smooth = ~rough
+                e2 = np.sum((pred - acc)[:, smooth] ** 2)
+                a2 = np.sum(acc[:, smooth] ** 2)
+                fallback = not (np.sqrt(e2 / a2) <= threshold)
             since = 0
         else:
-            acc = predict()
-            if flags.size:
+            acc, rough = predict()
+            if rough.any():
                 ex = orc.accel(m, pos, eps)
-                acc[:, flags] = ex[:, flags]
+                acc[:, rough] = ex[:, rough]
             st["predicted"] += 1
+            st["subset"] += int(rough.sum())
             since += 1
         hist.append(acc.copy())
         hist = hist[-4:]
@@ -133,13 +132,15 @@ def test_cubic_mode_matches_restatement(solver, orc):
     steps, dt, eps = 30, 1.0 / 256, 0.05
     solver.learned_configure(period=4, threshold=1e-3, exact_form="fp64")
     solver.learned_set_mode(1)
+    solver.learned_set_flag_ratio(2e-3)
     solver.set_state(m, pos, vel)
     solver.step(steps, dt, 2, "learned", eps)
     p, v = solver.get_state()
     st = solver.learned_stats()
-    pr, vr, sr = restated_learned(orc, m, pos, vel, steps, dt, eps, 4, 1e-3, 10.0)
+    pr, vr, sr = restated_learned(orc, m, pos, vel, steps, dt, eps, 4, 1e-3, 2e-3)
     assert st["exact_evals"] == sr["exact"] and st["predicted_evals"] == sr["predicted"]
-    assert sr["predicted"] > 0
+    assert st["subset_rows"] == sr["subset"]
+    assert sr["predicted"] > 0 and sr["subset"] > 0
     assert np.max(np.abs(p - pr)) < 1e-11
     assert np.max(np.abs(v - vr)) < 1e-9 * np.max(np.abs(vr))
 
