@@ -195,6 +195,24 @@ NB_API nb_status nb_learned_configure(nb_ctx* ctx, int period, double threshold,
  * exactly instead of extrapolated (default 1e-2; 0 = every particle exact). */
 NB_API nb_status nb_learned_set_flag_ratio(nb_ctx* ctx, double rho);
 
+/* kappa > 0: a particle whose extrapolation missed the exact acceleration at
+ * a check by more than kappa times the RMS acceleration is summed exactly for
+ * the next 4 substeps (default 1e-3; INFINITY never holds on this ground). */
+NB_API nb_status nb_learned_set_hold_ratio(nb_ctx* ctx, double kappa);
+
+/* rc >= 0: near-field radius.  With rc > 0 pairs closer than rc (tapered
+ * smoothly from rc/2) are summed exactly on every substep from a sorted cell
+ * grid and only the remainder a - a_near is extrapolated.  0 (default): the
+ * whole acceleration is extrapolated. */
+NB_API nb_status nb_learned_set_near_radius(nb_ctx* ctx, double rc);
+
+/* Near-field accelerations of the resident state, as used by the split:
+ * acc_i = sum_{j : 0 < r_ij < rc} m_j S(r_ij) d_ij / (r_ij^2 + eps^2)^(3/2),
+ * S = 1 for r <= rc/2, 0 for r >= rc, C2 quintic in r^2/rc^2 between.
+ * form NB_FORM_FP64: FP64 pair arithmetic; a mixed form: FP32 pair arithmetic
+ * on FP64 differences (what the learned form uses with that exact form). */
+NB_API nb_status nb_near_accel(nb_ctx* ctx, int form, double rc, double eps, double* const acc[3]);
+
 /* 0: cubic extrapolation corrected by the network (default once trained);
  * 1: cubic extrapolation alone (the method without the network). */
 NB_API nb_status nb_learned_set_mode(nb_ctx* ctx, int mode);

@@ -172,9 +172,9 @@ static hipError_t sym_plan(nb_ctx* c, int K) {
 // stays within 16 GiB, the tile kernel otherwise.
 static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     const long n = long(c->n);
-    // 8 particles per lane for FP64 at large N; the mixed form keeps FP32 and
-    // FP64 sums per particle, so 8 would not fit two waves per SIMD.
-    K = (n >= 131072 && kform == nbk::FORM_FP64) ? 8 : 4;
+    // 8 particles per lane at large N (fewer LDS reads and rotations per
+    // pair; the mixed loop is left unrolled once so 8 still fits the VGPRs).
+    K = (n >= 131072) ? 8 : 4;
     const long S = 64L * K;
     const long nB = (n + S - 1) / S;
     const bool fits = double(nB) * nB * S * sizeof(P4) <= 16.0 * (1L << 30);

@@ -1,22 +1,42 @@
 // nbody_learned.hip -- the learned acceleration extrapolator (NB_FORM_LEARNED).
 //
 // No counterpart in the reference.  Every force evaluation of the leapfrog is
-// a "substep".  Each particle keeps its last four accelerations a_n..a_{n-3}
-// (the ones the integrator used).  On three of every `period` substeps the
-// acceleration is extrapolated,
-//     a_{n+1} ~ cubic + |D3| tau f(x),   cubic = 4a_n - 6a_{n-1} + 4a_{n-2} - a_{n-3},
-// where D1..D3 are the backward differences, x = (a_n/|a_n|, D1/S, D2/S,
-// D3/S, v/|v|) with S = |D1|+|D2|+|D3| (15 numbers) and f is the MLP of
-// mlp_kernels.hip; the network predicts the residual of the cubic in units of
-// |D3|, so it cannot disturb smooth orbits (where |D3| is tiny).  On the
-// remaining substep the exact direct sum runs and is compared with the
-// extrapolation made for the same substep.  Particles whose history is rough,
-// |D3| > rho |a_n| (close encounters), are never extrapolated: their rows of
-// the direct sum are computed exactly on every extrapolated substep.  While
-// the RMS relative error sqrt(sum|da|^2 / sum|a|^2) over the other particles
-// exceeds `threshold` (or is NaN) every substep is exact (fallback); the first
-// extrapolation is used only after a passed check, so a network producing
-// garbage never reaches the integrator.
+// a "substep".  Every `period`-th substep is exact (the configured direct sum);
+// the others are extrapolated from the last four EXACT samples a_0..a_3 of
+// each particle (newest first, taken at substeps t_0 > t_1 > t_2 > t_3):
+//     a(t) ~ C(t) + |D3| tau w(t) f(x),   C = sum_k l_k(t) a_k,
+// where l_k are the Lagrange weights of the cubic through the four samples at
+// their substeps, D1..D3 the backward differences of the samples, x =
+// (a_0/|a_0|, D1/S, D2/S, D3/S, v/|v|) with S = |D1|+|D2|+|D3| (15 numbers)
+// and f the MLP of mlp_kernels.hip.  The network is trained on the residual of
+// the cubic one period after the newest sample of a history spaced `period`
+// substeps apart, in units of |D3| tau; at an earlier substep t = t_0 + k that
+// residual is taken in proportion to the nodal polynomial,
+// w = prod_k (t - t_k) / prod_k (t_0 + period - t_k), the shape of the
+// interpolation error.  The network is applied only to such evenly spaced
+// histories (after a fallback the spacing is uneven for a few periods and
+// the cubic runs alone).
+// Only exact samples enter the history, so extrapolation errors never feed
+// back into later extrapolations: the scheme is stable however long it runs.
+// (Extrapolating from extrapolated samples evaluates one polynomial further
+// and further ahead; the history errors it leaves grow geometrically from
+// cycle to cycle.)
+// On an exact substep the extrapolation for the same substep is compared with
+// the exact result.  While the RMS relative error sqrt(sum|da|^2 / sum|a|^2)
+// over the checked particles exceeds `threshold` (or is NaN) every substep is
+// exact (fallback); the first extrapolation is used only after a passed check.
+// The check is made for the cubic alone and, when the network applies, with
+// it; the network is used in the next cycle only if it passed, so a network
+// producing garbage never reaches the integrator (the cubic carries on).  Particles
+// whose history is rough, |D3| > rho |a_0| (close encounters), or whose
+// extrapolation missed at the last check by more than kappa a_rms are not
+// extrapolated: their rows of the direct sum are computed exactly on every
+// extrapolated substep, and they are left out of the check.
+// With a near-field radius rc > 0 the force is split: pairs closer than rc
+// (tapered from rc/2, nbody_near.hip) are summed exactly on every substep
+// from a sorted cell grid, and only the smooth remainder a - a_near is
+// extrapolated (the history holds it), which removes the close-encounter
+// noise the extrapolation cannot follow.
 // Mode 1 drops the network (f = 0): the method without it.
 #include <hip/hip_runtime.h>
 
@@ -30,6 +50,7 @@
 #include "mlp_kernels.h"
 #include "nbody_host.h"
 #include "nbody_learned.h"
+#include "nbody_near.h"
 
 using nbk::P4;
 using nbh::DevBuf;
@@ -40,18 +61,14 @@ namespace {
 
 constexpr int BLK = 256;
 
-// history slot k steps back from head (k = 0 newest)
-__device__ __forceinline__ const P4& hist_at(const P4* __restrict__ hist, int head, int k, long n, long i) {
-    return hist[(long)((head - k + 4) & 3) * n + i];
-}
-
-// Features, cubic extrapolation and output scale of particle i.
-__device__ __forceinline__ void features(const P4* __restrict__ hist, int head, long n, long i, const P4& v,
-                                         float* x, double* cubic, double& scale, double& anorm) {
-    const P4& a0 = hist_at(hist, head, 0, n, i);
-    const P4& a1 = hist_at(hist, head, 1, n, i);
-    const P4& a2 = hist_at(hist, head, 2, n, i);
-    const P4& a3 = hist_at(hist, head, 3, n, i);
+// Features, cubic (weights w0..w3 on the samples in slots s0..s3, newest
+// first) and output scale |D3| of particle i.
+__device__ __forceinline__ void features(const P4* __restrict__ hist, int4 slot, double4 w, long n, long i,
+                                         const P4& v, float* x, double* cubic, double& scale, double& anorm) {
+    const P4& a0 = hist[(long)slot.x * n + i];
+    const P4& a1 = hist[(long)slot.y * n + i];
+    const P4& a2 = hist[(long)slot.z * n + i];
+    const P4& a3 = hist[(long)slot.w * n + i];
     const double A0[3] = {a0.x, a0.y, a0.z}, A1[3] = {a1.x, a1.y, a1.z}, A2[3] = {a2.x, a2.y, a2.z},
                  A3[3] = {a3.x, a3.y, a3.z};
     double d1[3], d2[3], d3[3];
@@ -59,7 +76,7 @@ __device__ __forceinline__ void features(const P4* __restrict__ hist, int head, 
         d1[k] = A0[k] - A1[k];
         d2[k] = A0[k] - 2.0 * A1[k] + A2[k];
         d3[k] = A0[k] - 3.0 * A1[k] + 3.0 * A2[k] - A3[k];
-        cubic[k] = A0[k] + d1[k] + d2[k] + d3[k];
+        cubic[k] = w.x * A0[k] + w.y * A1[k] + w.z * A2[k] + w.w * A3[k];
     }
     const double n0 = sqrt(A0[0] * A0[0] + A0[1] * A0[1] + A0[2] * A0[2]);
     const double n1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
@@ -82,21 +99,23 @@ __device__ __forceinline__ void features(const P4* __restrict__ hist, int head, 
 }
 
 // X[i] = features, cub[i] = (cubic, |D3|); with `mask`, particles whose
-// history is rough (|D3| > rho |a_n|, or not finite) are marked and listed.
-__global__ __launch_bounds__(BLK) void k_features(const P4* __restrict__ hist, int head, long n, const P4* __restrict__ vel,
-                                                  float* __restrict__ X, P4* __restrict__ cub, double rho,
-                                                  unsigned char* __restrict__ mask, int* __restrict__ flags,
-                                                  int* __restrict__ nflags) {
+// history is rough (|D3| > rho |a_0|, or not finite) or that are held after a
+// missed check are marked and listed.
+__global__ __launch_bounds__(BLK) void k_features(const P4* __restrict__ hist, int4 slot, double4 w, long n,
+                                                  const P4* __restrict__ vel, float* __restrict__ X,
+                                                  P4* __restrict__ cub, double rho, unsigned char* __restrict__ mask,
+                                                  int* __restrict__ flags, int* __restrict__ nflags,
+                                                  const unsigned char* __restrict__ held) {
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     float x[16];
     double c[3], s, a0;
-    features(hist, head, n, i, vel[i], x, c, s, a0);
+    features(hist, slot, w, n, i, vel[i], x, c, s, a0);
     float4* xo = (float4*)(X + i * 16);
     for (int q = 0; q < 4; q++) xo[q] = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
     cub[i] = P4{c[0], c[1], c[2], s};
     if (mask) {
-        const bool rough = !(s <= rho * a0);
+        const bool rough = !(s <= rho * a0) || held[i] != 0;
         mask[i] = rough ? 1 : 0;
         if (rough) flags[atomicAdd(nflags, 1)] = (int)i;
     }
@@ -118,27 +137,34 @@ __global__ __launch_bounds__(BLK) void k_combine(const P4* __restrict__ cub, con
     out[i] = P4{c.x, c.y, c.z, 0.0};
 }
 
-// Training pair: T[row] = (a_exact - cubic) / |D3|  (0 where |D3| = 0)
-__global__ __launch_bounds__(BLK) void k_targets(const P4* __restrict__ cub, const P4* __restrict__ acc, long n,
-                                                 float* __restrict__ T) {
+// Training pair: T[row] = (a_exact - a_near - cubic) / |D3|  (0 where |D3| = 0)
+__global__ __launch_bounds__(BLK) void k_targets(const P4* __restrict__ cub, const P4* __restrict__ acc,
+                                                 const P4* __restrict__ near, long n, float* __restrict__ T) {
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
-    const P4 c = cub[i], a = acc[i];
+    const P4 c = cub[i];
+    P4 a = acc[i];
+    if (near) {
+        const P4 q = near[i];
+        a.x -= q.x; a.y -= q.y; a.z -= q.z;
+    }
     const double inv = c.m > 0 ? 1.0 / c.m : 0.0;
     *(float4*)(T + i * 4) = make_float4((float)((a.x - c.x) * inv), (float)((a.y - c.y) * inv),
                                         (float)((a.z - c.z) * inv), 0.f);
 }
 
-// Per block, over the particles not marked rough: sum |pred - exact|^2,
-// sum |exact|^2 and the max relative error.
+// Per block, over the particles not marked rough: sum |pred - (acc - near)|^2,
+// sum |acc|^2 and the max relative error (pred extrapolates acc - near).
 __global__ __launch_bounds__(BLK) void k_check(const P4* __restrict__ pred, const P4* __restrict__ acc,
-                                               const unsigned char* __restrict__ mask, long n, double* __restrict__ out) {
+                                               const P4* __restrict__ near, const unsigned char* __restrict__ mask,
+                                               long n, double* __restrict__ out) {
     __shared__ double s0[BLK], s1[BLK], s2[BLK];
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     double e2 = 0.0, a2 = 0.0, rel = 0.0;
     if (i < n && !mask[i]) {
         const P4 p = pred[i], a = acc[i];
-        const double dx = p.x - a.x, dy = p.y - a.y, dz = p.z - a.z;
+        const P4 q = near ? near[i] : P4{0.0, 0.0, 0.0, 0.0};
+        const double dx = p.x + q.x - a.x, dy = p.y + q.y - a.y, dz = p.z + q.z - a.z;
         e2 = dx * dx + dy * dy + dz * dz;
         a2 = a.x * a.x + a.y * a.y + a.z * a.z;
         rel = a2 > 0 ? sqrt(e2 / a2) : 0.0;
@@ -161,6 +187,41 @@ __global__ __launch_bounds__(BLK) void k_check(const P4* __restrict__ pred, cons
         out[3 * blockIdx.x + 1] = s1[0];
         out[3 * blockIdx.x + 2] = s2[0];
     }
+}
+
+// After a check: held[i] = whether particle i's extrapolation missed the
+// exact remainder by more than sqrt(thr2) (NaN counts as a miss).
+__global__ __launch_bounds__(BLK) void k_hold(const P4* __restrict__ pred, const P4* __restrict__ acc,
+                                              const P4* __restrict__ near, long n, double thr2,
+                                              unsigned char* __restrict__ held) {
+    const long i = (long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const P4 p = pred[i], a = acc[i];
+    const P4 q = near ? near[i] : P4{0.0, 0.0, 0.0, 0.0};
+    const double dx = a.x - q.x - p.x, dy = a.y - q.y - p.y, dz = a.z - q.z - p.z;
+    held[i] = (dx * dx + dy * dy + dz * dz <= thr2) ? 0 : 1;
+}
+
+// dst = a - b (b may be null: dst = a)
+__global__ __launch_bounds__(BLK) void k_sub(const P4* __restrict__ a, const P4* __restrict__ b, long n,
+                                             P4* __restrict__ dst) {
+    const long i = (long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    P4 v = a[i];
+    if (b) {
+        const P4 q = b[i];
+        v.x -= q.x; v.y -= q.y; v.z -= q.z;
+    }
+    dst[i] = P4{v.x, v.y, v.z, 0.0};
+}
+
+// dst = a + b
+__global__ __launch_bounds__(BLK) void k_add(const P4* __restrict__ a, const P4* __restrict__ b, long n,
+                                             P4* __restrict__ dst) {
+    const long i = (long)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const P4 u = a[i], v = b[i];
+    dst[i] = P4{u.x + v.x, u.y + v.y, u.z + v.z, 0.0};
 }
 
 __global__ __launch_bounds__(BLK) void k_gather(const P4* __restrict__ src, const int* __restrict__ idx, int m,
@@ -224,6 +285,9 @@ struct Learned {
     int period = 4;
     double threshold = 1e-3;
     double rho = 1e-2;   // roughness ratio |D3| / |a_n| above which a particle is summed exactly
+    double kappa = 1e-3; // miss at a check, in units of the RMS acceleration, above which a particle is held
+    double rc = 0.0;     // near-field radius: the history holds a - a_near (0: no split)
+    nbn::Near near;
     int exact_kform = nbk::FORM_MIXED_NR;
     int mode = 0;
     // network: fp32 master parameters, Adam moments, gradient, device copies
@@ -239,10 +303,15 @@ struct Learned {
     DevBuf bX, bT, bY, bZ, bH, bD, dbpart, losspart;
     // per-particle state
     size_t n = 0;
-    DevBuf hist, X, Y, cub, pred, chk, mask, flags, nflags, sub_p, sub_part, sub_acc;
-    int head = 3, count = 0;
+    DevBuf hist, X, Y, cub, pred, chk, mask, held, flags, nflags, sub_p, sub_part, sub_acc, nearacc;
+    int head = 3, count = 0;   // ring of the last four exact samples (head newest)
+    long htime[4] = {0, 0, 0, 0};   // their substep indices
+    long sub = 0;              // substeps since the state was set
     long since_exact = 0;
     bool fallback = true;
+    bool pred_net = false;     // the last extrapolation applied the network
+    bool net_ok = false;       // the network passed the last check: use it until the next
+    bool clear_held = true;    // held to be cleared before the next substep
     int nflag = 0;
     std::vector<double> chk_h;
     nb_learned_stats st{};
@@ -334,6 +403,9 @@ nb_status ensure_particles(Learned& L, size_t n) {
     LH(L.pred.ensure(m * sizeof(P4)), "allocate prediction");
     LH(L.chk.ensure(((m + BLK - 1) / BLK) * 3 * sizeof(double)), "allocate check sums");
     LH(L.mask.ensure(m), "allocate roughness mask");
+    LH(L.held.ensure(m), "allocate holds");
+    L.clear_held = true;
+    LH(L.nearacc.ensure(m * sizeof(P4)), "allocate near-field accelerations");
     LH(L.flags.ensure(m * sizeof(int)), "allocate flags");
     LH(L.nflags.ensure(sizeof(int)), "allocate flag counter");
     L.n = n;
@@ -342,20 +414,67 @@ nb_status ensure_particles(Learned& L, size_t n) {
     return NB_OK;
 }
 
-// Extrapolated accelerations for every particle into `out`; marks and lists
-// the rough particles.  With `count`, waits for and records how many there are.
-nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count) {
+// Lagrange weights at substep t of the cubic through the history samples
+// (newest first), and the network weight: the nodal polynomial at t over its
+// value one period after the newest sample, when the samples are spaced
+// exactly one period apart (else 0: the network is not applied).
+struct Weights {
+    double l[4];
+    double wnet;
+};
+
+Weights weights(const Learned& L, long t) {
+    Weights W{};
+    double tk[4];
+    for (int k = 0; k < 4; k++) tk[k] = (double)L.htime[(L.head - k + 4) & 3];
+    const double x = (double)t;
+    for (int k = 0; k < 4; k++) {
+        double num = 1.0, den = 1.0;
+        for (int j = 0; j < 4; j++)
+            if (j != k) {
+                num *= x - tk[j];
+                den *= tk[k] - tk[j];
+            }
+        W.l[k] = num / den;
+    }
+    const double P = (double)L.period;
+    bool even = true;
+    for (int k = 0; k < 3; k++) even = even && (tk[k] - tk[k + 1] == P);
+    if (even) {
+        double num = 1.0, den = 1.0;
+        for (int k = 0; k < 4; k++) {
+            num *= x - tk[k];
+            den *= tk[0] + P - tk[k];
+        }
+        W.wnet = num / den;
+    }
+    return W;
+}
+
+inline int4 ring_slots(const Learned& L) {
+    return make_int4(L.head, (L.head + 3) & 3, (L.head + 2) & 3, (L.head + 1) & 3);
+}
+
+// Extrapolated accelerations (remainders, with the split) of every particle
+// for the current substep into `out` (the cubic alone stays in L.cub); marks
+// and lists the particles to be summed exactly.  The network is applied when
+// `allow_net`, it is on and trained, and the history is evenly spaced (noted
+// in L.pred_net).  With `count`, waits for and records how many are marked.
+nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
+    const Weights W = weights(L, L.sub);
     LH(hipMemsetAsync(L.nflags.p, 0, sizeof(int), st), "clear flag counter");
-    hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, L.head, n,
-                       (const P4*)nbh::ctx_vel(c), (float*)L.X.p, (P4*)L.cub.p, L.rho, (unsigned char*)L.mask.p,
-                       (int*)L.flags.p, (int*)L.nflags.p);
+    hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, ring_slots(L),
+                       make_double4(W.l[0], W.l[1], W.l[2], W.l[3]), n, (const P4*)nbh::ctx_vel(c), (float*)L.X.p,
+                       (P4*)L.cub.p, L.rho, (unsigned char*)L.mask.p, (int*)L.flags.p, (int*)L.nflags.p,
+                       (const unsigned char*)L.held.p);
     LH(hipGetLastError(), "features kernel");
-    const bool use_net = (L.mode == 0 && L.trained);
+    const bool use_net = allow_net && L.mode == 0 && L.trained && W.wnet != 0.0;
+    L.pred_net = use_net;
     if (use_net) LH(mlp::forward(L.net, (const float*)L.X.p, (int)n, (float*)L.Y.p, false, nullptr, nullptr, 0, st), "network forward");
-    hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float*)L.Y.p, L.tau,
-                       use_net ? 1 : 0, n, out);
+    hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float*)L.Y.p,
+                       (float)(L.tau * W.wnet), use_net ? 1 : 0, n, out);
     LH(hipGetLastError(), "combine kernel");
     if (count) {
         int nf = 0;
@@ -406,53 +525,96 @@ nb_status exact_subset(nb_ctx* c, Learned& L, double eps, int kform) {
 
 // Compare the extrapolation in L.pred with the exact acc over the smooth
 // particles; update stats and the fallback state.  Synchronises the stream.
-nb_status check(nb_ctx* c, Learned& L) {
+nb_status check_sums(nb_ctx* c, Learned& L, const P4* pred, double& e2, double& a2) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
     const long nb = (n + BLK - 1) / BLK;
-    hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, (const P4*)L.pred.p,
-                       (const P4*)nbh::ctx_acc(c), (const unsigned char*)L.mask.p, n, (double*)L.chk.p);
+    hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, pred, (const P4*)nbh::ctx_acc(c),
+                       L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, (const unsigned char*)L.mask.p, n,
+                       (double*)L.chk.p);
     LH(hipGetLastError(), "check kernel");
     L.chk_h.resize(3 * nb);
     LH(hipMemcpyAsync(L.chk_h.data(), L.chk.p, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, st), "download check");
     LH(hipStreamSynchronize(st), "hipStreamSynchronize");
-    double e2 = 0.0, a2 = 0.0, mx = 0.0;
+    e2 = 0.0;
+    a2 = 0.0;
     for (long b = 0; b < nb; b++) {
         e2 += L.chk_h[3 * b];
         a2 += L.chk_h[3 * b + 1];
-        mx = std::max(mx, L.chk_h[3 * b + 2]);
     }
-    const double err = a2 > 0 ? std::sqrt(e2 / a2) : (e2 > 0 ? INFINITY : 0.0);
-    L.st.last_check_err = err;
-    L.st.max_check_err = std::max(L.st.max_check_err, std::isnan(err) ? INFINITY : err);
-    const bool ok = (err <= L.threshold);   // NaN fails
-    if (!ok) L.st.fallbacks++;
-    L.fallback = !ok;
     return NB_OK;
 }
 
-// Push acc into the history ring.
+// Compare the extrapolations made for this substep (the cubic alone in
+// L.cub, and with the network in L.pred when it was applied) with the exact
+// acc over the particles not marked; update stats, the fallback state and
+// whether the network may be used in the next cycle (only after it passed a
+// check itself), and hold the particles that missed.  Synchronises the stream.
+nb_status check(nb_ctx* c, Learned& L) {
+    const long n = (long)L.n;
+    // nothing checked (every particle rough or held, or all at rest) validates nothing
+    auto rel = [](double e2, double a2) { return a2 > 0 ? std::sqrt(e2 / a2) : INFINITY; };
+    double e2c, a2;
+    if (nb_status e = check_sums(c, L, (const P4*)L.cub.p, e2c, a2)) return e;
+    const double err_c = rel(e2c, a2);
+    double err_n = INFINITY;
+    if (L.pred_net) {
+        double e2n, a2n;
+        if (nb_status e = check_sums(c, L, (const P4*)L.pred.p, e2n, a2n)) return e;
+        err_n = rel(e2n, a2n);
+    }
+    L.net_ok = L.pred_net && (err_n <= L.threshold);   // NaN fails
+    const double err = L.net_ok ? err_n : err_c;
+    L.st.last_check_err = err;
+    L.st.max_check_err = std::max(L.st.max_check_err, std::isnan(err) ? INFINITY : err);
+    const bool ok = (err <= L.threshold);
+    if (!ok) L.st.fallbacks++;
+    L.fallback = !ok;
+    // hold the particles that missed by more than kappa a_rms
+    const double thr2 = L.kappa * L.kappa * a2 / (double)std::max<long>(n, 1);
+    hipLaunchKernelGGL(k_hold, blocks(n), dim3(BLK), 0, nbh::ctx_stream(c),
+                       L.net_ok ? (const P4*)L.pred.p : (const P4*)L.cub.p, (const P4*)nbh::ctx_acc(c),
+                       L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, n, thr2, (unsigned char*)L.held.p);
+    LH(hipGetLastError(), "hold kernel");
+    return NB_OK;
+}
+
+// Push the exact acc (minus the near field) into the history ring.
 nb_status push_history(nb_ctx* c, Learned& L) {
     L.head = (L.head + 1) & 3;
-    LH(hipMemcpyAsync((P4*)L.hist.p + (long)L.head * L.n, nbh::ctx_acc(c), L.n * sizeof(P4), hipMemcpyDeviceToDevice,
-                      nbh::ctx_stream(c)),
-       "push history");
+    L.htime[L.head] = L.sub;
+    const long n = (long)L.n;
+    hipLaunchKernelGGL(k_sub, blocks(n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)nbh::ctx_acc(c),
+                       L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, n, (P4*)L.hist.p + (long)L.head * n);
+    LH(hipGetLastError(), "history kernel");
     L.count = std::min(L.count + 1, 4);
     return NB_OK;
 }
 
-// Append this substep's (features, target) pairs: features from the history
-// before the exact acceleration is pushed.
-nb_status record(nb_ctx* c, Learned& L) {
+// Near-field accelerations of the current positions (when the split is on).
+nb_status near_field(nb_ctx* c, Learned& L, double eps) {
+    if (!(L.rc > 0)) return NB_OK;
+    LH(L.near.near_accel(nbh::ctx_pm(c), (int)L.n, L.rc, eps, (P4*)L.nearacc.p, nbh::ctx_stream(c),
+                         L.exact_kform != nbk::FORM_FP64),
+       "near-field kernels");
+    return NB_OK;
+}
+
+// Append this substep's (features, target) pairs: features from the exact
+// samples one to four periods back in the collection ring (R slots, sample t
+// in slot t mod R), target the exact remainder now.
+nb_status record(nb_ctx* c, Learned& L, const DevBuf& ring, long R, long t) {
     hipStream_t st = nbh::ctx_stream(c);
-    const long n = (long)L.n;
+    const long n = (long)L.n, P = L.period;
     if (L.ds_rows + n > L.ds_cap) return NB_OK;
     float* X = (float*)L.dsX.p + L.ds_rows * 16;
     float* T = (float*)L.dsT.p + L.ds_rows * 4;
-    hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, L.head, n,
-                       (const P4*)nbh::ctx_vel(c), X, (P4*)L.cub.p, 0.0, nullptr, nullptr, nullptr);
+    const int4 slot = make_int4((int)((t - P) % R), (int)((t - 2 * P) % R), (int)((t - 3 * P) % R), (int)((t - 4 * P) % R));
+    hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)ring.p, slot, make_double4(4.0, -6.0, 4.0, -1.0),
+                       n, (const P4*)nbh::ctx_vel(c), X, (P4*)L.cub.p, 0.0, nullptr, nullptr, nullptr, nullptr);
     LH(hipGetLastError(), "features kernel");
-    hipLaunchKernelGGL(k_targets, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const P4*)nbh::ctx_acc(c), n, T);
+    hipLaunchKernelGGL(k_targets, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const P4*)nbh::ctx_acc(c),
+                       L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, n, T);
     LH(hipGetLastError(), "targets kernel");
     L.ds_rows += n;
     return NB_OK;
@@ -493,9 +655,11 @@ nb_status kick_drift(nb_ctx* c, const Schedule& q) {
 
 void destroy(Learned* l) {
     if (!l) return;
+    l->near.release();
     for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
                       &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub,
-                      &l->pred, &l->chk, &l->mask, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc})
+                      &l->pred, &l->chk, &l->mask, &l->held, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc,
+                      &l->nearacc})
         b->release();
     delete l;
 }
@@ -505,7 +669,10 @@ void on_new_state(Learned* l) {
     l->count = 0;
     l->head = 3;
     l->since_exact = 0;
+    l->sub = 0;
     l->fallback = true;
+    l->pred_net = l->net_ok = false;
+    l->clear_held = true;
     l->nflag = 0;
 }
 
@@ -518,6 +685,10 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
     if (n == 0 || time_steps == 0) return NB_OK;
     if (nb_status s = ensure_particles(L, n)) return s;
     hipStream_t st = nbh::ctx_stream(c);
+    if (L.clear_held) {
+        LH(hipMemsetAsync(L.held.p, 0, n, st), "clear holds");
+        L.clear_held = false;
+    }
     LH(nbk::launch_drift((int)n, nbh::ctx_pm(c), nbh::ctx_vel(c), cc[0] * dt, st), "drift kernel");
     for (size_t t = 0; t < time_steps; t++) {
         for (size_t s = 0; s + 1 < lf; s++) {
@@ -525,24 +696,34 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
             const bool exact = !full || L.fallback || (L.since_exact + 1 >= L.period);
             if (exact) {
                 if (full) {
-                    if (nb_status e = predict(c, L, (P4*)L.pred.p, false)) return e;
+                    if (nb_status e = predict(c, L, (P4*)L.pred.p, false, true)) return e;
                 }
                 if (nb_status e = exact_force(c, L.exact_kform, eps)) return e;
+                if (nb_status e = near_field(c, L, eps)) return e;
                 L.st.exact_evals++;
                 // extrapolation starts only after a check has passed, so a
                 // network that produces garbage never reaches the integrator
                 if (full) {
                     if (nb_status e = check(c, L)) return e;
                 }
+                if (nb_status e = push_history(c, L)) return e;
                 L.since_exact = 0;
             } else {
-                if (nb_status e = predict(c, L, nbh::ctx_acc(c), true)) return e;
+                if (L.rc > 0) {
+                    if (nb_status e = near_field(c, L, eps)) return e;
+                    if (nb_status e = predict(c, L, (P4*)L.pred.p, true, L.net_ok)) return e;
+                    hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.pred.p,
+                                       (const P4*)L.nearacc.p, (long)n, nbh::ctx_acc(c));
+                    LH(hipGetLastError(), "add kernel");
+                } else {
+                    if (nb_status e = predict(c, L, nbh::ctx_acc(c), true, L.net_ok)) return e;
+                }
                 if (nb_status e = exact_subset(c, L, eps, L.exact_kform)) return e;
                 L.st.predicted_evals++;
                 L.st.subset_rows += (uint64_t)L.nflag;
                 L.since_exact++;
             }
-            if (nb_status e = push_history(c, L)) return e;
+            L.sub++;
             if (nb_status e = kick_drift(c, schedule(t, time_steps, s, dt, cc, dd, lf))) return e;
         }
     }
@@ -581,6 +762,50 @@ nb_status nb_learned_set_flag_ratio(nb_ctx* c, double rho) {
     return NB_OK;
 }
 
+nb_status nb_learned_set_hold_ratio(nb_ctx* c, double kappa) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_hold_ratio: null context");
+    if (!(kappa > 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_hold_ratio: ratio must be > 0 (inf: never hold)");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_set_hold_ratio: out of host memory");
+    L->kappa = kappa;
+    return NB_OK;
+}
+
+nb_status nb_learned_set_near_radius(nb_ctx* c, double rc) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_near_radius: null context");
+    if (!(rc >= 0.0) || !std::isfinite(rc)) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_near_radius: radius must be finite and >= 0");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_set_near_radius: out of host memory");
+    L->rc = rc;
+    nbl::on_new_state(L);
+    return NB_OK;
+}
+
+nb_status nb_near_accel(nb_ctx* c, int form, double rc, double eps, double* const acc[3]) {
+    if (!c || !acc || !acc[0] || !acc[1] || !acc[2]) return nbh::set_error(NB_ERR_ARG, "nb_near_accel: null argument");
+    const int kf = nbh::kernel_form(form);
+    if (kf < 0) return nbh::set_error(NB_ERR_ARG, "nb_near_accel: form must be FP64 or a mixed form");
+    if (!(rc > 0.0) || !std::isfinite(rc) || !(eps >= 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_near_accel: need rc > 0, eps >= 0");
+    const size_t n = nbh::ctx_n(c);
+    if (n == 0) return NB_OK;
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_near_accel: out of host memory");
+    hipStream_t st = nbh::ctx_stream(c);
+    nbh::DevBuf out;
+    LH(out.ensure(n * sizeof(P4)), "allocate output");
+    LH(L->near.near_accel(nbh::ctx_pm(c), (int)n, rc, eps, (P4*)out.p, st, kf != nbk::FORM_FP64), "near-field kernels");
+    std::vector<P4> h(n);
+    LH(hipMemcpyAsync(h.data(), out.p, n * sizeof(P4), hipMemcpyDeviceToHost, st), "download");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    out.release();
+    for (size_t i = 0; i < n; i++) {
+        acc[0][i] = h[i].x;
+        acc[1][i] = h[i].y;
+        acc[2][i] = h[i].z;
+    }
+    return NB_OK;
+}
+
 nb_status nb_learned_set_mode(nb_ctx* c, int mode) {
     if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_mode: null context");
     if (mode != 0 && mode != 1) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_mode: mode must be 0 or 1");
@@ -605,8 +830,10 @@ nb_status nb_learned_collect(nb_ctx* c, size_t time_steps, double dt, const doub
     if (n == 0 || time_steps == 0) return NB_OK;
     if (nb_status s = nbl::ensure_particles(L, n)) return s;
     nbl::on_new_state(&L);
+    const long P = L.period, R = 4 * P;
+    if (R > 64) return nbh::set_error(NB_ERR_ARG, "nb_learned_collect: period must be <= 16");
     const long subs = (long)time_steps * (long)(lf - 1);
-    const long want = L.ds_rows + std::max(0L, subs - 4) * (long)n;
+    const long want = L.ds_rows + std::max(0L, subs - R) * (long)n;
     if (want > L.ds_cap) {   // grow, keeping what was collected before
         DevBuf x2, t2;
         LH(x2.ensure(want * 16 * sizeof(float)), "allocate training features");
@@ -625,16 +852,24 @@ nb_status nb_learned_collect(nb_ctx* c, size_t time_steps, double dt, const doub
         L.ds_cap = want;
     }
     hipStream_t st = nbh::ctx_stream(c);
+    DevBuf ring;   // the last R exact remainders, sample t in slot t mod R
+    LH(ring.ensure((size_t)R * n * sizeof(P4)), "allocate collection ring");
     LH(nbk::launch_drift((int)n, nbh::ctx_pm(c), nbh::ctx_vel(c), cc[0] * dt, st), "drift kernel");
+    long sub = 0;
     for (size_t t = 0; t < time_steps; t++)
-        for (size_t s = 0; s + 1 < lf; s++) {
+        for (size_t s = 0; s + 1 < lf; s++, sub++) {
             if (nb_status e = nbl::exact_force(c, kf, eps)) return e;
-            if (L.count == 4)
-                if (nb_status e = nbl::record(c, L)) return e;
-            if (nb_status e = nbl::push_history(c, L)) return e;
+            if (nb_status e = nbl::near_field(c, L, eps)) return e;
+            if (sub >= R)
+                if (nb_status e = nbl::record(c, L, ring, R, sub)) return e;
+            hipLaunchKernelGGL(nbl::k_sub, nbl::blocks((long)n), dim3(nbl::BLK), 0, st, (const P4*)nbh::ctx_acc(c),
+                               L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, (long)n,
+                               (P4*)ring.p + (sub % R) * (long)n);
+            LH(hipGetLastError(), "history kernel");
             if (nb_status e = nbl::kick_drift(c, nbl::schedule(t, time_steps, s, dt, cc, dd, lf))) return e;
         }
     LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    ring.release();
     nbl::on_new_state(&L);
     return NB_OK;
 }

@@ -1,0 +1,23 @@
+// nbody_near.h -- short-range accelerations on a sorted uniform grid
+// (nbody_near.hip), used by the learned form to split the force.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nbody_host.h"
+#include "nbody_kernels.h"
+
+namespace nbn {
+
+struct Near {
+    nbh::DevBuf keys_in, keys_out, idx_in, idx_out, sorted, temp, scan_temp, cols, chunks;
+
+    // out[i] = sum over j with 0 < r_ij < rc of m_j S(r_ij) d_ij / (r_ij^2 + eps^2)^(3/2);
+    // synchronises the stream once (to size the launch)
+    // fp32: FP32 pair arithmetic on FP64 differences (the mixed form's precision)
+    hipError_t near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk::P4* out, hipStream_t st,
+                          bool fp32 = false);
+    void release();
+};
+
+}  // namespace nbn

@@ -1,0 +1,349 @@
+// nbody_near.hip -- short-range ("near-field") accelerations on a sorted
+// uniform grid, for the learned form's force split.
+//
+// a_near,i = sum_{j : 0 < r_ij < rc} m_j S(q_ij) d_ij / (r_ij^2 + eps^2)^(3/2),
+// q = r^2 / rc^2, S = 1 for q <= 1/4 (r <= rc/2) falling to 0 at q = 1 with
+// the C2 quintic 1 - u^3 (10 - 15u + 6u^2), u = (q - 1/4) / (3/4); the
+// remainder a - a_near (pairs farther than rc/2, tapered) varies smoothly in
+// time and is what the learned form extrapolates.  A polynomial in r^2 needs
+// no square root per pair.
+//
+// Layout: cells of edge rc/2.  A cell (x, y, z) has the key
+// x << 32 | y << 21 | z (x, y biased by 1024 and clamped to [0, 2048), z by
+// 2^20 and clamped to [0, 2^21)), so the 2^22 (x, y) columns are the keys'
+// top bits and each column's cells are contiguous in key order.  Particles
+// are radix-sorted by key (stable: inside a cell, index order); a table gives
+// every column's run [start, end) of the sorted array.  Clamping only merges
+// far cells, and two particles closer than rc stay within two cells of each
+// other after it, so results stay exact (only slower for far-flung systems).
+//
+// Work: a chunk is a run of consecutive sorted particles, one per lane: up to
+// 64 particles of one dense column (kDense or more particles), or the
+// particles of consecutive sparser columns of one group of kGroup columns
+// (same x, y / kGroup equal), merged greedily up to 64, so sparse regions do
+// not cost one launch slot per column.  A block of five waves
+// takes a chunk spanning columns (x0, ys..ye) and z cells zmin..zmax: wave w
+// sums over the columns x0 - 2 + w, ys - 2..ye + 2, each restricted by binary
+// search to the z cells zmin - 2..zmax + 2, streaming those runs through LDS;
+// the five partial sums are added in wave order (bitwise reproducible).
+// Pairs beyond rc get S = 0, so the pair code is branch-free.
+//
+// FP64 throughout, or (fp32) FP64 differences with FP32 pair arithmetic and
+// FP32 sums over each 64-particle batch, flushed to FP64 -- the precision of
+// the mixed force form.
+//
+// Bound: VALU.  About 30 operations per examined pair; the 5 x 5 x
+// (5 + chunk height) cells around a chunk hold at least 15.6 rc^3 against the
+// 4.19 rc^3 of the cut-off sphere, so at most 27% of the examined pairs
+// contribute.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "nbody_near.h"
+
+namespace nbn {
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr int kWaves = 5;                 // waves per chunk: one per x-slice of neighbour columns
+constexpr int kColSide = 1 << 11;         // columns per side of the (x, y) table
+constexpr int kColBias = kColSide / 2;
+constexpr int kZBias = 1 << 20;
+constexpr int kZMax = (1 << 21) - 1;
+constexpr int kReach = 2;                 // neighbour cells each way (cell edge rc/2)
+constexpr int kDense = 32;                // particles from which a column gets chunks of its own
+constexpr int kGroup = 16;                // sparser columns are merged within groups of 16 along y
+
+__device__ __forceinline__ int cell_xy(double x, double inv_h) {
+    const double c = floor(x * inv_h) + (double)kColBias;
+    return (int)fmin(fmax(c, 0.0), (double)(kColSide - 1));
+}
+
+__device__ __forceinline__ int cell_z(double z, double inv_h) {
+    const double c = floor(z * inv_h) + (double)kZBias;
+    return (int)fmin(fmax(c, 0.0), (double)kZMax);
+}
+
+__device__ __forceinline__ unsigned long long pack(int ix, int iy, int iz) {
+    return ((unsigned long long)ix << 32) | ((unsigned long long)iy << 21) | (unsigned long long)iz;
+}
+
+__device__ __forceinline__ int col_of(unsigned long long key) { return (int)(key >> 21); }
+__device__ __forceinline__ int z_of(unsigned long long key) { return (int)(key & kZMax); }
+
+__global__ __launch_bounds__(BLK) void k_keys(const nbk::P4* __restrict__ pm, int n, double inv_h,
+                                              unsigned long long* __restrict__ keys, int* __restrict__ idx) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const nbk::P4 p = pm[i];
+    keys[i] = pack(cell_xy(p.x, inv_h), cell_xy(p.y, inv_h), cell_z(p.z, inv_h));
+    idx[i] = i;
+}
+
+__global__ __launch_bounds__(BLK) void k_gather_sorted(const nbk::P4* __restrict__ pm, const int* __restrict__ order,
+                                                       int n, nbk::P4* __restrict__ sorted) {
+    const int p = blockIdx.x * BLK + threadIdx.x;
+    if (p < n) sorted[p] = pm[order[p]];
+}
+
+__global__ __launch_bounds__(BLK) void k_clear_cols(int2* __restrict__ cols, int count) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c < count) cols[c] = make_int2(0, 0);
+}
+
+// Column runs of the sorted keys.
+__global__ __launch_bounds__(BLK) void k_col_runs(const unsigned long long* __restrict__ keys, int n,
+                                                  int2* __restrict__ cols) {
+    const int p = blockIdx.x * BLK + threadIdx.x;
+    if (p >= n) return;
+    const int c = col_of(keys[p]);
+    if (p == 0 || col_of(keys[p - 1]) != c) cols[c].x = p;
+    if (p == n - 1 || col_of(keys[p + 1]) != c) cols[c].y = p + 1;
+}
+
+// Chunk starts (flag[p] = 1; flag was cleared), one thread per group of
+// kGroup columns walking them in order: a dense column starts a chunk at its
+// first particle and every 64th after it; sparser columns join the open chunk
+// while it stays within 64 particles, else start a new one.
+__global__ __launch_bounds__(BLK) void k_chunk_flags(const int2* __restrict__ cols, int ngroups,
+                                                     int* __restrict__ flag) {
+    const int g = blockIdx.x * BLK + threadIdx.x;
+    if (g >= ngroups) return;
+    int open = 64;   // particles in the open chunk (64: none open)
+    for (int c = g * kGroup; c < (g + 1) * kGroup; c++) {
+        const int2 run = cols[c];
+        const int k = run.y - run.x;
+        if (k <= 0) continue;
+        if (k >= kDense) {
+            for (int q = run.x; q < run.y; q += 64) flag[q] = 1;
+            open = 64;
+        } else if (open + k > 64) {
+            flag[run.x] = 1;
+            open = k;
+        } else {
+            open += k;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_chunk_list(const int* __restrict__ flag, const int* __restrict__ pos, int n,
+                                                    int* __restrict__ chunks, int* __restrict__ nchunks) {
+    const int p = blockIdx.x * BLK + threadIdx.x;
+    if (p >= n) return;
+    if (flag[p]) chunks[pos[p]] = p;
+    if (p == n - 1) {
+        *nchunks = pos[p] + flag[p];
+        chunks[pos[p] + flag[p]] = n;   // sentinel: the end of the last chunk
+    }
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// first position in [lo, hi) with keys[pos] >= k (hi if none)
+__device__ __forceinline__ int lower_bound(const unsigned long long* __restrict__ keys, int lo, int hi,
+                                           unsigned long long k) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Order this lane's LDS accesses against the other lanes' of the same wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One ordered pair: a += m S(q) w d, w = (r^2 + eps^2)^(-3/2) from v_rsq_f64
+// and the correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2 (e = 1 - s2 y^2).
+__device__ __forceinline__ void near_pair(const nbk::P4& me, const nbk::P4& o, double inv_rc2, double eps2,
+                                          double& ax, double& ay, double& az) {
+    const double dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    const double u = fmin(fmax((r2 * inv_rc2 - 0.25) * (4.0 / 3.0), 0.0), 1.0);
+    const double S = __builtin_fma(-u * u * u, __builtin_fma(u, __builtin_fma(6.0, u, -15.0), 10.0), 1.0);
+    const double s2 = r2 + eps2;
+    const double y = __builtin_amdgcn_rsq(s2);
+    const double e = __builtin_fma(-s2, y * y, 1.0);
+    const double w = (y * y) * y * __builtin_fma(e, __builtin_fma(1.875, e, 1.5), 1.0);
+    const double f = (r2 > 0.0) ? o.m * S * w : 0.0;
+    ax = __builtin_fma(f, dx, ax);
+    ay = __builtin_fma(f, dy, ay);
+    az = __builtin_fma(f, dz, az);
+}
+
+// The same pair with FP64 differences rounded to FP32 and FP32 arithmetic:
+// FP32 taper, v_rsq_f32 with the first-order correction of the cube,
+// w = y^3 (1 + 3/2 (1 - s2 y^2)).
+__device__ __forceinline__ void near_pair32(const nbk::P4& me, const nbk::P4& o, float inv_rc2, float eps2,
+                                            float& ax, float& ay, float& az) {
+    const float dx = (float)(o.x - me.x), dy = (float)(o.y - me.y), dz = (float)(o.z - me.z);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    const float u = fminf(fmaxf((r2 * inv_rc2 - 0.25f) * (4.0f / 3.0f), 0.0f), 1.0f);
+    const float S = __builtin_fmaf(-u * u * u, __builtin_fmaf(u, __builtin_fmaf(6.0f, u, -15.0f), 10.0f), 1.0f);
+    const float s2 = r2 + eps2;
+    const float y = __builtin_amdgcn_rsqf(s2);
+    const float t = y * y;
+    const float w = t * y * __builtin_fmaf(__builtin_fmaf(-s2, t, 1.0f), 1.5f, 1.0f);
+    const float f = (r2 > 0.0f) ? (float)o.m * S * w : 0.0f;
+    ax = __builtin_fmaf(f, dx, ax);
+    ay = __builtin_fmaf(f, dy, ay);
+    az = __builtin_fmaf(f, dz, az);
+}
+
+template <bool F32>
+__global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict__ sorted,
+                                                      const unsigned long long* __restrict__ keys,
+                                                      const int* __restrict__ order, const int2* __restrict__ cols,
+                                                      const int* __restrict__ chunks, int n, double inv_rc2,
+                                                      double eps2, nbk::P4* __restrict__ out) {
+    __shared__ nbk::P4 stage[kWaves][64];
+    __shared__ nbk::P4 part[kWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int begin = chunks[blockIdx.x];
+    const int cnt = chunks[blockIdx.x + 1] - begin;   // 1..64
+    const int i = begin + (lane < cnt ? lane : 0);
+    const nbk::P4 me = sorted[i];
+    const unsigned long long ki = keys[i];
+    const int x0 = (int)(ki >> 32);   // one x for the whole chunk
+    const int ys = wave_min((int)((ki >> 21) & (kColSide - 1))), ye = wave_max((int)((ki >> 21) & (kColSide - 1)));
+    const int zlo = max(wave_min(z_of(ki)) - kReach, 0), zhi = min(wave_max(z_of(ki)) + kReach, kZMax);
+    const int X = x0 - kReach + w;
+    const int ny = ye - ys + 1 + 2 * kReach;   // <= kGroup + 2 kReach <= 64
+    // lanes 0..ny-1 locate the z-runs of columns (X, ys - 2 .. ye + 2)
+    int a = 0, b = 0;
+    if (lane < ny && X >= 0 && X < kColSide) {
+        const int Y = ys - kReach + lane;
+        if (Y >= 0 && Y < kColSide) {
+            const int2 run = cols[X * kColSide + Y];
+            a = lower_bound(keys, run.x, run.y, pack(X, Y, zlo));
+            b = lower_bound(keys, a, run.y, pack(X, Y, zhi) + 1);
+        }
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    nbk::P4* L = stage[w];
+    for (int k = 0; k < ny; k++) {
+        const int ak = __shfl(a, k, 64), bk = __shfl(b, k, 64);
+        for (int j0 = ak; j0 < bk; j0 += 64) {
+            const int jj = j0 + lane;
+            // filler: this lane's position with zero mass, so it adds nothing
+            // to any lane (r^2 = 0 for this one, m = 0 for the others)
+            nbk::P4 J = nbk::P4{me.x, me.y, me.z, 0.0};
+            if (jj < bk) J = sorted[jj];
+            L[lane] = J;
+            wave_sync();
+            const int m4 = (min(64, bk - j0) + 3) & ~3;   // fillers pad to a multiple of 4
+            if constexpr (F32) {
+                const float ir = (float)inv_rc2, e2 = (float)eps2;
+                float fx = 0.f, fy = 0.f, fz = 0.f;
+                for (int t = 0; t < m4; t += 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) near_pair32(me, L[t + q], ir, e2, fx, fy, fz);
+                }
+                ax += fx;
+                ay += fy;
+                az += fz;
+            } else {
+                for (int t = 0; t < m4; t += 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) near_pair(me, L[t + q], inv_rc2, eps2, ax, ay, az);
+                }
+            }
+            wave_sync();
+        }
+    }
+    part[w][lane] = nbk::P4{ax, ay, az, 0.0};
+    __syncthreads();
+    if (w == 0 && lane < cnt) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int q = 0; q < kWaves; q++) {
+            sx += part[q][lane].x;
+            sy += part[q][lane].y;
+            sz += part[q][lane].z;
+        }
+        out[order[begin + lane]] = nbk::P4{sx, sy, sz, 0.0};
+    }
+}
+
+}  // namespace
+
+hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk::P4* out, hipStream_t st,
+                            bool fp32) {
+    if (n <= 0) return hipSuccess;
+    const double inv_h = 2.0 / rc;
+    const int ncols = kColSide * kColSide;
+    hipError_t e;
+    if ((e = keys_in.ensure((size_t)n * 8)) != hipSuccess) return e;
+    if ((e = keys_out.ensure((size_t)n * 8)) != hipSuccess) return e;
+    if ((e = idx_in.ensure((size_t)n * 4)) != hipSuccess) return e;
+    if ((e = idx_out.ensure((size_t)n * 4)) != hipSuccess) return e;
+    if ((e = sorted.ensure((size_t)n * sizeof(nbk::P4))) != hipSuccess) return e;
+    if ((e = cols.ensure((size_t)ncols * sizeof(int2))) != hipSuccess) return e;
+    if ((e = chunks.ensure((size_t)(4 * n + 3) * 4 + 64)) != hipSuccess) return e;
+    const dim3 grid((n + BLK - 1) / BLK);
+    hipLaunchKernelGGL(k_keys, grid, dim3(BLK), 0, st, pm, n, inv_h, (unsigned long long*)keys_in.p, (int*)idx_in.p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    size_t bytes = 0;
+    e = rocprim::radix_sort_pairs(nullptr, bytes, (const unsigned long long*)keys_in.p, (unsigned long long*)keys_out.p,
+                                  (const int*)idx_in.p, (int*)idx_out.p, (unsigned int)n, 0, 43, st);
+    if (e != hipSuccess) return e;
+    if ((e = temp.ensure(bytes + 16)) != hipSuccess) return e;
+    e = rocprim::radix_sort_pairs(temp.p, bytes, (const unsigned long long*)keys_in.p, (unsigned long long*)keys_out.p,
+                                  (const int*)idx_in.p, (int*)idx_out.p, (unsigned int)n, 0, 43, st);
+    if (e != hipSuccess) return e;
+    const unsigned long long* keys = (const unsigned long long*)keys_out.p;
+    int* flag = (int*)chunks.p;              // n flags, then n + 1 scan results, the list (n + 1) and the count
+    int* pos = flag + n;
+    int* list = pos + n + 1;
+    int* nchunks = list + n + 1;
+    hipLaunchKernelGGL(k_gather_sorted, grid, dim3(BLK), 0, st, pm, (const int*)idx_out.p, n, (nbk::P4*)sorted.p);
+    hipLaunchKernelGGL(k_clear_cols, dim3((ncols + BLK - 1) / BLK), dim3(BLK), 0, st, (int2*)cols.p, ncols);
+    hipLaunchKernelGGL(k_col_runs, grid, dim3(BLK), 0, st, keys, n, (int2*)cols.p);
+    if ((e = hipMemsetAsync(flag, 0, (size_t)n * sizeof(int), st)) != hipSuccess) return e;
+    const int ngroups = ncols / kGroup;
+    hipLaunchKernelGGL(k_chunk_flags, dim3((ngroups + BLK - 1) / BLK), dim3(BLK), 0, st, (const int2*)cols.p, ngroups,
+                       flag);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    size_t sbytes = 0;
+    e = rocprim::exclusive_scan(nullptr, sbytes, (const int*)flag, pos, 0, (size_t)n, rocprim::plus<int>(), st);
+    if (e != hipSuccess) return e;
+    if ((e = scan_temp.ensure(sbytes + 16)) != hipSuccess) return e;
+    e = rocprim::exclusive_scan(scan_temp.p, sbytes, (const int*)flag, pos, 0, (size_t)n, rocprim::plus<int>(), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_chunk_list, grid, dim3(BLK), 0, st, (const int*)flag, (const int*)pos, n, list, nchunks);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    int count = 0;
+    if ((e = hipMemcpyAsync(&count, nchunks, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    if (count <= 0 || count > n) return hipErrorUnknown;
+    if (fp32)
+        hipLaunchKernelGGL(k_near<true>, dim3(count), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
+                           (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, n, 1.0 / (rc * rc),
+                           eps * eps, out);
+    else
+        hipLaunchKernelGGL(k_near<false>, dim3(count), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
+                           (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, n, 1.0 / (rc * rc),
+                           eps * eps, out);
+    return hipGetLastError();
+}
+
+void Near::release() {
+    for (nbh::DevBuf* b : {&keys_in, &keys_out, &idx_in, &idx_out, &sorted, &temp, &scan_temp, &cols, &chunks})
+        b->release();
+}
+
+}  // namespace nbn

@@ -73,17 +73,21 @@ __device__ __forceinline__ void sym64(double xi, double yi, double zi, double mi
     bz = __builtin_fma(-sj, dz, bz);
 }
 
-// Mixed: FP64 difference rounded to FP32, FP32 rsqrt (+ one Newton step when
-// NR) and pair arithmetic, FP32 sums (flushed to FP64 per 64-particle chunk).
+// Mixed: FP64 difference rounded to FP32, FP32 rsqrt and pair arithmetic,
+// FP32 sums (flushed to FP64 per 64-particle chunk).  With NR the Newton-Raphson
+// step is applied to the cube: y = rsq(r2) (1 ulp), e = 1 - r2 y^2, and
+// r^-3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 3/2 e) + O(e^2) -- the same first-order
+// gain as refining y by y (3 - r2 y^2)/2 and cubing, for two FMAs fewer.
 template <bool EPS0, bool MASK, bool NR>
 __device__ __forceinline__ void sym32(double xi, double yi, double zi, float mi, double xj, double yj,
                                       double zj, float mj, float eps2, bool keep, float& ax, float& ay,
                                       float& az, float& bx, float& by, float& bz) {
     const float dx = (float)(xj - xi), dy = (float)(yj - yi), dz = (float)(zj - zi);
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-    float y = __builtin_amdgcn_rsqf(r2);
-    if (NR) y = y * __builtin_fmaf(-0.5f * r2, y * y, 1.5f);
-    float w = (y * y) * y;
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float t = y * y;
+    float w = t * y;
+    if (NR) w = w * __builtin_fmaf(__builtin_fmaf(-r2, t, 1.0f), 1.5f, 1.0f);
     if (EPS0) w = (r2 > 0.0f) ? w : 0.0f;
     if (MASK) w = keep ? w : 0.0f;
     const float si = mj * w, sj = mi * w;
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
 #pragma unroll
                 for (int k = 0; k < K; k++) fax[k] = fay[k] = faz[k] = 0.f;
                 if (!diag) {
-#pragma unroll 2
+#pragma unroll 1
                     for (int t = 0; t < 64; t++) {
                         const P4 J = lds[lane + t];
                         const float mj = (float)J.m;

@@ -37,7 +37,8 @@ EXPORTS = (
     "nb_timing_read", "nb_timing_reset", "nb_synchronize", "nb_learned_configure",
     "nb_learned_set_mode", "nb_learned_collect", "nb_learned_train", "nb_learned_stats_read",
     "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
-    "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio",
+    "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
+    "nb_near_accel", "nb_learned_set_hold_ratio",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -94,6 +95,9 @@ def load_library(path: str = LIB_PATH):
     L.nb_learned_set_mode.argtypes = [_VP, C.c_int]
     L.nb_learned_reset.argtypes = [_VP, C.c_uint64]
     L.nb_learned_set_flag_ratio.argtypes = [_VP, C.c_double]
+    L.nb_learned_set_near_radius.argtypes = [_VP, C.c_double]
+    L.nb_near_accel.argtypes = [_VP, C.c_int, C.c_double, C.c_double, _P3]
+    L.nb_learned_set_hold_ratio.argtypes = [_VP, C.c_double]
     L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
     L.nb_learned_train.argtypes = [_VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]
     L.nb_learned_stats_read.argtypes = [_VP, C.POINTER(LearnedStats)]
@@ -273,6 +277,17 @@ class Solver:
 
     def learned_set_flag_ratio(self, rho: float):
         _check(self.L.nb_learned_set_flag_ratio(self.h, rho), "nb_learned_set_flag_ratio")
+
+    def learned_set_near_radius(self, rc: float):
+        _check(self.L.nb_learned_set_near_radius(self.h, rc), "nb_learned_set_near_radius")
+
+    def learned_set_hold_ratio(self, kappa: float):
+        _check(self.L.nb_learned_set_hold_ratio(self.h, kappa), "nb_learned_set_hold_ratio")
+
+    def near_accel(self, rc: float, eps: float = 0.0, form="fp64") -> np.ndarray:
+        a = np.empty((3, self.n))
+        _check(self.L.nb_near_accel(self.h, _form(form), rc, eps, _planes(a)), "nb_near_accel")
+        return a
 
     def learned_set_mode(self, mode: int):
         _check(self.L.nb_learned_set_mode(self.h, mode), "nb_learned_set_mode")

@@ -164,6 +164,22 @@ ORC_API void orc_poly_extrap(size_t n3, const double *a0, const double *a1, cons
     for (size_t i = 0; i < n3; i++) out[i] = 4.0 * a0[i] - 6.0 * a1[i] + 4.0 * a2[i] - a3[i];
 }
 
+/* Lagrange weights of the cubic through samples at times t[0..3] (distinct),
+ * evaluated at x: value = sum_k w[k] a(t[k]).  For t = (0, -1, -2, -3) and
+ * x = 1 they are (4, -6, 4, -1), the equally spaced case above. */
+ORC_API void orc_lagrange4(const double *t, double x, double *w)
+{
+    for (int k = 0; k < 4; k++) {
+        double num = 1.0, den = 1.0;
+        for (int j = 0; j < 4; j++)
+            if (j != k) {
+                num *= x - t[j];
+                den *= t[k] - t[j];
+            }
+        w[k] = num / den;
+    }
+}
+
 /* Extrapolation features of particle i from its last four accelerations
  * a0 = a_n, a1 = a_{n-1}, a2, a3 (each 3 doubles) and velocity v:
  *   D1 = a0 - a1, D2 = a0 - 2a1 + a2, D3 = a0 - 3a1 + 3a2 - a3 (backward
@@ -203,4 +219,45 @@ ORC_API void orc_features(size_t n, const double *a0, const double *a1, const do
         }
         scale[i] = n3;
     }
+}
+
+/* Taper of the learned form's near field as a function of q = r^2 / rc^2:
+ * S = 1 for q <= 1/4 (r <= rc/2), 0 for q >= 1 (r >= rc), and in between
+ * S = 1 - u^3 (10 - 15u + 6u^2), u = (q - 1/4) / (3/4) (the C2 quintic
+ * "smootherstep"; smooth in r as q is). */
+static double taper_q(double q)
+{
+    if (q >= 1.0) return 0.0;
+    if (q <= 0.25) return 1.0;
+    double u = (q - 0.25) * (4.0 / 3.0);
+    return 1.0 - u * u * u * (10.0 - 15.0 * u + 6.0 * u * u);
+}
+
+/* Near-field acceleration of the learned form's force split (brute force):
+ * a_i = sum_{j : 0 < r_ij < rc} m_j S(r_ij) d_ij / (r_ij^2 + eps^2)^(3/2).
+ * Pinned by the tests: S's joins (value and C2 order), the two-body case, and
+ * for rc beyond the system size the near field equals the full softened sum. */
+ORC_API void orc_near_accel(size_t n, const double *m, const double *x, const double *y, const double *z,
+                            double rc, double eps, double *ax, double *ay, double *az)
+{
+    const double e2 = eps * eps, rc2 = rc * rc;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t i = 0; i < n; i++) {
+        double sx = 0, sy = 0, sz = 0;
+        for (size_t j = 0; j < n; j++) {
+            double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0 || r2 >= rc2) continue;
+            double s2 = r2 + e2;
+            double f = m[j] * taper_q(r2 / rc2) / (s2 * sqrt(s2));
+            sx += f * dx; sy += f * dy; sz += f * dz;
+        }
+        ax[i] = sx; ay[i] = sy; az[i] = sz;
+    }
+}
+
+/* The taper S(r) on its own (for the tests). */
+ORC_API double orc_near_taper(double r, double rc)
+{
+    return taper_q((r * r) / (rc * rc));
 }

@@ -66,6 +66,10 @@ class Oracle:
         L.orc_mlp_grad.restype = C.c_double
         L.orc_adam.argtypes = [_SZ, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
         L.orc_features.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_lagrange4.argtypes = [_D, C.c_double, _D]
+        L.orc_near_accel.argtypes = [_SZ, _D, _D, _D, _D, C.c_double, C.c_double, _D, _D, _D]
+        L.orc_near_taper.argtypes = [C.c_double, C.c_double]
+        L.orc_near_taper.restype = C.c_double
 
     # --- initial conditions -------------------------------------------------
     def rng_stream(self, seed: int, n: int) -> np.ndarray:
@@ -191,6 +195,22 @@ class Oracle:
         args = [np.ascontiguousarray(a, dtype=np.float64) for a in (a0, a1, a2, a3, v)]
         self.lib.orc_features(n, *[_p(a) for a in args], _p(x), _p(s), _p(cub))
         return x, s, cub
+
+    def lagrange4(self, t, x: float) -> np.ndarray:
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        w = np.empty(4)
+        self.lib.orc_lagrange4(_p(t), x, _p(w))
+        return w
+
+    def near_accel(self, m, pos, rc: float, eps: float) -> np.ndarray:
+        n = m.shape[0]
+        a = np.empty((3, n))
+        self.lib.orc_near_accel(n, _p(m), *[_p_row(pos, k) for k in range(3)], rc, eps,
+                                *[_p_row(a, k) for k in range(3)])
+        return a
+
+    def near_taper(self, r: float, rc: float) -> float:
+        return self.lib.orc_near_taper(r, rc)
 
     def poly_extrap(self, a0, a1, a2, a3):
         out = np.empty_like(a0)

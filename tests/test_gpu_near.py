@@ -1,0 +1,104 @@
+"""The near-field split of the learned form on the GPU: the sorted-grid
+near-field kernel against the oracle's brute-force sum, and learned stepping
+with the split against a numpy restatement of its schedule."""
+import numpy as np
+import pytest
+
+import nbody_amd
+
+from test_gpu_learned import restated_learned
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kind,n,rc,eps", [("plummer", 3000, 0.1, 0.01), ("plummer", 3000, 0.4, 0.0),
+                                           ("sphere", 2049, 0.25, 1e-3), ("plummer", 700, 100.0, 0.01)])
+def test_near_accel_vs_oracle(solver, orc, kind, n, rc, eps):
+    m, pos, vel, _ = nbody_amd.init_system(kind, n, 3)
+    solver.set_state(m, pos, vel)
+    a = solver.near_accel(rc, eps)
+    ao = orc.near_accel(m, pos, rc, eps)
+    scale = np.max(np.linalg.norm(ao, axis=0))
+    assert scale > 0
+    assert np.max(np.linalg.norm(a - ao, axis=0)) < 1e-12 * scale
+    if rc > 80:   # every pair inside rc/2 (radii stay below 20): the whole softened sum
+        full = orc.accel(m, pos, eps)
+        assert np.max(np.abs(a - full)) < 1e-12 * np.max(np.abs(full))
+
+
+@pytest.mark.parametrize("rc", [0.1, 0.3])
+def test_near_accel_mixed_vs_oracle(solver, orc, rc):
+    """FP32 pair arithmetic on FP64 differences: within 1e-5 of the largest
+    near-field acceleration (FP32 rounding of ~1e3 terms)."""
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 4000, 8)
+    solver.set_state(m, pos, vel)
+    a = solver.near_accel(rc, 0.005, form="mixed")
+    ao = orc.near_accel(m, pos, rc, 0.005)
+    scale = np.max(np.linalg.norm(ao, axis=0))
+    err = np.max(np.linalg.norm(a - ao, axis=0))
+    assert err < 1e-5 * scale, err / scale
+
+
+def test_near_accel_clustered_and_isolated(solver, orc):
+    """A dense clump sharing a few cells, plus isolated particles with no
+    neighbour inside rc (their near field is exactly zero)."""
+    rng = np.random.default_rng(5)
+    clump = rng.normal(0.0, 0.01, (3, 500))
+    far = np.array([[5.0, -7.0, 3.0], [5.0, 2.0, -9.0], [5.0, 1.0, 4.0]])
+    pos = np.ascontiguousarray(np.concatenate([clump, far], axis=1))
+    n = pos.shape[1]
+    m = np.full(n, 1.0 / n)
+    solver.set_state(m, pos, np.zeros_like(pos))
+    a = solver.near_accel(0.05, 1e-3)
+    ao = orc.near_accel(m, pos, 0.05, 1e-3)
+    assert np.max(np.linalg.norm(a - ao, axis=0)) < 1e-12 * np.max(np.linalg.norm(ao, axis=0))
+    assert np.all(a[:, -3:] == 0.0)
+
+
+def test_near_accel_far_flung(solver, orc):
+    """Positions far beyond the cell table share its edge columns: still exact,
+    including a close pair sitting there and particles on the far side."""
+    rng = np.random.default_rng(9)
+    clump = rng.normal(0.0, 0.2, (3, 400))
+    far = np.array([[3000.0, 3000.0 + 0.01, -2500.0, 0.0, 1e5],
+                    [5.0, 5.0, 2500.0, -4000.0, 1e5],
+                    [1.0, 1.0 + 0.02, 0.0, 7000.0, -1e5]])
+    pos = np.ascontiguousarray(np.concatenate([clump, far], axis=1))
+    n = pos.shape[1]
+    m = np.full(n, 1.0 / n)
+    solver.set_state(m, pos, np.zeros_like(pos))
+    a = solver.near_accel(0.5, 1e-3)
+    ao = orc.near_accel(m, pos, 0.5, 1e-3)
+    assert np.max(np.linalg.norm(a - ao, axis=0)) < 1e-12 * np.max(np.linalg.norm(ao, axis=0))
+    assert np.linalg.norm(ao[:, 400]) > 0 and np.all(a[:, 402:] == 0.0)
+
+
+def test_near_accel_rejects_bad_radius(solver):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 64, 1)
+    solver.set_state(m, pos, vel)
+    with pytest.raises(nbody_amd.NBodyError):
+        solver.near_accel(0.0, 0.01)
+    with pytest.raises(nbody_amd.NBodyError):
+        solver.near_accel(float("inf"), 0.01)
+
+
+def test_learned_split_matches_restatement(solver, orc):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 400, 2)
+    steps, dt, eps, rc = 30, 1.0 / 256, 0.05, 0.3
+    solver.learned_configure(period=4, threshold=1e-3, exact_form="fp64")
+    solver.learned_set_mode(1)
+    solver.learned_set_flag_ratio(2e-3)
+    solver.learned_set_near_radius(rc)
+    try:
+        solver.set_state(m, pos, vel)
+        solver.step(steps, dt, 2, "learned", eps)
+        p, v = solver.get_state()
+        st = solver.learned_stats()
+    finally:
+        solver.learned_set_near_radius(0.0)
+    pr, vr, sr = restated_learned(orc, m, pos, vel, steps, dt, eps, 4, 1e-3, 2e-3, rc=rc)
+    assert st["exact_evals"] == sr["exact"] and st["predicted_evals"] == sr["predicted"]
+    assert st["subset_rows"] == sr["subset"]
+    assert sr["predicted"] > 0
+    assert np.max(np.abs(p - pr)) < 1e-11
+    assert np.max(np.abs(v - vr)) < 1e-9 * np.max(np.abs(vr))

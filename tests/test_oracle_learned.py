@@ -88,3 +88,65 @@ def test_features_zero_history(orc):
     z = np.zeros((3, 3))
     x, s, cub = orc.features(z, z, z, z, z)
     assert np.all(x == 0) and np.all(s == 0) and np.all(cub == 0)
+
+
+def test_near_taper_properties(orc):
+    """S = 1 inside rc/2, 0 from rc on, C2 at both joins, monotone between,
+    1/2 where r^2 / rc^2 is midway (5/8)."""
+    rc = 0.3
+    assert orc.near_taper(0.0, rc) == 1.0 and orc.near_taper(0.15, rc) == 1.0
+    assert orc.near_taper(0.3, rc) == 0.0 and orc.near_taper(0.45, rc) == 0.0
+    assert abs(orc.near_taper(rc * np.sqrt(5 / 8), rc) - 0.5) < 1e-14
+    # C2 at the joins: S(r0 +- h) - S(r0) = O(h^3) on the inner side, 0 outside
+    for r0, side in ((0.15, 1.0), (0.3, -1.0)):
+        d = [abs(orc.near_taper(r0 + side * h, rc) - orc.near_taper(r0, rc)) for h in (1e-3, 1e-4)]
+        assert 0.9e3 < d[0] / d[1] < 1.1e3, (r0, d)
+        assert orc.near_taper(r0 - side * 1e-3, rc) == orc.near_taper(r0, rc)
+    r = np.linspace(0.15, 0.3, 301)
+    s = np.array([orc.near_taper(x, rc) for x in r])
+    assert np.all(np.diff(s) <= 0)
+
+
+def test_near_field_two_bodies(orc):
+    """Two bodies: the near field is the softened pair force times S(r), zero beyond rc."""
+    rc, eps = 0.4, 0.05
+    m = np.array([0.7, 0.3])
+    for r in (0.1, 0.25, 0.39, 0.41):
+        pos = np.array([[0.0, r], [0.0, 0.0], [0.0, 0.0]])
+        a = orc.near_accel(m, pos, rc, eps)
+        full = orc.accel(m, pos, eps)
+        S = orc.near_taper(r, rc)
+        assert np.allclose(a, S * full, rtol=1e-14, atol=0.0), r
+        if r > rc:
+            assert np.all(a == 0.0)
+
+
+def test_near_field_large_radius_is_full_sum(orc):
+    """With every pair inside rc/2 the near field is the whole softened sum."""
+    rng = np.random.default_rng(8)
+    n = 300
+    m = rng.uniform(0.5, 1.5, n) / n
+    pos = rng.uniform(-1, 1, (3, n))
+    a = orc.near_accel(m, pos, 10.0, 0.01)
+    full = orc.accel(m, pos, 0.01)
+    assert np.max(np.abs(a - full)) < 1e-13 * np.max(np.abs(full))
+
+
+def test_lagrange_weights(orc):
+    """Equally spaced nodes give the backward-difference cubic; any distinct
+    nodes reproduce cubics exactly and the weights sum to one."""
+    assert np.allclose(orc.lagrange4([0, -1, -2, -3], 1.0), [4, -6, 4, -1], rtol=0, atol=1e-14)
+    assert np.allclose(orc.lagrange4([0, -4, -8, -12], 4.0), [4, -6, 4, -1], rtol=0, atol=1e-13)
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        t = np.sort(rng.choice(40, 4, replace=False))[::-1].astype(float)
+        x = t[0] + rng.integers(1, 6)
+        c = rng.normal(size=4)
+        f = lambda u: c[0] + c[1] * u + c[2] * u ** 2 + c[3] * u ** 3
+        w = orc.lagrange4(t, x)
+        assert abs(np.sum(w) - 1) < 1e-10
+        assert abs(np.dot(w, [f(u) for u in t]) - f(x)) < 1e-8 * (1 + abs(f(x)))
+    # a quartic is missed by exactly its leading coefficient times the nodal polynomial
+    t = np.array([0.0, -4.0, -8.0, -12.0])
+    w = orc.lagrange4(t, 3.0)
+    assert abs(np.dot(w, t ** 4) - 3.0 ** 4 + np.prod(3.0 - t)) < 1e-9
