@@ -1,26 +1,36 @@
 """Benchmark: softened all-pairs force + leapfrog on one MI355X.
 
 Workload (default): Plummer sphere N = 262144 (a = 1, r < 20, m = 1/N, seeded),
-eps = 0.005, dt = 1/256, 2nd-order DKD leapfrog, K steps (default 1000) for
-each precision form -- plain FP64, mixed, learned (cubic extrapolation +
-network correction, exact sum every 4th substep).  A step is one leapfrog
-step of the whole system: one force evaluation (N^2 ordered pairs) plus the
-kick/drift.  The state is resident in HBM before the timed region starts.
+eps = 0.005, dt = 1/256, 2nd-order DKD leapfrog, K steps (default 1000), for
+each precision form:
+  fp64          plain FP64 direct sum (symmetric kernel);
+  mixed         FP64 differences, FP32 rsqrt + Newton step and pair math;
+  learned       exact (mixed) sum every 6th substep, the others extrapolated
+                from the last four exact samples (cubic + MLP correction) with
+                the pairs closer than rc = 0.4 summed exactly on every substep;
+                the MLP is trained first on 200 steps of a Plummer N = 65536;
+  learned-cubic the same without the network (its ablation).
+A step is one leapfrog step of the whole system: one force evaluation (N^2
+ordered pairs) plus the kick/drift.  The state is resident in HBM before the
+timed region starts.
 
 Prints ONE JSON line.  `value` is the effective throughput in GPairs/s
 (N^2 pairs per simulated step x steps / wall time) of the fastest form whose
 energy drift |dE/E| over the run stays <= 1e-6; `forms` holds every form's
-GPairs/s, steps/s, |dE/E| and max relative force error against FP64.
-`roofline` is the FP64 force kernel against the FP64 vector peak (20 flops
-per pair), from HIP events around every force launch; `cpu_baseline` times
-the reference's own AVX-512 force kernel (compiled from its sources) on this
-host's cores on a bounded sample.
+GPairs/s, steps/s, |dE/E|, speed-up over FP64 and force error (max relative
+against FP64 for fp64/mixed; RMS and max relative at the last check for the
+learned forms).  `roofline` is the dominant kernel of the headline form,
+timed with HIP events around every launch on the solver's stream;
+`rooflines` adds the others.  `cpu_baseline` times the reference's own
+AVX-512 force kernel (compiled from its sources) on this host's cores on a
+bounded sample.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -34,8 +44,10 @@ import nbody_amd  # noqa: E402
 METRIC = "GPairs/s + sim steps/s at |dE/E|<=1e-6, FP64 vs mixed vs learned, 1 GPU"
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-FLOPS_PER_PAIR = 20
+FLOPS_PER_PAIR = 20          # the conventional count for one softened pair interaction
 DE_BOUND = 1e-6
+LEARNED = {"period": 6, "threshold": 2e-3, "rc": 0.4, "rho": 0.03, "kappa": 1e-3, "exact_form": "mixed"}
+TRAIN = {"n": 65536, "steps": 200, "seed": 2, "iters": 3000, "batch": 4096, "lr": 1e-3}
 
 
 def log(*a):
@@ -48,22 +60,58 @@ def rel_err(a, b):
     return float(np.max(np.linalg.norm(a - b, axis=0)[mask] / nb[mask]))
 
 
-def run_form(s, torch, state, form, steps, warmup, dt, eps, learned_cfg=None):
+def profile_traffic(name: str):
+    """HBM bytes per launch recorded in a committed rocprof summary (or None)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def configure_learned(s, mode):
+    s.learned_configure(period=LEARNED["period"], threshold=LEARNED["threshold"], exact_form=LEARNED["exact_form"])
+    s.learned_set_near_radius(LEARNED["rc"])
+    s.learned_set_flag_ratio(LEARNED["rho"])
+    s.learned_set_hold_ratio(LEARNED["kappa"])
+    s.learned_set_mode(mode)
+
+
+def train_network(s, dt, eps):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", TRAIN["n"], TRAIN["seed"], [1.0, 20.0])
+    configure_learned(s, 0)
+    s.learned_reset(1)
+    s.set_state(m, pos, vel)
+    t0 = time.perf_counter()
+    s.learned_collect(TRAIN["steps"], dt, 2, LEARNED["exact_form"], eps)
+    loss0 = s.learned_train(1, TRAIN["batch"], TRAIN["lr"], 1)
+    loss = s.learned_train(TRAIN["iters"], TRAIN["batch"], TRAIN["lr"], 1)
+    s.synchronize()
+    return {"loss_first": loss0, "loss_final": loss, "seconds": time.perf_counter() - t0, **TRAIN}
+
+
+def run_form(s, torch, state, form, steps, warmup, dt, eps):
     m, pos, vel = state
     n = m.shape[0]
+    kform = "learned" if form.startswith("learned") else form
+    if form.startswith("learned"):
+        configure_learned(s, 1 if form == "learned-cubic" else 0)
     if warmup:
         s.set_state(m, pos, vel)
-        s.step(warmup, dt, 2, form, eps)
+        s.step(warmup, dt, 2, kform, eps)
     s.set_state(m, pos, vel)
     T0, U0 = s.energy(eps)
+    if form.startswith("learned"):   # counters start at the timed run
+        configure_learned(s, 1 if form == "learned-cubic" else 0)
+        s.set_state(m, pos, vel)
     s.timing(True)
     s.timing_reset()
-    if form == "learned":
-        s.learned_stats()  # reset happens at configure; read for the record below
     s.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    s.step(steps, dt, 2, form, eps)
+    s.step(steps, dt, 2, kform, eps)
     s.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -71,21 +119,19 @@ def run_form(s, torch, state, form, steps, warmup, dt, eps, learned_cfg=None):
     s.timing(False)
     T1, U1 = s.energy(eps)
     E0, E1 = T0 + U0, T1 + U1
-    out = {
+    return {
         "steps": steps,
         "wall_s": wall,
         "ms_per_step": 1e3 * wall / steps,
         "steps_per_s": steps / wall,
         "gpairs_per_s": float(n) * n * steps / wall / 1e9,
         "dE_E": abs((E0 - E1) / E0),
-        "force_evals": int(evals),
+        "exact_force_evals": int(evals),
         "force_kernel_ms_avg": force_ms / max(evals, 1),
-        "kernel_gpairs_per_s": float(n) * n / (force_ms / max(evals, 1) * 1e-3) / 1e9 if evals else None,
     }
-    return out
 
 
-def cpu_baseline(state, eps):
+def cpu_baseline(state):
     """The reference's compute_forces_avx512_rsqrt_4i_blocked on this host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     try:
@@ -108,15 +154,22 @@ def cpu_baseline(state, eps):
     from oracle import Oracle
     orc = Oracle()
     ni = 4096
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    orc.accel_block(pj[:64], pj, 0.0)   # warm
     t0 = time.perf_counter()
-    orc.accel(m[:ni], pos[:, :ni].copy(), eps)   # warm
-    sub = np.ascontiguousarray(pos)
-    t0 = time.perf_counter()
-    pj = np.stack([sub[0], sub[1], sub[2], m], axis=1)
-    orc.accel_block(pj[:ni], pj, eps)
+    orc.accel_block(pj[:ni], pj, 0.0)
     t = time.perf_counter() - t0
     return {"value": ni * n / t / 1e9, "unit": "GPairs/s", "cores": os.cpu_count(), "kind": "port",
             "sample": f"oracle C port, {ni} i-particles x {n} j-particles"}
+
+
+def roofline_entry(name, kernel_ms, n, peak, traffic=None, pairs="N^2 ordered pairs (20 flops each)"):
+    t = kernel_ms * 1e-3
+    achieved = FLOPS_PER_PAIR * float(n) * n / t / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": traffic, "kernel": name, "launch_avg_ms": kernel_ms,
+            "algorithmic_flops_per_launch": FLOPS_PER_PAIR * float(n) * n, "counted_as": pairs,
+            "algorithmic_bytes_per_launch": 64 * n}
 
 
 def main():
@@ -128,7 +181,7 @@ def main():
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=1.0 / 256)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--forms", default="fp64,mixed,learned")
+    ap.add_argument("--forms", default="fp64,mixed,learned,learned-cubic")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (a known-size "
@@ -144,8 +197,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": src.numel() * 4}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU ring is a design "
-              f"note, not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the ring over GPUs is a "
+              f"design note, not part of this build", file=sys.stderr)
         sys.exit(2)
 
     import torch  # only for the device-wide synchronisation around the timed region
@@ -157,46 +210,51 @@ def main():
     s.set_state(m, pos, vel)
     a64 = s.accel("fp64", args.eps)
 
-    forms = {}
+    forms, training = {}, None
     for form in args.forms.split(","):
         log(f"[bench] form {form}: N={n} steps={args.steps} warmup={args.warmup}")
         try:
-            if form == "learned":
-                s.learned_configure(period=4, threshold=1e-3, exact_form="mixed")
-                s.learned_reset(1)
-                s.set_state(m, pos, vel)
-                s.learned_collect(200, args.dt, 2, "mixed", args.eps)
-                loss = s.learned_train(2000, 4096, 1e-3, 1)
-                log(f"[bench] learned: training loss {loss:.3e}")
+            if form == "learned" and training is None:
+                training = train_network(s, args.dt, args.eps)
+                log(f"[bench] learned: trained {json.dumps(training)}")
             r = run_form(s, torch, state, form, args.steps, args.warmup, args.dt, args.eps)
-            if form != "learned":
+            if form.startswith("learned"):
+                st = s.learned_stats()
+                r["learned_stats"] = st
+                r["force_err_rms_at_check"] = st["last_check_err"]
+                r["force_err_max_at_check"] = st["last_check_max"]
+                r["config"] = dict(LEARNED, mode="cubic only" if form == "learned-cubic" else "cubic + network")
+            else:
                 s.set_state(m, pos, vel)
                 r["max_rel_force_err_vs_fp64"] = rel_err(s.accel(form, args.eps), a64) if form != "fp64" else 0.0
-            else:
-                r["learned_stats"] = s.learned_stats()
             forms[form] = r
             log(f"[bench] {form}: {json.dumps(r)}")
         except nbody_amd.NBodyError as e:
             forms[form] = {"error": str(e)}
             log(f"[bench] {form}: {e}")
+    if "fp64" in forms and "steps_per_s" in forms["fp64"]:
+        for r in forms.values():
+            if "steps_per_s" in r:
+                r["speedup_vs_fp64"] = r["steps_per_s"] / forms["fp64"]["steps_per_s"]
 
     ok = {k: v for k, v in forms.items() if "gpairs_per_s" in v and v["dE_E"] <= DE_BOUND}
     pool = ok if ok else {k: v for k, v in forms.items() if "gpairs_per_s" in v}
     head = max(pool, key=lambda k: pool[k]["gpairs_per_s"])
     hr = forms[head]
 
-    f64 = forms.get("fp64", {})
-    roofline = None
-    if "force_kernel_ms_avg" in f64 and f64["force_evals"]:
-        t = f64["force_kernel_ms_avg"] * 1e-3
-        achieved = FLOPS_PER_PAIR * float(n) * n / t / 1e12
-        roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "k_accel<FP64> (FP64 vector ALU; 20 flops per pair)",
-                    "algorithmic_bytes_per_launch": 32 * n + 32 * n,
-                    "launch_avg_ms": f64["force_kernel_ms_avg"]}
+    rooflines = {}
+    if forms.get("fp64", {}).get("exact_force_evals"):
+        rooflines["fp64"] = roofline_entry("k_accel_sym<FP64, K=8> (symmetric, FP64 VALU)",
+                                           forms["fp64"]["force_kernel_ms_avg"], n, FP64_PEAK_TFLOPS,
+                                           profile_traffic("k_accel_sym_plummer_n262144_pmc.md"))
+    if forms.get("mixed", {}).get("exact_force_evals"):
+        rooflines["mixed"] = roofline_entry("k_accel_sym<MIXED_NR, K=8> (symmetric, FP32 pair math on FP64 "
+                                            "differences; against the FP32 peak)",
+                                            forms["mixed"]["force_kernel_ms_avg"], n, FP32_PEAK_TFLOPS)
+    dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
+    roofline = rooflines.get(dominant) or rooflines.get("fp64")
 
-    cpu = None if args.no_cpu else cpu_baseline(state, args.eps)
+    cpu = None if args.no_cpu else cpu_baseline(state)
     line = {
         "metric": METRIC,
         "value": hr["gpairs_per_s"],
@@ -216,7 +274,9 @@ def main():
         "steps_per_s": hr["steps_per_s"],
         "dE_E": hr["dE_E"],
         "forms": forms,
+        "training": training,
         "roofline": roofline,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -169,22 +169,30 @@ NB_API nb_status nb_synchronize(nb_ctx* ctx);
 
 /* ---------------------------------------------------------------------------
  * Learned acceleration extrapolator (NB_FORM_LEARNED).  No counterpart in the
- * reference.  A 15 -> 128 -> 128 -> 128 -> 3 MLP (BF16 weights on MFMA, FP32
- * accumulate) corrects the cubic extrapolation a_{n+1} ~ 4a_n - 6a_{n-1} +
- * 4a_{n-2} - a_{n-3} from each particle's last four accelerations and its
- * velocity.  In nb_step with NB_FORM_LEARNED every `period`-th substep runs
- * the exact direct sum and compares it with the extrapolation; while the RMS
- * relative acceleration error exceeds `threshold` all substeps are exact.
- * Particles with a rough history (|third difference| > rho |a|, close
- * encounters) always get their exact rows of the direct sum.
+ * reference.  In nb_step with NB_FORM_LEARNED every `period`-th substep runs
+ * the exact direct sum (the configured exact form); the others extrapolate
+ * each particle's acceleration from its last four EXACT samples: the cubic
+ * through them (Lagrange weights at their substeps) plus, on evenly spaced
+ * histories, a 15 -> 128 -> 128 -> 128 -> 3 MLP correction (BF16 weights on
+ * MFMA, FP32 accumulate) in units of the third difference.  Each exact substep
+ * checks the extrapolation made for it; while the RMS relative error exceeds
+ * `threshold` every substep is exact, and the network is used only in cycles
+ * after it passed a check itself.  Particles with a rough history (|third
+ * difference| > rho |a|) or that missed the last check by more than kappa
+ * times the RMS acceleration get their exact rows of the direct sum.  With a
+ * near-field radius, pairs closer than it are summed exactly on every substep
+ * (sorted cell grid) and only the remainder is extrapolated.
  * ------------------------------------------------------------------------- */
 typedef struct nb_learned_stats {
     uint64_t exact_evals;      /* direct sums run by learned-form steps */
     uint64_t predicted_evals;  /* substeps served by the extrapolator */
     uint64_t fallbacks;        /* exact checks that exceeded the threshold */
-    uint64_t subset_rows;      /* rough particles summed exactly on extrapolated substeps */
+    uint64_t subset_rows;      /* rough or held particles summed exactly on extrapolated substeps */
     double last_check_err;     /* RMS relative acceleration error at the last check */
     double max_check_err;      /* largest such error since configure */
+    uint64_t net_evals;        /* extrapolated substeps that applied the network */
+    double last_cubic_err;     /* the cubic alone at the last check */
+    double last_check_max;     /* largest per-particle relative error at the last check */
 } nb_learned_stats;
 
 /* period >= 2 (exact every period-th substep), threshold > 0 (0 = always

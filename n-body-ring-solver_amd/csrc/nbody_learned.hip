@@ -525,7 +525,7 @@ nb_status exact_subset(nb_ctx* c, Learned& L, double eps, int kform) {
 
 // Compare the extrapolation in L.pred with the exact acc over the smooth
 // particles; update stats and the fallback state.  Synchronises the stream.
-nb_status check_sums(nb_ctx* c, Learned& L, const P4* pred, double& e2, double& a2) {
+nb_status check_sums(nb_ctx* c, Learned& L, const P4* pred, double& e2, double& a2, double& mx) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
     const long nb = (n + BLK - 1) / BLK;
@@ -538,9 +538,11 @@ nb_status check_sums(nb_ctx* c, Learned& L, const P4* pred, double& e2, double& 
     LH(hipStreamSynchronize(st), "hipStreamSynchronize");
     e2 = 0.0;
     a2 = 0.0;
+    mx = 0.0;
     for (long b = 0; b < nb; b++) {
         e2 += L.chk_h[3 * b];
         a2 += L.chk_h[3 * b + 1];
+        mx = std::max(mx, L.chk_h[3 * b + 2]);
     }
     return NB_OK;
 }
@@ -554,18 +556,20 @@ nb_status check(nb_ctx* c, Learned& L) {
     const long n = (long)L.n;
     // nothing checked (every particle rough or held, or all at rest) validates nothing
     auto rel = [](double e2, double a2) { return a2 > 0 ? std::sqrt(e2 / a2) : INFINITY; };
-    double e2c, a2;
-    if (nb_status e = check_sums(c, L, (const P4*)L.cub.p, e2c, a2)) return e;
+    double e2c, a2, mxc, mxn = INFINITY;
+    if (nb_status e = check_sums(c, L, (const P4*)L.cub.p, e2c, a2, mxc)) return e;
     const double err_c = rel(e2c, a2);
     double err_n = INFINITY;
     if (L.pred_net) {
         double e2n, a2n;
-        if (nb_status e = check_sums(c, L, (const P4*)L.pred.p, e2n, a2n)) return e;
+        if (nb_status e = check_sums(c, L, (const P4*)L.pred.p, e2n, a2n, mxn)) return e;
         err_n = rel(e2n, a2n);
     }
     L.net_ok = L.pred_net && (err_n <= L.threshold);   // NaN fails
     const double err = L.net_ok ? err_n : err_c;
     L.st.last_check_err = err;
+    L.st.last_check_max = L.net_ok ? mxn : mxc;
+    L.st.last_cubic_err = err_c;
     L.st.max_check_err = std::max(L.st.max_check_err, std::isnan(err) ? INFINITY : err);
     const bool ok = (err <= L.threshold);
     if (!ok) L.st.fallbacks++;
@@ -720,6 +724,7 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 }
                 if (nb_status e = exact_subset(c, L, eps, L.exact_kform)) return e;
                 L.st.predicted_evals++;
+                if (L.pred_net) L.st.net_evals++;
                 L.st.subset_rows += (uint64_t)L.nflag;
                 L.since_exact++;
             }

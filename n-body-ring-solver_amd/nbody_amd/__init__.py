@@ -57,7 +57,8 @@ class NBodyError(RuntimeError):
 
 class LearnedStats(C.Structure):
     _fields_ = [("exact_evals", C.c_uint64), ("predicted_evals", C.c_uint64), ("fallbacks", C.c_uint64),
-                ("subset_rows", C.c_uint64), ("last_check_err", C.c_double), ("max_check_err", C.c_double)]
+                ("subset_rows", C.c_uint64), ("last_check_err", C.c_double), ("max_check_err", C.c_double),
+                ("net_evals", C.c_uint64), ("last_cubic_err", C.c_double), ("last_check_max", C.c_double)]
 
 
 _lib = None
