@@ -7,7 +7,7 @@ each precision form:
   mixed         FP64 differences, FP32 rsqrt + Newton step and pair math;
   learned       exact (mixed) sum every 6th substep, the others extrapolated
                 from the last four exact samples (cubic + MLP correction) with
-                the pairs closer than rc = 0.4 summed exactly on every substep;
+                the pairs closer than rc = 0.35 summed exactly on every substep;
                 the MLP is trained first on 200 steps of a Plummer N = 65536;
   learned-cubic the same without the network (its ablation).
 A step is one leapfrog step of the whole system: one force evaluation (N^2
@@ -46,8 +46,9 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_PAIR = 20          # the conventional count for one softened pair interaction
 DE_BOUND = 1e-6
-LEARNED = {"period": 6, "threshold": 2e-3, "rc": 0.4, "rho": 0.03, "kappa": 1e-3, "exact_form": "mixed"}
-TRAIN = {"n": 65536, "steps": 200, "seed": 2, "iters": 3000, "batch": 4096, "lr": 1e-3}
+LEARNED = {"period": 6, "threshold": 3e-3, "rc": 0.35, "rho": 0.03, "kappa": 1e-3, "exact_form": "mixed"}
+TRAIN = {"n": 65536, "steps": 200, "seed": 2, "iters": 4000, "batch": 4096, "lr": 1e-3, "iters_fine": 4000,
+         "lr_fine": 2e-4}
 
 
 def log(*a):
@@ -87,7 +88,8 @@ def train_network(s, dt, eps):
     t0 = time.perf_counter()
     s.learned_collect(TRAIN["steps"], dt, 2, LEARNED["exact_form"], eps)
     loss0 = s.learned_train(1, TRAIN["batch"], TRAIN["lr"], 1)
-    loss = s.learned_train(TRAIN["iters"], TRAIN["batch"], TRAIN["lr"], 1)
+    s.learned_train(TRAIN["iters"], TRAIN["batch"], TRAIN["lr"], 1)
+    loss = s.learned_train(TRAIN["iters_fine"], TRAIN["batch"], TRAIN["lr_fine"], 2)
     s.synchronize()
     return {"loss_first": loss0, "loss_final": loss, "seconds": time.perf_counter() - t0, **TRAIN}
 

@@ -4,11 +4,15 @@
 // a "substep".  Every `period`-th substep is exact (the configured direct sum);
 // the others are extrapolated from the last four EXACT samples a_0..a_3 of
 // each particle (newest first, taken at substeps t_0 > t_1 > t_2 > t_3):
-//     a(t) ~ C(t) + |D3| tau w(t) f(x),   C = sum_k l_k(t) a_k,
+//     a(t) ~ C(t) + |D3| tau w(t) (f1 e1 + f2 e2 + f3 e3),   C = sum_k l_k(t) a_k,
 // where l_k are the Lagrange weights of the cubic through the four samples at
-// their substeps, D1..D3 the backward differences of the samples, x =
-// (a_0/|a_0|, D1/S, D2/S, D3/S, v/|v|) with S = |D1|+|D2|+|D3| (15 numbers)
-// and f the MLP of mlp_kernels.hip.  The network is trained on the residual of
+// their substeps, D1..D3 the backward differences of the samples, f = f(x) the
+// MLP of mlp_kernels.hip on 15 rotation- and scale-invariant numbers x (norms,
+// dot products and a triple product of a_0, D1..D3 and v, normalised; the
+// log of S / |a_0|, S = |D1|+|D2|+|D3|), and (e1, e2, e3) the particle's own
+// frame (e1 along D3, e2 in the D3-D2 plane): the correction turns with the
+// particle's history, as the physics does, instead of being learned per
+// lab-frame axis.  The network is trained on the residual of
 // the cubic one period after the newest sample of a history spaced `period`
 // substeps apart, in units of |D3| tau; at an earlier substep t = t_0 + k that
 // residual is taken in proportion to the nodal polynomial,
@@ -61,16 +65,61 @@ namespace {
 
 constexpr int BLK = 256;
 
-// Features, cubic (weights w0..w3 on the samples in slots s0..s3, newest
-// first) and output scale |D3| of particle i.
+__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Local frame of the network's output: e1 = D3/|D3|; e2 = the unit part of D2
+// orthogonal to e1 (of D1 when D2 is parallel to within 1e-6, else of the
+// coordinate axis least aligned with e1); e3 = e1 x e2.  Zero when D3 = 0.
+__device__ void frame_of(const double* d1, const double* d2, const double* d3, double* e) {
+    for (int k = 0; k < 9; k++) e[k] = 0.0;
+    const double n3 = sqrt(dot3(d3, d3));
+    if (!(n3 > 0.0)) return;
+    const double e1[3] = {d3[0] / n3, d3[1] / n3, d3[2] / n3};
+    double p[3];
+    bool found = false;
+    for (int c = 0; c < 2 && !found; c++) {
+        const double* q = c == 0 ? d2 : d1;
+        const double pr = dot3(q, e1), nq = sqrt(dot3(q, q));
+        for (int k = 0; k < 3; k++) p[k] = q[k] - pr * e1[k];
+        const double np_ = sqrt(dot3(p, p));
+        if (np_ > 1e-6 * nq && np_ > 0.0) {
+            for (int k = 0; k < 3; k++) p[k] /= np_;
+            found = true;
+        }
+    }
+    if (!found) {
+        int ax = 0;
+        for (int k = 1; k < 3; k++)
+            if (fabs(e1[k]) < fabs(e1[ax])) ax = k;
+        double u[3] = {0.0, 0.0, 0.0};
+        u[ax] = 1.0;
+        const double pr = dot3(u, e1);
+        for (int k = 0; k < 3; k++) p[k] = u[k] - pr * e1[k];
+        const double np_ = sqrt(dot3(p, p));
+        for (int k = 0; k < 3; k++) p[k] /= np_;
+    }
+    e[0] = e1[0]; e[1] = e1[1]; e[2] = e1[2];
+    e[3] = p[0]; e[4] = p[1]; e[5] = p[2];
+    e[6] = e1[1] * p[2] - e1[2] * p[1];
+    e[7] = e1[2] * p[0] - e1[0] * p[2];
+    e[8] = e1[0] * p[1] - e1[1] * p[0];
+}
+
+// Features of particle i from the samples in slots s0..s3 (newest first):
+// backward differences D1..D3, S = |D1|+|D2|+|D3|, unit vectors u0 = a0/|a0|,
+// u1..u3 = D1..D3 / S, u4 = v/|v|, and the 15 rotation- and scale-invariant
+// numbers (|u1|, |u2|, |u3|, u0.u1, u0.u2, u0.u3, u0.u4, u1.u2, u1.u3, u1.u4,
+// u2.u3, u2.u4, u3.u4, clamp(log10(S/|a0|), -8, 2)/4 (-2 for 0), 10 u1.(u2 x u3));
+// also the cubic with weights w, the scale |D3|, |a0| and the output frame.
 __device__ __forceinline__ void features(const P4* __restrict__ hist, int4 slot, double4 w, long n, long i,
-                                         const P4& v, float* x, double* cubic, double& scale, double& anorm) {
+                                         const P4& v, float* x, double* cubic, double& scale, double& anorm,
+                                         double* e) {
     const P4& a0 = hist[(long)slot.x * n + i];
     const P4& a1 = hist[(long)slot.y * n + i];
     const P4& a2 = hist[(long)slot.z * n + i];
     const P4& a3 = hist[(long)slot.w * n + i];
     const double A0[3] = {a0.x, a0.y, a0.z}, A1[3] = {a1.x, a1.y, a1.z}, A2[3] = {a2.x, a2.y, a2.z},
-                 A3[3] = {a3.x, a3.y, a3.z};
+                 A3[3] = {a3.x, a3.y, a3.z}, V[3] = {v.x, v.y, v.z};
     double d1[3], d2[3], d3[3];
     for (int k = 0; k < 3; k++) {
         d1[k] = A0[k] - A1[k];
@@ -78,24 +127,38 @@ __device__ __forceinline__ void features(const P4* __restrict__ hist, int4 slot,
         d3[k] = A0[k] - 3.0 * A1[k] + 3.0 * A2[k] - A3[k];
         cubic[k] = w.x * A0[k] + w.y * A1[k] + w.z * A2[k] + w.w * A3[k];
     }
-    const double n0 = sqrt(A0[0] * A0[0] + A0[1] * A0[1] + A0[2] * A0[2]);
-    const double n1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
-    const double n2 = sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2]);
-    const double n3 = sqrt(d3[0] * d3[0] + d3[1] * d3[1] + d3[2] * d3[2]);
-    const double nv = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
-    const double S = n1 + n2 + n3;
+    const double n0 = sqrt(dot3(A0, A0)), n1 = sqrt(dot3(d1, d1)), n2 = sqrt(dot3(d2, d2)), n3 = sqrt(dot3(d3, d3));
+    const double nv = sqrt(dot3(V, V)), S = n1 + n2 + n3;
     const double i0 = n0 > 0 ? 1.0 / n0 : 0.0, iS = S > 0 ? 1.0 / S : 0.0, iv = nv > 0 ? 1.0 / nv : 0.0;
-    const double V[3] = {v.x, v.y, v.z};
+    double u[5][3];
     for (int k = 0; k < 3; k++) {
-        x[k] = (float)(A0[k] * i0);
-        x[3 + k] = (float)(d1[k] * iS);
-        x[6 + k] = (float)(d2[k] * iS);
-        x[9 + k] = (float)(d3[k] * iS);
-        x[12 + k] = (float)(V[k] * iv);
+        u[0][k] = A0[k] * i0;
+        u[1][k] = d1[k] * iS;
+        u[2][k] = d2[k] * iS;
+        u[3][k] = d3[k] * iS;
+        u[4][k] = V[k] * iv;
     }
+    x[0] = (float)(n1 * iS);
+    x[1] = (float)(n2 * iS);
+    x[2] = (float)(n3 * iS);
+    x[3] = (float)dot3(u[0], u[1]);
+    x[4] = (float)dot3(u[0], u[2]);
+    x[5] = (float)dot3(u[0], u[3]);
+    x[6] = (float)dot3(u[0], u[4]);
+    x[7] = (float)dot3(u[1], u[2]);
+    x[8] = (float)dot3(u[1], u[3]);
+    x[9] = (float)dot3(u[1], u[4]);
+    x[10] = (float)dot3(u[2], u[3]);
+    x[11] = (float)dot3(u[2], u[4]);
+    x[12] = (float)dot3(u[3], u[4]);
+    x[13] = (S > 0 && n0 > 0) ? (float)(fmin(fmax(log10(S / n0), -8.0), 2.0) / 4.0) : -2.0f;
+    const double c23[3] = {u[2][1] * u[3][2] - u[2][2] * u[3][1], u[2][2] * u[3][0] - u[2][0] * u[3][2],
+                           u[2][0] * u[3][1] - u[2][1] * u[3][0]};
+    x[14] = (float)(10.0 * dot3(u[1], c23));
     x[15] = 0.f;
     scale = n3;
     anorm = n0;
+    frame_of(d1, d2, d3, e);
 }
 
 // X[i] = features, cub[i] = (cubic, |D3|); with `mask`, particles whose
@@ -103,17 +166,19 @@ __device__ __forceinline__ void features(const P4* __restrict__ hist, int4 slot,
 // missed check are marked and listed.
 __global__ __launch_bounds__(BLK) void k_features(const P4* __restrict__ hist, int4 slot, double4 w, long n,
                                                   const P4* __restrict__ vel, float* __restrict__ X,
-                                                  P4* __restrict__ cub, double rho, unsigned char* __restrict__ mask,
-                                                  int* __restrict__ flags, int* __restrict__ nflags,
-                                                  const unsigned char* __restrict__ held) {
+                                                  P4* __restrict__ cub, float4* __restrict__ frame, double rho,
+                                                  unsigned char* __restrict__ mask, int* __restrict__ flags,
+                                                  int* __restrict__ nflags, const unsigned char* __restrict__ held) {
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     float x[16];
-    double c[3], s, a0;
-    features(hist, slot, w, n, i, vel[i], x, c, s, a0);
+    double c[3], s, a0, e[9];
+    features(hist, slot, w, n, i, vel[i], x, c, s, a0, e);
     float4* xo = (float4*)(X + i * 16);
     for (int q = 0; q < 4; q++) xo[q] = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
     cub[i] = P4{c[0], c[1], c[2], s};
+    for (int k = 0; k < 3; k++)
+        frame[3 * i + k] = make_float4((float)e[3 * k], (float)e[3 * k + 1], (float)e[3 * k + 2], 0.f);
     if (mask) {
         const bool rough = !(s <= rho * a0) || held[i] != 0;
         mask[i] = rough ? 1 : 0;
@@ -121,25 +186,44 @@ __global__ __launch_bounds__(BLK) void k_features(const P4* __restrict__ hist, i
     }
 }
 
-// out = cubic + |D3| tau y  (use_net) or cubic
-__global__ __launch_bounds__(BLK) void k_combine(const P4* __restrict__ cub, const float* __restrict__ Y, float tau,
+// out = cubic + the network's correction |D3| tau (y1 e1 + y2 e2 + y3 e3)
+// with its component along the velocity removed (use_net), or the cubic: a
+// correction orthogonal to v does no work, so the network cannot make the
+// energy drift (a learned correction with a bias along v would heat or cool
+// the system however small its RMS error).
+__global__ __launch_bounds__(BLK) void k_combine(const P4* __restrict__ cub, const float4* __restrict__ frame,
+                                                 const float* __restrict__ Y, const P4* __restrict__ vel, float tau,
                                                  int use_net, long n, P4* __restrict__ out) {
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     P4 c = cub[i];
     if (use_net) {
         const float4 y = *(const float4*)(Y + i * 4);
+        const float4 e1 = frame[3 * i], e2 = frame[3 * i + 1], e3 = frame[3 * i + 2];
         const double f = c.m * (double)tau;
-        c.x += f * y.x;
-        c.y += f * y.y;
-        c.z += f * y.z;
+        double dx = f * ((double)y.x * e1.x + (double)y.y * e2.x + (double)y.z * e3.x);
+        double dy = f * ((double)y.x * e1.y + (double)y.y * e2.y + (double)y.z * e3.y);
+        double dz = f * ((double)y.x * e1.z + (double)y.y * e2.z + (double)y.z * e3.z);
+        const P4 v = vel[i];
+        const double v2 = v.x * v.x + v.y * v.y + v.z * v.z;
+        if (v2 > 0.0) {
+            const double pr = (dx * v.x + dy * v.y + dz * v.z) / v2;
+            dx -= pr * v.x;
+            dy -= pr * v.y;
+            dz -= pr * v.z;
+        }
+        c.x += dx;
+        c.y += dy;
+        c.z += dz;
     }
     out[i] = P4{c.x, c.y, c.z, 0.0};
 }
 
-// Training pair: T[row] = (a_exact - a_near - cubic) / |D3|  (0 where |D3| = 0)
-__global__ __launch_bounds__(BLK) void k_targets(const P4* __restrict__ cub, const P4* __restrict__ acc,
-                                                 const P4* __restrict__ near, long n, float* __restrict__ T) {
+// Training pair: T[row] = components of (a_exact - a_near - cubic) / |D3| in
+// the particle's frame (0 where |D3| = 0)
+__global__ __launch_bounds__(BLK) void k_targets(const P4* __restrict__ cub, const float4* __restrict__ frame,
+                                                 const P4* __restrict__ acc, const P4* __restrict__ near, long n,
+                                                 float* __restrict__ T) {
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     const P4 c = cub[i];
@@ -149,8 +233,13 @@ __global__ __launch_bounds__(BLK) void k_targets(const P4* __restrict__ cub, con
         a.x -= q.x; a.y -= q.y; a.z -= q.z;
     }
     const double inv = c.m > 0 ? 1.0 / c.m : 0.0;
-    *(float4*)(T + i * 4) = make_float4((float)((a.x - c.x) * inv), (float)((a.y - c.y) * inv),
-                                        (float)((a.z - c.z) * inv), 0.f);
+    const double rx = (a.x - c.x) * inv, ry = (a.y - c.y) * inv, rz = (a.z - c.z) * inv;
+    float t[3];
+    for (int k = 0; k < 3; k++) {
+        const float4 e = frame[3 * i + k];
+        t[k] = (float)(rx * e.x + ry * e.y + rz * e.z);
+    }
+    *(float4*)(T + i * 4) = make_float4(t[0], t[1], t[2], 0.f);
 }
 
 // Per block, over the particles not marked rough: sum |pred - (acc - near)|^2,
@@ -303,7 +392,7 @@ struct Learned {
     DevBuf bX, bT, bY, bZ, bH, bD, dbpart, losspart;
     // per-particle state
     size_t n = 0;
-    DevBuf hist, X, Y, cub, pred, chk, mask, held, flags, nflags, sub_p, sub_part, sub_acc, nearacc;
+    DevBuf hist, X, Y, cub, frame, pred, chk, mask, held, flags, nflags, sub_p, sub_part, sub_acc, nearacc;
     int head = 3, count = 0;   // ring of the last four exact samples (head newest)
     long htime[4] = {0, 0, 0, 0};   // their substep indices
     long sub = 0;              // substeps since the state was set
@@ -400,6 +489,7 @@ nb_status ensure_particles(Learned& L, size_t n) {
     LH(L.X.ensure(m * 16 * sizeof(float)), "allocate features");
     LH(L.Y.ensure(m * 4 * sizeof(float)), "allocate network output");
     LH(L.cub.ensure(m * sizeof(P4)), "allocate extrapolation");
+    LH(L.frame.ensure(m * 3 * sizeof(float4)), "allocate output frames");
     LH(L.pred.ensure(m * sizeof(P4)), "allocate prediction");
     LH(L.chk.ensure(((m + BLK - 1) / BLK) * 3 * sizeof(double)), "allocate check sums");
     LH(L.mask.ensure(m), "allocate roughness mask");
@@ -467,14 +557,15 @@ nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
     LH(hipMemsetAsync(L.nflags.p, 0, sizeof(int), st), "clear flag counter");
     hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, ring_slots(L),
                        make_double4(W.l[0], W.l[1], W.l[2], W.l[3]), n, (const P4*)nbh::ctx_vel(c), (float*)L.X.p,
-                       (P4*)L.cub.p, L.rho, (unsigned char*)L.mask.p, (int*)L.flags.p, (int*)L.nflags.p,
-                       (const unsigned char*)L.held.p);
+                       (P4*)L.cub.p, (float4*)L.frame.p, L.rho, (unsigned char*)L.mask.p, (int*)L.flags.p,
+                       (int*)L.nflags.p, (const unsigned char*)L.held.p);
     LH(hipGetLastError(), "features kernel");
     const bool use_net = allow_net && L.mode == 0 && L.trained && W.wnet != 0.0;
     L.pred_net = use_net;
     if (use_net) LH(mlp::forward(L.net, (const float*)L.X.p, (int)n, (float*)L.Y.p, false, nullptr, nullptr, 0, st), "network forward");
-    hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float*)L.Y.p,
-                       (float)(L.tau * W.wnet), use_net ? 1 : 0, n, out);
+    hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float4*)L.frame.p,
+                       (const float*)L.Y.p, (const P4*)nbh::ctx_vel(c), (float)(L.tau * W.wnet), use_net ? 1 : 0, n,
+                       out);
     LH(hipGetLastError(), "combine kernel");
     if (count) {
         int nf = 0;
@@ -615,10 +706,11 @@ nb_status record(nb_ctx* c, Learned& L, const DevBuf& ring, long R, long t) {
     float* T = (float*)L.dsT.p + L.ds_rows * 4;
     const int4 slot = make_int4((int)((t - P) % R), (int)((t - 2 * P) % R), (int)((t - 3 * P) % R), (int)((t - 4 * P) % R));
     hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)ring.p, slot, make_double4(4.0, -6.0, 4.0, -1.0),
-                       n, (const P4*)nbh::ctx_vel(c), X, (P4*)L.cub.p, 0.0, nullptr, nullptr, nullptr, nullptr);
+                       n, (const P4*)nbh::ctx_vel(c), X, (P4*)L.cub.p, (float4*)L.frame.p, 0.0, nullptr, nullptr,
+                       nullptr, nullptr);
     LH(hipGetLastError(), "features kernel");
-    hipLaunchKernelGGL(k_targets, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const P4*)nbh::ctx_acc(c),
-                       L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, n, T);
+    hipLaunchKernelGGL(k_targets, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float4*)L.frame.p,
+                       (const P4*)nbh::ctx_acc(c), L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, n, T);
     LH(hipGetLastError(), "targets kernel");
     L.ds_rows += n;
     return NB_OK;
@@ -661,7 +753,7 @@ void destroy(Learned* l) {
     if (!l) return;
     l->near.release();
     for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
-                      &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub,
+                      &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub, &l->frame,
                       &l->pred, &l->chk, &l->mask, &l->held, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc,
                       &l->nearacc})
         b->release();

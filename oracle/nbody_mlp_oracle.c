@@ -180,44 +180,111 @@ ORC_API void orc_lagrange4(const double *t, double x, double *w)
     }
 }
 
-/* Extrapolation features of particle i from its last four accelerations
- * a0 = a_n, a1 = a_{n-1}, a2, a3 (each 3 doubles) and velocity v:
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Local frame of the network's output: e1 = D3/|D3|; e2 = the unit part of D2
+ * orthogonal to e1 (of D1 when D2 is parallel to within 1e-6, else of the
+ * coordinate axis least aligned with e1); e3 = e1 x e2.  All zero when
+ * D3 = 0. */
+static void frame_of(const double *d1, const double *d2, const double *d3, double e[9])
+{
+    memset(e, 0, 9 * sizeof(double));
+    double n3 = sqrt(dot3(d3, d3));
+    if (!(n3 > 0.0)) return;
+    double e1[3] = {d3[0] / n3, d3[1] / n3, d3[2] / n3}, p[3];
+    const double *cand[2] = {d2, d1};
+    int found = 0;
+    for (int c = 0; c < 2 && !found; c++) {
+        double pr = dot3(cand[c], e1), nc = sqrt(dot3(cand[c], cand[c]));
+        for (int k = 0; k < 3; k++) p[k] = cand[c][k] - pr * e1[k];
+        double np_ = sqrt(dot3(p, p));
+        if (np_ > 1e-6 * nc && np_ > 0.0) {
+            for (int k = 0; k < 3; k++) p[k] /= np_;
+            found = 1;
+        }
+    }
+    if (!found) {
+        int ax = 0;
+        for (int k = 1; k < 3; k++)
+            if (fabs(e1[k]) < fabs(e1[ax])) ax = k;
+        double u[3] = {0.0, 0.0, 0.0};
+        u[ax] = 1.0;
+        double pr = dot3(u, e1);
+        for (int k = 0; k < 3; k++) p[k] = u[k] - pr * e1[k];
+        double np_ = sqrt(dot3(p, p));
+        for (int k = 0; k < 3; k++) p[k] /= np_;
+    }
+    e[0] = e1[0]; e[1] = e1[1]; e[2] = e1[2];
+    e[3] = p[0]; e[4] = p[1]; e[5] = p[2];
+    e[6] = e1[1] * p[2] - e1[2] * p[1];
+    e[7] = e1[2] * p[0] - e1[0] * p[2];
+    e[8] = e1[0] * p[1] - e1[1] * p[0];
+}
+
+/* Extrapolation features of particle i from its last four (exact) samples
+ * a0 = newest .. a3 (each 3 doubles) and velocity v:
  *   D1 = a0 - a1, D2 = a0 - 2a1 + a2, D3 = a0 - 3a1 + 3a2 - a3 (backward
- *   differences; the cubic extrapolation is a0 + D1 + D2 + D3),
- *   S = |D1| + |D2| + |D3|,
- *   x = (a0/|a0|, D1/S, D2/S, D3/S, v/|v|)   (15 numbers, each <= 1),
- * and the output scale |D3|: the network predicts the extrapolation residual
- * (a_{n+1} - cubic) / |D3|.  Zero norms give zero features. */
+ *   differences; for equally spaced samples the cubic one spacing ahead is
+ *   a0 + D1 + D2 + D3), S = |D1| + |D2| + |D3|,
+ *   u0 = a0/|a0|, u1 = D1/S, u2 = D2/S, u3 = D3/S, u4 = v/|v| (0 for 0/0),
+ *   x = (|u1|, |u2|, |u3|, u0.u1, u0.u2, u0.u3, u0.u4, u1.u2, u1.u3, u1.u4,
+ *        u2.u3, u2.u4, u3.u4, clamp(log10(S/|a0|), -8, 2)/4 (-2 if S or |a0|
+ *        is 0), 10 u1.(u2 x u3))
+ * -- 15 numbers invariant under rotations and under scaling of the
+ * accelerations and of v; the output scale |D3|, the local frame (frame_of)
+ * in which the network's 3 outputs are components, and the equally spaced
+ * cubic.  The network predicts the residual (a_next - cubic) / |D3| in that
+ * frame, so its correction turns with the particle's history. */
 ORC_API void orc_features(size_t n, const double *a0, const double *a1, const double *a2,
                           const double *a3, const double *v, double *x, double *scale,
-                          double *cubic)
+                          double *cubic, double *frame)
 {
     for (size_t i = 0; i < n; i++) {
-        double d1[3], d2[3], d3[3], c[3];
+        double d1[3], d2[3], d3[3];
+        const double *A0 = a0 + 3 * i, *V = v + 3 * i;
         for (int k = 0; k < 3; k++) {
             double p0 = a0[3 * i + k], p1 = a1[3 * i + k], p2 = a2[3 * i + k], p3 = a3[3 * i + k];
             d1[k] = p0 - p1;
             d2[k] = p0 - 2.0 * p1 + p2;
             d3[k] = p0 - 3.0 * p1 + 3.0 * p2 - p3;
-            c[k] = p0 + d1[k] + d2[k] + d3[k];
+            cubic[3 * i + k] = p0 + d1[k] + d2[k] + d3[k];
         }
-        double n0 = sqrt(a0[3 * i] * a0[3 * i] + a0[3 * i + 1] * a0[3 * i + 1] + a0[3 * i + 2] * a0[3 * i + 2]);
-        double n1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
-        double n2 = sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2]);
-        double n3 = sqrt(d3[0] * d3[0] + d3[1] * d3[1] + d3[2] * d3[2]);
-        double nv = sqrt(v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
-        double S = n1 + n2 + n3;
+        double n0 = sqrt(dot3(A0, A0)), n1 = sqrt(dot3(d1, d1)), n2 = sqrt(dot3(d2, d2)), n3 = sqrt(dot3(d3, d3));
+        double nv = sqrt(dot3(V, V)), S = n1 + n2 + n3;
         double i0 = n0 > 0 ? 1.0 / n0 : 0.0, iS = S > 0 ? 1.0 / S : 0.0, iv = nv > 0 ? 1.0 / nv : 0.0;
-        double *xr = x + 15 * i;
+        double u[5][3];
         for (int k = 0; k < 3; k++) {
-            xr[k] = a0[3 * i + k] * i0;
-            xr[3 + k] = d1[k] * iS;
-            xr[6 + k] = d2[k] * iS;
-            xr[9 + k] = d3[k] * iS;
-            xr[12 + k] = v[3 * i + k] * iv;
-            cubic[3 * i + k] = c[k];
+            u[0][k] = A0[k] * i0;
+            u[1][k] = d1[k] * iS;
+            u[2][k] = d2[k] * iS;
+            u[3][k] = d3[k] * iS;
+            u[4][k] = V[k] * iv;
         }
+        double *xr = x + 15 * i;
+        xr[0] = n1 * iS;
+        xr[1] = n2 * iS;
+        xr[2] = n3 * iS;
+        xr[3] = dot3(u[0], u[1]);
+        xr[4] = dot3(u[0], u[2]);
+        xr[5] = dot3(u[0], u[3]);
+        xr[6] = dot3(u[0], u[4]);
+        xr[7] = dot3(u[1], u[2]);
+        xr[8] = dot3(u[1], u[3]);
+        xr[9] = dot3(u[1], u[4]);
+        xr[10] = dot3(u[2], u[3]);
+        xr[11] = dot3(u[2], u[4]);
+        xr[12] = dot3(u[3], u[4]);
+        if (S > 0 && n0 > 0) {
+            double l = log10(S / n0);
+            xr[13] = (l < -8.0 ? -8.0 : (l > 2.0 ? 2.0 : l)) / 4.0;
+        } else {
+            xr[13] = -2.0;
+        }
+        double c23[3] = {u[2][1] * u[3][2] - u[2][2] * u[3][1], u[2][2] * u[3][0] - u[2][0] * u[3][2],
+                         u[2][0] * u[3][1] - u[2][1] * u[3][0]};
+        xr[14] = 10.0 * dot3(u[1], c23);
         scale[i] = n3;
+        frame_of(d1, d2, d3, frame + 9 * i);
     }
 }
 

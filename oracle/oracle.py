@@ -65,7 +65,7 @@ class Oracle:
         L.orc_mlp_grad.argtypes = [_D, _SZ, _D, _D, C.c_int, _D]
         L.orc_mlp_grad.restype = C.c_double
         L.orc_adam.argtypes = [_SZ, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
-        L.orc_features.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_features.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _D, _D]
         L.orc_lagrange4.argtypes = [_D, C.c_double, _D]
         L.orc_near_accel.argtypes = [_SZ, _D, _D, _D, _D, C.c_double, C.c_double, _D, _D, _D]
         L.orc_near_taper.argtypes = [C.c_double, C.c_double]
@@ -189,12 +189,13 @@ class Oracle:
         self.lib.orc_adam(p.shape[0], _p(p), _p(g), _p(m), _p(v), lr, b1, b2, eps, t)
 
     def features(self, a0, a1, a2, a3, v):
-        """(n, 3) arrays -> features (n, 15), scale (n,), cubic (n, 3)."""
+        """(n, 3) arrays -> features (n, 15), scale (n,), cubic (n, 3), frame (n, 3, 3)
+        (frame[i, k] is the unit vector e_{k+1} of particle i)."""
         n = a0.shape[0]
-        x, s, cub = np.empty((n, 15)), np.empty(n), np.empty((n, 3))
+        x, s, cub, fr = np.empty((n, 15)), np.empty(n), np.empty((n, 3)), np.empty((n, 3, 3))
         args = [np.ascontiguousarray(a, dtype=np.float64) for a in (a0, a1, a2, a3, v)]
-        self.lib.orc_features(n, *[_p(a) for a in args], _p(x), _p(s), _p(cub))
-        return x, s, cub
+        self.lib.orc_features(n, *[_p(a) for a in args], _p(x), _p(s), _p(cub), _p(fr))
+        return x, s, cub, fr
 
     def lagrange4(self, t, x: float) -> np.ndarray:
         t = np.ascontiguousarray(t, dtype=np.float64)

@@ -70,24 +70,63 @@ def test_cubic_extrapolation_exact_on_cubics(orc):
     assert abs(orc.poly_extrap(*[np.array([g(t)]) for t in (3.0, 2.0, 1.0, 0.0)])[0] - g(4.0)) == 24.0
 
 
-def test_features_scale_invariance(orc):
+def random_rotation(rng):
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def test_features_invariance_and_frame(orc):
+    """Features are invariant under rotations and under scaling of a and v;
+    the frame is orthonormal, right-handed, e1 along D3, e2 in the D3-D2
+    plane, and turns with the data."""
     rng = np.random.default_rng(2)
     a = [rng.standard_normal((50, 3)) for _ in range(4)]
     v = rng.standard_normal((50, 3))
-    x, s, cub = orc.features(*a, v)
-    x2, s2, cub2 = orc.features(*[7.5 * ai for ai in a], 0.1 * v)
-    np.testing.assert_allclose(x2, x, rtol=1e-12, atol=1e-14)
+    x, s, cub, fr = orc.features(*a, v)
+    R = random_rotation(rng)
+    x2, s2, cub2, fr2 = orc.features(*[7.5 * ai @ R.T for ai in a], 0.1 * v @ R.T)
+    np.testing.assert_allclose(x2, x, rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(s2, 7.5 * s, rtol=1e-12)
+    np.testing.assert_allclose(fr2, fr @ R.T, rtol=0, atol=1e-10)
     np.testing.assert_allclose(cub, 4 * a[0] - 6 * a[1] + 4 * a[2] - a[3], rtol=1e-12, atol=1e-12)
+    d2 = a[0] - 2 * a[1] + a[2]
     d3 = a[0] - 3 * a[1] + 3 * a[2] - a[3]
     np.testing.assert_allclose(s, np.linalg.norm(d3, axis=1), rtol=1e-12)
-    assert np.all(np.abs(x) <= 1.0 + 1e-12)
+    for i in range(50):
+        E = fr[i]
+        np.testing.assert_allclose(E @ E.T, np.eye(3), atol=1e-12)
+        assert np.linalg.det(E) > 0.999
+        np.testing.assert_allclose(E[0], d3[i] / np.linalg.norm(d3[i]), atol=1e-12)
+        assert abs(np.dot(E[2], d2[i])) < 1e-10 * np.linalg.norm(d2[i])
+    assert np.all(np.abs(x[:, :13]) <= 1.0 + 1e-12) and np.all(np.abs(x[:, 13]) <= 2.0)
 
 
-def test_features_zero_history(orc):
+def test_features_reflection_flips_only_the_pseudoscalar(orc):
+    rng = np.random.default_rng(4)
+    a = [rng.standard_normal((20, 3)) for _ in range(4)]
+    v = rng.standard_normal((20, 3))
+    x, _, _, _ = orc.features(*a, v)
+    M = np.diag([1.0, 1.0, -1.0])
+    xm, _, _, _ = orc.features(*[ai @ M for ai in a], v @ M)
+    np.testing.assert_allclose(xm[:, :14], x[:, :14], atol=1e-12)
+    np.testing.assert_allclose(xm[:, 14], -x[:, 14], atol=1e-12)
+
+
+def test_features_degenerate_frames(orc):
+    """D2 parallel to D3 falls back to D1; all three parallel to an axis
+    choice; a zero history gives zero features, scale and frame."""
     z = np.zeros((3, 3))
-    x, s, cub = orc.features(z, z, z, z, z)
-    assert np.all(x == 0) and np.all(s == 0) and np.all(cub == 0)
+    x, s, cub, fr = orc.features(z, z, z, z, z)
+    assert np.all(s == 0) and np.all(cub == 0) and np.all(fr == 0)
+    assert np.all(x[:, :13] == 0) and np.all(x[:, 13] == -2.0) and np.all(x[:, 14] == 0)
+    # samples on a line: every difference along one direction
+    d = np.array([[0.0, 0.6, 0.8]])
+    a = [d * c for c in (1.0, 2.0, 4.0, 8.0)]
+    _, _, _, fr = orc.features(*a, d)
+    E = fr[0]
+    np.testing.assert_allclose(E @ E.T, np.eye(3), atol=1e-12)
+    np.testing.assert_allclose(np.abs(E[0]), np.abs(d[0]), atol=1e-12)
 
 
 def test_near_taper_properties(orc):
