@@ -5,11 +5,15 @@ eps = 0.005, dt = 1/256, 2nd-order DKD leapfrog, K steps (default 1000), for
 each precision form:
   fp64          plain FP64 direct sum (symmetric kernel);
   mixed         FP64 differences, FP32 rsqrt + Newton step and pair math;
-  learned       exact (mixed) sum every 6th substep, the others extrapolated
+  learned       exact (mixed) sum every 4th substep, the other 3 extrapolated
                 from the last four exact samples (cubic + MLP correction) with
-                the pairs closer than rc = 0.35 summed exactly on every substep;
+                the pairs closer than rc = 0.25 summed exactly on every substep;
                 the MLP is trained first on 200 steps of a Plummer N = 65536;
-  learned-cubic the same without the network (its ablation).
+  learned-cubic the same without the network (its ablation);
+  learned-p8    cubic extrapolation with an exact sum every 8th substep and
+                rc = 0.35: how far the schedule stretches at the same energy
+                bound (reported, but outside the 3-of-4 form, so never the
+                headline).
 A step is one leapfrog step of the whole system: one force evaluation (N^2
 ordered pairs) plus the kick/drift.  The state is resident in HBM before the
 timed region starts.
@@ -46,7 +50,9 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_PAIR = 20          # the conventional count for one softened pair interaction
 DE_BOUND = 1e-6
-LEARNED = {"period": 6, "threshold": 3e-3, "rc": 0.35, "rho": 0.03, "kappa": 1e-3, "exact_form": "mixed"}
+LEARNED = {"period": 4, "threshold": 3e-3, "rc": 0.25, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
+LEARNED_P8 = {"period": 8, "threshold": 4e-3, "rc": 0.35, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
+HEADLINE_FORMS = ("fp64", "mixed", "learned", "learned-cubic")
 TRAIN = {"n": 65536, "steps": 200, "seed": 2, "iters": 4000, "batch": 4096, "lr": 1e-3, "iters_fine": 4000,
          "lr_fine": 2e-4}
 
@@ -72,17 +78,22 @@ def profile_traffic(name: str):
         return None
 
 
-def configure_learned(s, mode):
-    s.learned_configure(period=LEARNED["period"], threshold=LEARNED["threshold"], exact_form=LEARNED["exact_form"])
-    s.learned_set_near_radius(LEARNED["rc"])
-    s.learned_set_flag_ratio(LEARNED["rho"])
-    s.learned_set_hold_ratio(LEARNED["kappa"])
-    s.learned_set_mode(mode)
+def learned_config(form):
+    return LEARNED_P8 if form == "learned-p8" else LEARNED
+
+
+def configure_learned(s, form):
+    cfg = learned_config(form)
+    s.learned_configure(period=cfg["period"], threshold=cfg["threshold"], exact_form=cfg["exact_form"])
+    s.learned_set_near_radius(cfg["rc"])
+    s.learned_set_flag_ratio(cfg["rho"])
+    s.learned_set_hold_ratio(cfg["kappa"])
+    s.learned_set_mode(0 if form == "learned" else 1)
 
 
 def train_network(s, dt, eps):
     m, pos, vel, _ = nbody_amd.init_system("plummer", TRAIN["n"], TRAIN["seed"], [1.0, 20.0])
-    configure_learned(s, 0)
+    configure_learned(s, "learned")
     s.learned_reset(1)
     s.set_state(m, pos, vel)
     t0 = time.perf_counter()
@@ -99,14 +110,14 @@ def run_form(s, torch, state, form, steps, warmup, dt, eps):
     n = m.shape[0]
     kform = "learned" if form.startswith("learned") else form
     if form.startswith("learned"):
-        configure_learned(s, 1 if form == "learned-cubic" else 0)
+        configure_learned(s, form)
     if warmup:
         s.set_state(m, pos, vel)
         s.step(warmup, dt, 2, kform, eps)
     s.set_state(m, pos, vel)
     T0, U0 = s.energy(eps)
     if form.startswith("learned"):   # counters start at the timed run
-        configure_learned(s, 1 if form == "learned-cubic" else 0)
+        configure_learned(s, form)
         s.set_state(m, pos, vel)
     s.timing(True)
     s.timing_reset()
@@ -183,7 +194,7 @@ def main():
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=1.0 / 256)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--forms", default="fp64,mixed,learned,learned-cubic")
+    ap.add_argument("--forms", default="fp64,mixed,learned,learned-cubic,learned-p8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (a known-size "
@@ -225,7 +236,7 @@ def main():
                 r["learned_stats"] = st
                 r["force_err_rms_at_check"] = st["last_check_err"]
                 r["force_err_max_at_check"] = st["last_check_max"]
-                r["config"] = dict(LEARNED, mode="cubic only" if form == "learned-cubic" else "cubic + network")
+                r["config"] = dict(learned_config(form), mode="cubic + network" if form == "learned" else "cubic only")
             else:
                 s.set_state(m, pos, vel)
                 r["max_rel_force_err_vs_fp64"] = rel_err(s.accel(form, args.eps), a64) if form != "fp64" else 0.0
@@ -239,8 +250,8 @@ def main():
             if "steps_per_s" in r:
                 r["speedup_vs_fp64"] = r["steps_per_s"] / forms["fp64"]["steps_per_s"]
 
-    ok = {k: v for k, v in forms.items() if "gpairs_per_s" in v and v["dE_E"] <= DE_BOUND}
-    pool = ok if ok else {k: v for k, v in forms.items() if "gpairs_per_s" in v}
+    ok = {k: v for k, v in forms.items() if "gpairs_per_s" in v and v["dE_E"] <= DE_BOUND and k in HEADLINE_FORMS}
+    pool = ok if ok else {k: v for k, v in forms.items() if "gpairs_per_s" in v and k in HEADLINE_FORMS}
     head = max(pool, key=lambda k: pool[k]["gpairs_per_s"])
     hr = forms[head]
 

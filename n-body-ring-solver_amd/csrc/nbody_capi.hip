@@ -99,7 +99,7 @@ int choose_ipt(const nb_ctx* c, long ni) { return ni >= long(c->num_cus) * 512 ?
 // Number of j-chunks so that the grid holds about 8 waves per CU.
 int choose_splits(const nb_ctx* c, long ni, long nj, int ipt) {
     const long waves = (ni + 64 * ipt - 1) / (64 * ipt);
-    const long target = long(c->num_cus) * 8;
+    const long target = long(c->num_cus) * 16;   // four waves per SIMD to hide LDS and FMA latency
     long S = 1;
     if (waves > 0 && waves < target) S = (target + waves - 1) / waves;
     S = std::min(S, std::max(1L, nj / 64));

@@ -25,7 +25,10 @@
 // takes a chunk spanning columns (x0, ys..ye) and z cells zmin..zmax: wave w
 // sums over the columns x0 - 2 + w, ys - 2..ye + 2, each restricted by binary
 // search to the z cells zmin - 2..zmax + 2, streaming those runs through LDS;
-// the five partial sums are added in wave order (bitwise reproducible).
+// the five partial sums are added in wave order (bitwise reproducible).  Of
+// each 64-particle batch of a run only the particles within rc of the chunk's
+// bounding box are kept (ballot + prefix count, order preserved), so far
+// corners of the cell box cost one distance test instead of a pair each.
 // Pairs beyond rc get S = 0, so the pair code is branch-free.
 //
 // FP64 throughout, or (fp32) FP64 differences with FP32 pair arithmetic and
@@ -150,6 +153,16 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+__device__ __forceinline__ double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // first position in [lo, hi) with keys[pos] >= k (hi if none)
 __device__ __forceinline__ int lower_bound(const unsigned long long* __restrict__ keys, int lo, int hi,
                                            unsigned long long k) {
@@ -235,18 +248,35 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
         }
     }
     double ax = 0.0, ay = 0.0, az = 0.0;
+    // bounding box of the chunk's particles (lanes past the chunk repeat its first)
+    const double3 lo = make_double3(wave_min(me.x), wave_min(me.y), wave_min(me.z));
+    const double3 hi = make_double3(wave_max(me.x), wave_max(me.y), wave_max(me.z));
+    const double rc2 = 1.0 / inv_rc2;
     nbk::P4* L = stage[w];
     for (int k = 0; k < ny; k++) {
         const int ak = __shfl(a, k, 64), bk = __shfl(b, k, 64);
         for (int j0 = ak; j0 < bk; j0 += 64) {
+            // keep only the particles within rc of the chunk's bounding box,
+            // packed in order to the front of the stage
             const int jj = j0 + lane;
+            nbk::P4 J = nbk::P4{0.0, 0.0, 0.0, 0.0};
+            bool use = false;
+            if (jj < bk) {
+                J = sorted[jj];
+                const double ex = fmax(fmax(lo.x - J.x, J.x - hi.x), 0.0);
+                const double ey = fmax(fmax(lo.y - J.y, J.y - hi.y), 0.0);
+                const double ez = fmax(fmax(lo.z - J.z, J.z - hi.z), 0.0);
+                use = ex * ex + ey * ey + ez * ez < rc2;
+            }
+            const unsigned long long keep = __ballot(use);
+            const int nkeep = __popcll(keep);
+            if (nkeep == 0) continue;   // wave-uniform
+            const int m4 = (nkeep + 3) & ~3;   // fillers pad to a multiple of 4
+            if (use) L[__popcll(keep & ((1ull << lane) - 1ull))] = J;
             // filler: this lane's position with zero mass, so it adds nothing
             // to any lane (r^2 = 0 for this one, m = 0 for the others)
-            nbk::P4 J = nbk::P4{me.x, me.y, me.z, 0.0};
-            if (jj < bk) J = sorted[jj];
-            L[lane] = J;
+            if (lane >= nkeep && lane < m4) L[lane] = nbk::P4{me.x, me.y, me.z, 0.0};
             wave_sync();
-            const int m4 = (min(64, bk - j0) + 3) & ~3;   // fillers pad to a multiple of 4
             if constexpr (F32) {
                 const float ir = (float)inv_rc2, e2 = (float)eps2;
                 float fx = 0.f, fy = 0.f, fz = 0.f;
