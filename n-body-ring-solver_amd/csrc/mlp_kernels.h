@@ -20,14 +20,22 @@ constexpr long NPARAM = OFF_B4 + NOUT;
 
 // Device copies the kernels read: bf16 weights (padded, plus transposes for
 // the backward sweep) and fp32 biases.
+// MFMA operand tiles of the forward weights: tile t holds, for lane
+// l = r + 16 g, the 4 bf16 A-operand values W[16 m + r][16 kt + 4 g + j] of
+// one 16 x 16 block (m, kt); tiles 0..7 are W1 (m), 8..71 W2 (8 + 8 m + kt),
+// 72..135 W3, 136..143 W4 (kt).  Read linearly, 8 bytes per lane.
+constexpr int TILES = 144, T_W1 = 0, T_W2 = 8, T_W3 = 72, T_W4 = 136;
+
 struct Net {
     const unsigned short *W1, *W2, *W3, *W4;    // [128][16], [128][128] x2, [16][128]
     const unsigned short *W2T, *W3T, *W4T;      // [128][128] x2, [128][16]
     const float *b1, *b2, *b3, *b4;             // 128, 128, 128, 16
+    const unsigned short* Wtile;                // TILES x 64 lanes x 4 bf16
 };
 
 // bytes of the bf16 + bias copies addressed by Net
-constexpr long NET_BYTES = (128 * 16 + 2 * 128 * 128 + 16 * 128 + 2 * 128 * 128 + 128 * 16) * 2 + (3 * 128 + 16) * 4;
+constexpr long NET_BYTES = (128 * 16 + 2 * 128 * 128 + 16 * 128 + 2 * 128 * 128 + 128 * 16) * 2 + (3 * 128 + 16) * 4 +
+                           TILES * 64 * 4 * 2;
 
 // Forward on X [rows][16] (feature 15 zero) -> Y [rows][4].  With save, Z
 // ([3][ld][128] fp32) and H ([16 + 3*128][ld] bf16, feature-major) are kept;

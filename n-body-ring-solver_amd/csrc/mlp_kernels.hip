@@ -119,6 +119,71 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(Net net, const float* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Inference forward (no saved activations): the same arithmetic as k_mlp_fwd,
+// with every weight tile and bias staged once per workgroup in LDS (72 KiB +
+// 1.6 KiB, two workgroups per CU) and read back linearly, 8 bytes per lane,
+// at LDS latency, instead of from memory for every MFMA (144 dependent-latency
+// loads per 16-particle tile at one wave per SIMD).  An empty asm with a
+// memory clobber at the top of the tile loop keeps the compiler from hoisting
+// all 144 tiles into registers (which would cost the occupancy).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_infer(Net net, const float* __restrict__ X, int rows,
+                                                   float* __restrict__ Y) {
+    __shared__ uint2 wt[TILES * 64];
+    __shared__ float bias[3 * 128 + 16];
+    {
+        const uint4* src = (const uint4*)net.Wtile;
+        uint4* dst = (uint4*)wt;
+        for (int k = threadIdx.x; k < TILES * 32; k += 256) dst[k] = src[k];
+        for (int k = threadIdx.x; k < 3 * 128 + 16; k += 256)
+            bias[k] = k < 128 ? net.b1[k] : k < 256 ? net.b2[k - 128] : k < 384 ? net.b3[k - 256] : net.b4[k - 384];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int tiles = (rows + 15) / 16;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    for (int tile = wave; tile < tiles; tile += nwaves) {
+        asm volatile("" ::: "memory");
+        auto A = [&](int t) { return __builtin_bit_cast(bf16x4, wt[t * 64 + lane]); };
+        const long p = (long)tile * 16 + r;
+        const bool valid = p < rows;
+        const float4 xin = valid ? *(const float4*)(X + p * 16 + 4 * g) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bf16x4 h0 = pack4(xin.x, xin.y, xin.z, xin.w);
+        bf16x4 h[8];
+        f32x4 z[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) z[m] = mfma(A(T_W1 + m), h0, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+        for (int l = 0; l < 3; l++) {   // bias + SiLU of layer l+1, then layer l+2's product
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                f32x4 zz = z[m];
+#pragma unroll
+                for (int j = 0; j < 4; j++) zz[j] += bias[128 * l + 16 * m + 4 * g + j];
+                h[m] = pack4(silu(zz[0]), silu(zz[1]), silu(zz[2]), silu(zz[3]));
+            }
+            if (l < 2) {
+                const int t0 = l == 0 ? T_W2 : T_W3;
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kt = 0; kt < 8; kt++) acc = mfma(A(t0 + 8 * m + kt), h[kt], acc);
+                    z[m] = acc;
+                }
+            }
+        }
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 8; kt++) acc = mfma(A(T_W4 + kt), h[kt], acc);
+        if (g == 0 && valid)
+            *(float4*)(Y + p * 4) = make_float4(acc[0] + bias[384], acc[1] + bias[385], acc[2] + bias[386], 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Backward of L = sum (y - t)^2 / (2 rows): deltas of every layer, stored
 // feature-major bf16 (D4 [16][ld], D3..D1 [128][ld]); per-tile column sums of
 // the deltas (bias gradients, fp32) and of the squared error.
@@ -293,20 +358,44 @@ __global__ __launch_bounds__(256) void k_publish(const float* __restrict__ p, Ne
         ((float*)net.b3)[i] = p[OFF_B3 + i];
     }
     if (i < 16) ((float*)net.b4)[i] = (i < 3) ? p[OFF_B4 + i] : 0.f;
+    if (i < TILES * 256) {   // MFMA operand tiles (see Net)
+        const int tile = i / 256, w = i % 256, ln = w / 4, j = w % 4, r = ln % 16, g = ln / 16;
+        float v;
+        if (tile < T_W2) {
+            const int col = 4 * g + j;
+            v = col < NIN ? p[OFF_W1 + (16 * tile + r) * NIN + col] : 0.f;
+        } else if (tile < T_W4) {
+            const int t = tile < T_W3 ? tile - T_W2 : tile - T_W3, m = t / 8, kt = t % 8;
+            v = p[(tile < T_W3 ? OFF_W2 : OFF_W3) + (16 * m + r) * NH + 16 * kt + 4 * g + j];
+        } else {
+            const int kt = tile - T_W4;
+            v = r < NOUT ? p[OFF_W4 + r * NH + 16 * kt + 4 * g + j] : 0.f;
+        }
+        ((unsigned short*)net.Wtile)[i] = to_bf16(v);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+// Each wave keeps every weight tile in registers (the compiler hoists the
+// loop-invariant loads: 512 registers, one wave per SIMD), so waves should
+// stream many 16-particle tiles each: one workgroup per CU at most.
 static int grid_for(int rows) {
     const int tiles = (rows + 15) / 16;
-    return std::max(1, std::min(4096, (tiles + 3) / 4));
+    return std::max(1, std::min(256, (tiles + 3) / 4));
 }
 
 hipError_t forward(const Net& net, const float* X, int rows, float* Y, bool save, float* Z, unsigned short* H, long ld,
                    hipStream_t st) {
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_fwd, dim3(grid_for(rows)), dim3(256), 0, st, net, X, rows, Y, save ? 1 : 0, Z, H, ld);
+    if (!save) {
+        const int tiles = (rows + 15) / 16;
+        hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(512, (tiles + 3) / 4))), dim3(256), 0, st, net, X,
+                           rows, Y);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(grid_for(rows)), dim3(256), 0, st, net, X, rows, Y, 1, Z, H, ld);
     return hipGetLastError();
 }
 
@@ -332,7 +421,7 @@ hipError_t adam(float* p, const float* g, float* m, float* v, int n, float lr, i
 }
 
 hipError_t publish(const float* p, const Net& net, hipStream_t st) {
-    hipLaunchKernelGGL(k_publish, dim3(64), dim3(256), 0, st, p, net);
+    hipLaunchKernelGGL(k_publish, dim3(TILES), dim3(256), 0, st, p, net);
     return hipGetLastError();
 }
 

@@ -428,7 +428,7 @@ Learned* get(nb_ctx* c) {
 // Device copies of the network live in one allocation.
 nb_status net_alloc(Learned& L) {
     if (L.netmem.p) return NB_OK;
-    LH(L.netmem.ensure(mlp::NET_BYTES + 256), "allocate network");
+    LH(L.netmem.ensure(mlp::NET_BYTES + 1024), "allocate network");
     LH(L.params.ensure(mlp::NPARAM * sizeof(float)), "allocate parameters");
     LH(L.am.ensure(mlp::NPARAM * sizeof(float)), "allocate Adam moments");
     LH(L.av.ensure(mlp::NPARAM * sizeof(float)), "allocate Adam moments");
@@ -451,6 +451,7 @@ nb_status net_alloc(Learned& L) {
     L.net.b2 = (const float*)take(128 * 4);
     L.net.b3 = (const float*)take(128 * 4);
     L.net.b4 = (const float*)take(16 * 4);
+    L.net.Wtile = (const unsigned short*)take(mlp::TILES * 64 * 4 * 2);
     return NB_OK;
 }
 
