@@ -204,8 +204,9 @@ NB_API nb_status nb_learned_configure(nb_ctx* ctx, int period, double threshold,
 NB_API nb_status nb_learned_set_flag_ratio(nb_ctx* ctx, double rho);
 
 /* kappa > 0: a particle whose extrapolation missed the exact acceleration at
- * a check by more than kappa times the RMS acceleration is summed exactly for
- * the next 4 substeps (default 1e-3; INFINITY never holds on this ground). */
+ * a check by more than kappa times the RMS acceleration is summed exactly on
+ * the extrapolated substeps until the next check (default 1e-3; INFINITY
+ * never holds on this ground). */
 NB_API nb_status nb_learned_set_hold_ratio(nb_ctx* ctx, double kappa);
 
 /* rc >= 0: near-field radius.  With rc > 0 pairs closer than rc (tapered
@@ -225,8 +226,12 @@ NB_API nb_status nb_near_accel(nb_ctx* ctx, int form, double rc, double eps, dou
  * 1: cubic extrapolation alone (the method without the network). */
 NB_API nb_status nb_learned_set_mode(nb_ctx* ctx, int mode);
 
-/* Run `time_steps` exact steps from the resident state (advancing it) and
- * record training pairs (features, target correction) in device memory. */
+/* Run `time_steps` exact steps (form) from the resident state, advancing it,
+ * and record training pairs in device memory: for every substep from the
+ * 4*period-th on, the features of the exact samples one to four periods back
+ * and, as target, the residual of their cubic at the substep (remainder after
+ * the near field when a radius is set) in units of |D3|, as components in the
+ * particle's frame.  The pairs match the configured period and radius. */
 NB_API nb_status nb_learned_collect(nb_ctx* ctx, size_t time_steps, double dt, const double* c,
                              const double* d, size_t lf_steps, int form, double eps);
 

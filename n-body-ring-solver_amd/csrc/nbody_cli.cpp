@@ -10,7 +10,7 @@
 //   -ic ring|sphere|plummer, -seed <k>, -a <scale>, -rmax <r>
 //   -form fp64|mixed|mixed-raw|learned
 //   -device <d>
-//   -period <k> -threshold <x> -train-steps <n> -train-iters <n>  (learned form)
+//   -period <k> -threshold <x> -rc <r> -train-steps <n> -train-iters <n>  (learned form)
 // Output: the reference's stdout lines (coefficients, run time, MPairs/s,
 // deviations from the initial state) plus energies; with -o, one CSV row per
 // run appended under a header written when the file is new (the FPGA
@@ -49,9 +49,10 @@ struct Args {
     std::string form;
     int device = 0;
     int period = 4;
-    double threshold = 1e-3;
+    double threshold = 3e-3;
+    double rc = 0.25;   // near-field radius of the learned form (0: no split)
     size_t train_steps = 200;
-    int train_iters = 2000;
+    int train_iters = 4000;
 };
 
 bool file_exists(const std::string& p) {
@@ -108,6 +109,7 @@ void parse(int argc, char** argv, Args& a) {
         else if (f == "-device" && has) a.device = atoi(argv[++i]);
         else if (f == "-period" && has) a.period = atoi(argv[++i]);
         else if (f == "-threshold" && has) a.threshold = atof(argv[++i]);
+        else if (f == "-rc" && has) a.rc = atof(argv[++i]);
         else if (f == "-train-steps" && has) a.train_steps = strtoull(argv[++i], nullptr, 10);
         else if (f == "-train-iters" && has) a.train_iters = atoi(argv[++i]);
     }
@@ -227,6 +229,9 @@ int main(int argc, char** argv) {
 
     if (form == NB_FORM_LEARNED) {
         CHECK(nb_learned_configure(ctx, a.period, a.threshold, NB_FORM_MIXED));
+        CHECK(nb_learned_set_near_radius(ctx, a.rc));
+        CHECK(nb_learned_set_flag_ratio(ctx, 0.1));
+        CHECK(nb_learned_set_hold_ratio(ctx, 3e-3));
         printf("Training the extrapolator on %zu exact steps ...\n", a.train_steps);
         double loss = 0.0;
         CHECK(nb_learned_collect(ctx, a.train_steps, dt, c, d, lf, NB_FORM_MIXED, a.eps));
