@@ -67,12 +67,12 @@ def rel_err(a, b):
     return float(np.max(np.linalg.norm(a - b, axis=0)[mask] / nb[mask]))
 
 
-def profile_traffic(name: str):
+def profile_traffic(name: str, key: str = "traffic_bytes_per_launch"):
     """HBM bytes per launch recorded in a committed rocprof summary (or None)."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", f.read())
+            m = re.search(re.escape(key) + r":\s*([0-9.eE+-]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
         return None
@@ -263,7 +263,9 @@ def main():
     if forms.get("mixed", {}).get("exact_force_evals"):
         rooflines["mixed"] = roofline_entry("k_accel_sym<MIXED_NR, K=8> (symmetric, FP32 pair math on FP64 "
                                             "differences; against the FP32 peak)",
-                                            forms["mixed"]["force_kernel_ms_avg"], n, FP32_PEAK_TFLOPS)
+                                            forms["mixed"]["force_kernel_ms_avg"], n, FP32_PEAK_TFLOPS,
+                                            profile_traffic("k_accel_sym_plummer_n262144_pmc.md",
+                                                            "traffic_bytes_per_launch_mixed"))
     dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
     roofline = rooflines.get(dominant) or rooflines.get("fp64")
 
