@@ -394,8 +394,11 @@ struct Learned {
     size_t n = 0;
     DevBuf hist, X, Y, cub, frame, pred, chk, mask, held, flags, nflags, sub_p, sub_part, sub_acc, nearacc;
     int head = 3, count = 0;   // ring of the last four exact samples (head newest)
-    long htime[4] = {0, 0, 0, 0};   // their substep indices
+    double htime[4] = {0, 0, 0, 0};   // their times, in steps
     long sub = 0;              // substeps since the state was set
+    double clock = 0.0;        // time of the resident positions, in steps, since the state was set
+    int substeps = 1;          // force evaluations per step of the running integrator (lf - 1)
+    double dt = 0.0;           // the step the history's times are counted in
     long since_exact = 0;
     bool fallback = true;
     bool pred_net = false;     // the last extrapolation applied the network
@@ -505,20 +508,23 @@ nb_status ensure_particles(Learned& L, size_t n) {
     return NB_OK;
 }
 
-// Lagrange weights at substep t of the cubic through the history samples
-// (newest first), and the network weight: the nodal polynomial at t over its
-// value one period after the newest sample, when the samples are spaced
-// exactly one period apart (else 0: the network is not applied).
+// Lagrange weights at time t of the cubic through the history samples
+// (newest first, at their times), and the network weight: the nodal
+// polynomial at t over its value one period after the newest sample, when the
+// samples are spaced exactly one period apart (period substeps of the mean
+// substep interval, 1/substeps steps; else 0: the network is not applied).
+// Times are the positions' times, in steps: substeps of a higher-order
+// Yoshida step are not evenly spaced.
 struct Weights {
     double l[4];
     double wnet;
 };
 
-Weights weights(const Learned& L, long t) {
+Weights weights(const Learned& L, double t) {
     Weights W{};
     double tk[4];
-    for (int k = 0; k < 4; k++) tk[k] = (double)L.htime[(L.head - k + 4) & 3];
-    const double x = (double)t;
+    for (int k = 0; k < 4; k++) tk[k] = L.htime[(L.head - k + 4) & 3];
+    const double x = t;
     for (int k = 0; k < 4; k++) {
         double num = 1.0, den = 1.0;
         for (int j = 0; j < 4; j++)
@@ -528,9 +534,9 @@ Weights weights(const Learned& L, long t) {
             }
         W.l[k] = num / den;
     }
-    const double P = (double)L.period;
+    const double P = (double)L.period / (double)std::max(1, L.substeps);
     bool even = true;
-    for (int k = 0; k < 3; k++) even = even && (tk[k] - tk[k + 1] == P);
+    for (int k = 0; k < 3; k++) even = even && std::fabs(tk[k] - tk[k + 1] - P) <= 1e-9 * P;
     if (even) {
         double num = 1.0, den = 1.0;
         for (int k = 0; k < 4; k++) {
@@ -554,7 +560,7 @@ inline int4 ring_slots(const Learned& L) {
 nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
-    const Weights W = weights(L, L.sub);
+    const Weights W = weights(L, L.clock);
     LH(hipMemsetAsync(L.nflags.p, 0, sizeof(int), st), "clear flag counter");
     hipLaunchKernelGGL(k_features, blocks(n), dim3(BLK), 0, st, (const P4*)L.hist.p, ring_slots(L),
                        make_double4(W.l[0], W.l[1], W.l[2], W.l[3]), n, (const P4*)nbh::ctx_vel(c), (float*)L.X.p,
@@ -678,7 +684,7 @@ nb_status check(nb_ctx* c, Learned& L) {
 // Push the exact acc (minus the near field) into the history ring.
 nb_status push_history(nb_ctx* c, Learned& L) {
     L.head = (L.head + 1) & 3;
-    L.htime[L.head] = L.sub;
+    L.htime[L.head] = L.clock;
     const long n = (long)L.n;
     hipLaunchKernelGGL(k_sub, blocks(n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)nbh::ctx_acc(c),
                        L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, n, (P4*)L.hist.p + (long)L.head * n);
@@ -767,6 +773,7 @@ void on_new_state(Learned* l) {
     l->head = 3;
     l->since_exact = 0;
     l->sub = 0;
+    l->clock = 0.0;
     l->fallback = true;
     l->pred_net = l->net_ok = false;
     l->clear_held = true;
@@ -781,12 +788,18 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
     const size_t n = nbh::ctx_n(c);
     if (n == 0 || time_steps == 0) return NB_OK;
     if (nb_status s = ensure_particles(L, n)) return s;
+    if (dt != L.dt) {   // history times are in steps of dt: a new dt starts a new history
+        on_new_state(&L);
+        L.dt = dt;
+    }
     hipStream_t st = nbh::ctx_stream(c);
     if (L.clear_held) {
         LH(hipMemsetAsync(L.held.p, 0, n, st), "clear holds");
         L.clear_held = false;
     }
     LH(nbk::launch_drift((int)n, nbh::ctx_pm(c), nbh::ctx_vel(c), cc[0] * dt, st), "drift kernel");
+    L.clock += cc[0];
+    L.substeps = (int)(lf - 1);
     for (size_t t = 0; t < time_steps; t++) {
         for (size_t s = 0; s + 1 < lf; s++) {
             const bool full = (L.count == 4);
@@ -822,7 +835,9 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 L.since_exact++;
             }
             L.sub++;
-            if (nb_status e = kick_drift(c, schedule(t, time_steps, s, dt, cc, dd, lf))) return e;
+            const Schedule q = schedule(t, time_steps, s, dt, cc, dd, lf);
+            if (nb_status e = kick_drift(c, q)) return e;
+            L.clock += (q.c1 + q.c2) / dt;
         }
     }
     return NB_OK;

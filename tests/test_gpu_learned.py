@@ -262,3 +262,38 @@ def test_network_mode_matches_restatement(solver, orc):
     # and the network changed the trajectory (against the cubic alone)
     pc, _, _ = restated_learned(orc, m, pos, vel, steps, dt, eps, 4, 1e-2, 0.5)
     assert np.max(np.abs(pc - pr)) > 1e-12
+
+
+def test_learned_fourth_order_uses_real_substep_times(solver):
+    """Order-4 Yoshida substeps are unevenly spaced in time; the learned form
+    extrapolates in the positions' time and stays close to the exact run."""
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 4096, 13)
+    eps, dt, steps = 0.01, 1.0 / 128, 40
+    solver.set_state(m, pos, vel)
+    solver.step(steps, dt, 4, "fp64", eps)
+    p_exact, _ = solver.get_state()
+    solver.learned_configure(period=4, threshold=1e-2, exact_form="fp64")
+    solver.learned_set_mode(1)
+    solver.learned_set_near_radius(0.3)
+    try:
+        solver.set_state(m, pos, vel)
+        solver.step(steps, dt, 4, "learned", eps)
+        p, _ = solver.get_state()
+        st = solver.learned_stats()
+    finally:
+        solver.learned_set_near_radius(0.0)
+    assert st["predicted_evals"] > 40 and st["fallbacks"] <= 2, st
+    assert st["max_check_err"] < 1e-2
+    assert np.max(np.abs(p - p_exact)) < 1e-6 * np.max(np.abs(p_exact))
+
+
+def test_learned_new_dt_restarts_history(solver):
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 2048, 14)
+    solver.learned_configure(period=4, threshold=1e-2, exact_form="fp64")
+    solver.learned_set_mode(1)
+    solver.set_state(m, pos, vel)
+    solver.step(12, 1.0 / 256, 2, "learned", 0.01)
+    st1 = solver.learned_stats()
+    solver.step(6, 1.0 / 512, 2, "learned", 0.01)   # the history restarts: 4 exact samples first
+    st2 = solver.learned_stats()
+    assert st2["exact_evals"] - st1["exact_evals"] >= 5
