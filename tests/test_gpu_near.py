@@ -102,3 +102,40 @@ def test_learned_split_matches_restatement(solver, orc):
     assert sr["predicted"] > 0
     assert np.max(np.abs(p - pr)) < 1e-11
     assert np.max(np.abs(v - vr)) < 1e-9 * np.max(np.abs(vr))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 65, 130])
+def test_near_accel_tiny_systems(solver, orc, n):
+    """Systems smaller than a chunk, one particle, two coincident ones."""
+    rng = np.random.default_rng(n)
+    pos = np.ascontiguousarray(rng.uniform(-0.2, 0.2, (3, n)))
+    if n >= 2:
+        pos[:, 1] = pos[:, 0]   # a coincident pair contributes nothing
+    m = rng.uniform(0.5, 1.5, n) / n
+    solver.set_state(m, pos, np.zeros_like(pos))
+    a = solver.near_accel(0.3, 0.01)
+    ao = orc.near_accel(m, pos, 0.3, 0.01)
+    scale = max(np.max(np.linalg.norm(ao, axis=0)), 1e-300)
+    assert np.max(np.linalg.norm(a - ao, axis=0)) <= 1e-12 * scale
+    if n == 1:
+        assert np.all(a == 0.0)
+
+
+@pytest.mark.parametrize("n", [3, 17, 100])
+def test_learned_tiny_systems_match_restatement(solver, orc, n):
+    from test_gpu_learned import restated_learned
+    m, pos, vel, _ = nbody_amd.init_system("plummer", n, 21)
+    solver.learned_configure(period=4, threshold=1e-1, exact_form="fp64")
+    solver.learned_set_mode(1)
+    solver.learned_set_flag_ratio(0.5)
+    solver.learned_set_near_radius(0.3)
+    try:
+        solver.set_state(m, pos, vel)
+        solver.step(20, 1.0 / 256, 2, "learned", 0.05)
+        p, _ = solver.get_state()
+        st = solver.learned_stats()
+    finally:
+        solver.learned_set_near_radius(0.0)
+    pr, _, sr = restated_learned(orc, m, pos, vel, 20, 1.0 / 256, 0.05, 4, 1e-1, 0.5, rc=0.3)
+    assert st["exact_evals"] == sr["exact"] and st["predicted_evals"] == sr["predicted"]
+    assert np.max(np.abs(p - pr)) < 1e-11
