@@ -164,6 +164,19 @@ typedef enum nb_kernel {
 
 NB_API nb_status nb_set_kernel(nb_ctx* ctx, int kernel);
 
+/* Snapshots in the reference's binary layout (NBody_system::save_particles,
+ * CPU/nbody_system.cpp:133-152): frames of doubles [t, m[N], x[N], y[N],
+ * z[N], vx[N], vy[N], vz[N], fx[N], fy[N], fz[N]] one after another, host
+ * byte order.  Host-only (no context): write the arrays of nb_get_state /
+ * nb_compute_forces (force may be NULL: zeros), count the frames of a file
+ * for a given N, read frame `frame` (any output may be NULL: skipped). */
+NB_API nb_status nb_snapshot_write(const char* path, int append, double t, size_t n, const double* mass,
+                                   const double* const pos[3], const double* const vel[3],
+                                   const double* const force[3]);
+NB_API nb_status nb_snapshot_frames(const char* path, size_t n, size_t* frames);
+NB_API nb_status nb_snapshot_read(const char* path, size_t n, size_t frame, double* t, double* mass,
+                                  double* const pos[3], double* const vel[3], double* const force[3]);
+
 /* Wait for all work queued on the context's stream. */
 NB_API nb_status nb_synchronize(nb_ctx* ctx);
 

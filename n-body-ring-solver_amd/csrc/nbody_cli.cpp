@@ -11,6 +11,8 @@
 //   -form fp64|mixed|mixed-raw|learned
 //   -device <d>
 //   -period <k> -threshold <x> -rc <r> -train-steps <n> -train-iters <n>  (learned form)
+//   -snapshot <file>   initial and final state (with FP64 forces) as frames in
+//                      the reference's NBody_<N>.dat layout (CPU/nbody_system.cpp:133)
 // Output: the reference's stdout lines (coefficients, run time, MPairs/s,
 // deviations from the initial state) plus energies; with -o, one CSV row per
 // run appended under a header written when the file is new (the FPGA
@@ -48,6 +50,7 @@ struct Args {
     double a = 1.0, rmax = 20.0;
     std::string form;
     int device = 0;
+    std::string snapshot;
     int period = 4;
     double threshold = 3e-3;
     double rc = 0.25;   // near-field radius of the learned form (0: no split)
@@ -75,6 +78,7 @@ void parse(int argc, char** argv, Args& a) {
         else if (f == "-r" && has) a.ratio = atof(argv[++i]);
         else if (f == "-v") a.verbose = true;
         else if (f == "-o" && has) a.out = argv[++i];
+        else if (f == "-snapshot" && has) a.snapshot = argv[++i];
         else if (f == "-order" && has) a.order = int(strtoull(argv[++i], nullptr, 10));
         else if (f == "-t" && has) {
             a.exec = argv[++i];
@@ -226,6 +230,14 @@ int main(int argc, char** argv) {
     CHECK(nb_set_state(ctx, N, mass.data(), pos.p, vel.p));
     double T0, U0;
     CHECK(nb_energy(ctx, a.eps, &T0, &U0));
+    auto snapshot = [&](double t, bool append) -> nb_status {
+        Planes p(N), v(N), f(N);
+        nb_status s = nb_get_state(ctx, p.p, v.p);
+        if (s == NB_OK) s = nb_compute_forces(ctx, NB_FORM_FP64, a.eps, f.p);
+        if (s == NB_OK) s = nb_snapshot_write(a.snapshot.c_str(), append ? 1 : 0, t, N, mass.data(), p.p, v.p, f.p);
+        return s;
+    };
+    if (!a.snapshot.empty()) CHECK(snapshot(0.0, false));
 
     if (form == NB_FORM_LEARNED) {
         CHECK(nb_learned_configure(ctx, a.period, a.threshold, NB_FORM_MIXED));
@@ -261,6 +273,10 @@ int main(int argc, char** argv) {
     const double E0 = T0 + U0, E1 = T1 + U1;
     const double edev = std::fabs((E0 - E1) / E0);   // CPU/omp_solver.cpp:1205
     printf("E0 = %.15e, E = %.15e, |dE/E| = %.6e\n", E0, E1, edev);
+    if (!a.snapshot.empty()) {
+        CHECK(snapshot(dt * double(a.steps), true));
+        printf("Snapshot frames (t = 0 and t = %g) written to %s\n", dt * double(a.steps), a.snapshot.c_str());
+    }
 
     double max_rel = 0.0;
     if (form != NB_FORM_FP64 && N > 1) {

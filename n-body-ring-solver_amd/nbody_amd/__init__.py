@@ -18,6 +18,7 @@ import numpy as np
 
 __all__ = [
     "LIB_PATH", "NBodyError", "Solver", "load_library", "init_system", "yoshida", "FORMS", "IC_KINDS",
+    "snapshot_write", "snapshot_frames", "snapshot_read",
     "mlp_param_count",
 ]
 
@@ -38,7 +39,7 @@ EXPORTS = (
     "nb_learned_set_mode", "nb_learned_collect", "nb_learned_train", "nb_learned_stats_read",
     "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
     "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
-    "nb_near_accel", "nb_learned_set_hold_ratio",
+    "nb_near_accel", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -99,6 +100,9 @@ def load_library(path: str = LIB_PATH):
     L.nb_learned_set_near_radius.argtypes = [_VP, C.c_double]
     L.nb_near_accel.argtypes = [_VP, C.c_int, C.c_double, C.c_double, _P3]
     L.nb_learned_set_hold_ratio.argtypes = [_VP, C.c_double]
+    L.nb_snapshot_write.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_size_t, _D, _P3, _P3, _P3]
+    L.nb_snapshot_frames.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.nb_snapshot_read.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, _D, _D, _P3, _P3, _P3]
     L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
     L.nb_learned_train.argtypes = [_VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]
     L.nb_learned_stats_read.argtypes = [_VP, C.POINTER(LearnedStats)]
@@ -134,6 +138,34 @@ def _form(form) -> int:
     if isinstance(form, str):
         return FORMS[form]
     return int(form)
+
+
+def snapshot_write(path: str, mass, pos, vel, force=None, t: float = 0.0, append: bool = False):
+    """Append (or start) a frame in the reference's NBody_<N>.dat layout."""
+    L = load_library()
+    pos, vel = np.ascontiguousarray(pos, dtype=np.float64), np.ascontiguousarray(vel, dtype=np.float64)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    f = None if force is None else np.ascontiguousarray(force, dtype=np.float64)
+    _check(L.nb_snapshot_write(os.fsencode(path), int(append), t, mass.shape[0], _p(mass), _planes(pos), _planes(vel),
+                               None if f is None else _planes(f)), "nb_snapshot_write")
+
+
+def snapshot_frames(path: str, n: int) -> int:
+    L = load_library()
+    k = C.c_size_t()
+    _check(L.nb_snapshot_frames(os.fsencode(path), n, C.byref(k)), "nb_snapshot_frames")
+    return k.value
+
+
+def snapshot_read(path: str, n: int, frame: int = 0):
+    """Frame `frame` of a snapshot file: (t, mass (N,), pos, vel, force (3, N))."""
+    L = load_library()
+    t = C.c_double()
+    m = np.empty(n)
+    pos, vel, force = np.empty((3, n)), np.empty((3, n)), np.empty((3, n))
+    _check(L.nb_snapshot_read(os.fsencode(path), n, frame, C.byref(t), _p(m), _planes(pos), _planes(vel),
+                              _planes(force)), "nb_snapshot_read")
+    return t.value, m, pos, vel, force
 
 
 def init_system(kind: str, n: int, seed: int = 42, params=None):

@@ -243,6 +243,7 @@ class Reference:
         L.ref_leapfrog.argtypes = [_SZ, _D, P3, P3, _SZ, C.c_double, _D, _D, _SZ]
         L.ref_init_ring.argtypes = [_SZ, C.c_double, C.c_double, C.c_double, _D, P3, P3, P3]
         L.ref_energy.argtypes = [_SZ, _D, P3, P3, _D, _D, _D]
+        L.ref_save_particles.argtypes = [_SZ, _D, P3, P3, P3, C.c_double, _SZ]
 
     def forces(self, variant: int, m, pos) -> np.ndarray:
         n = m.shape[0]
@@ -265,6 +266,13 @@ class Reference:
                                    c.shape[0])
         assert rc == 0
         return pos, vel
+
+    def save_particles(self, m, pos, vel, force, dt: float, frames: int):
+        """NBody_system::save_particles for frames 0..frames-1 (velocities scaled by 1 + k) into
+        NBody_<N>.dat in the current directory."""
+        pos, vel, force = (np.ascontiguousarray(a) for a in (pos, vel, force))
+        rc = self.lib.ref_save_particles(m.shape[0], _p(m), _ptr3(pos), _ptr3(vel), _ptr3(force), dt, frames)
+        assert rc == 0
 
     def init_ring(self, n: int, R: float = 1.0, omega: float = 2 * np.pi, ratio: float = 1e5):
         m = np.empty(n)

@@ -214,4 +214,25 @@ __attribute__((visibility("default"))) int ref_energy(size_t n, const double* m,
     return 0;
 }
 
+// Snapshot frames written by NBody_system::save_particles
+// (CPU/nbody_system.cpp:133-152) into NBody_<N>.dat in the working directory:
+// frame k is written at time step k with the given dt, after scaling the
+// velocities by (1 + k) so the frames differ.
+__attribute__((visibility("default"))) int ref_save_particles(size_t n, const double* m, const double* const* pos,
+                                                              const double* const* vel, const double* const* force,
+                                                              double dt, size_t frames) {
+    NBody_system s;
+    load_system(s, n, m, pos, vel);
+    for (size_t k = 0; k < frames; k++) {
+        for (int d = 0; d < DIM; d++)
+            for (size_t i = 0; i < n; i++) {
+                s.vel[d][i] = vel[d][i] * double(1 + k);
+                s.force[d][i] = force[d][i];
+            }
+        s.save_particles(k, dt);
+    }
+    s.output_file.close();
+    return 0;
+}
+
 }  // extern "C"
