@@ -222,6 +222,57 @@ __global__ __launch_bounds__(BLOCK) void k_kick_drift(const P4* __restrict__ par
     pm[i] = p;
 }
 
+// The same two operations for many splits and few particles (the sequential
+// sum of S partials per thread is latency-bound there): 16 waves per 64
+// particles, wave w adds splits w, w + 16, ... in order, and the 16 sums are
+// added in wave order -- a fixed order, so results stay bitwise reproducible.
+constexpr int kWideWaves = 16;
+
+template <bool KICK>
+__global__ __launch_bounds__(64 * kWideWaves) void k_parts_wide(const P4* __restrict__ part, int S, long stride,
+                                                                int n, P4* __restrict__ pm, P4* __restrict__ vel,
+                                                                P4* __restrict__ acc, double kdt, double c1dt,
+                                                                double c2dt) {
+    __shared__ double sh[kWideWaves][3][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (i < n)
+        for (int q = w; q < S; q += kWideWaves) {
+            const P4 p = part[(long)q * stride + i];
+            ax += p.x; ay += p.y; az += p.z;
+        }
+    sh[w][0][lane] = ax;
+    sh[w][1][lane] = ay;
+    sh[w][2][lane] = az;
+    __syncthreads();
+    if (w != 0 || i >= n) return;
+    ax = ay = az = 0.0;
+    for (int q = 0; q < kWideWaves; q++) {
+        ax += sh[q][0][lane];
+        ay += sh[q][1][lane];
+        az += sh[q][2][lane];
+    }
+    if (!KICK) {
+        acc[i] = P4{ax, ay, az, 0.0};
+        return;
+    }
+    if (acc != part) acc[i] = P4{ax, ay, az, 0.0};
+    P4 v = vel[i];
+    v.x = __builtin_fma(kdt, ax, v.x);
+    v.y = __builtin_fma(kdt, ay, v.y);
+    v.z = __builtin_fma(kdt, az, v.z);
+    vel[i] = v;
+    P4 p = pm[i];
+    p.x = __builtin_fma(c1dt, v.x, p.x);
+    p.y = __builtin_fma(c1dt, v.y, p.y);
+    p.z = __builtin_fma(c1dt, v.z, p.z);
+    p.x = __builtin_fma(c2dt, v.x, p.x);
+    p.y = __builtin_fma(c2dt, v.y, p.y);
+    p.z = __builtin_fma(c2dt, v.z, p.z);
+    pm[i] = p;
+}
+
 // x += cdt v
 __global__ __launch_bounds__(BLOCK) void k_drift(int n, P4* __restrict__ pm, const P4* __restrict__ vel,
                                                  double cdt) {
@@ -373,15 +424,23 @@ hipError_t launch_accel(const AccelLaunch& L, hipStream_t st) {
 
 hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, S, stride, n, acc);
+    if (S >= kWideWaves)
+        hipLaunchKernelGGL(k_parts_wide<false>, dim3((n + 63) / 64), dim3(64 * kWideWaves), 0, st, part, S, stride, n,
+                           nullptr, nullptr, acc, 0.0, 0.0, 0.0);
+    else
+        hipLaunchKernelGGL(k_reduce, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, S, stride, n, acc);
     return hipGetLastError();
 }
 
 hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
                              double kdt, double c1dt, double c2dt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kick_drift, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, S, stride,
-                       n, pm, vel, acc, kdt, c1dt, c2dt);
+    if (S >= kWideWaves)
+        hipLaunchKernelGGL(k_parts_wide<true>, dim3((n + 63) / 64), dim3(64 * kWideWaves), 0, st, part, S, stride, n,
+                           pm, vel, acc, kdt, c1dt, c2dt);
+    else
+        hipLaunchKernelGGL(k_kick_drift, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, S, stride,
+                           n, pm, vel, acc, kdt, c1dt, c2dt);
     return hipGetLastError();
 }
 
