@@ -210,8 +210,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": src.numel() * 4}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the ring over GPUs is a "
-              f"design note, not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the ring's sharding over "
+              f"several GPUs is not implemented here", file=sys.stderr)
         sys.exit(2)
 
     import torch  # only for the device-wide synchronisation around the timed region

@@ -138,6 +138,30 @@ NB_API nb_status nb_accel_block_device(nb_ctx* ctx, int form, double eps, size_t
 NB_API nb_status nb_step(nb_ctx* ctx, size_t time_steps, double dt, const double* c, const double* d,
                   size_t lf_steps, int form, double eps);
 
+/* Advance the resident state by time_steps explicit Euler steps of size dt
+ * (reduced_solver_omp, CPU/omp_solver.cpp:16-137, the reference CLI's default
+ * integrator): per step, a = force/m of the current state, x += v dt with the
+ * velocity before the kick, then v += a dt.  form: NB_FORM_FP64 / NB_FORM_MIXED /
+ * NB_FORM_MIXED_RAW. */
+NB_API nb_status nb_step_euler(nb_ctx* ctx, size_t time_steps, double dt, int form, double eps);
+
+/* Explicit Runge-Kutta tableau of the reference's -i rk driver
+ * (CPU/omp_solver.cpp:772-797): order 4 (classic RK4) is the only one; any
+ * other order falls back to it, as the reference does.  a receives
+ * (rk_steps-1)^2 coefficients row-major (room for 49), b rk_steps (room for 8). */
+NB_API nb_status nb_runge_kutta(int order, double* a, double* b, size_t* rk_steps);
+
+/* Advance the resident state by time_steps explicit Runge-Kutta steps
+ * (reduced_solver_omp_rk, CPU/omp_solver.cpp:458-606) for the first-order
+ * system x' = v, v' = a(x): stage k has velocity kv_k and acceleration ka_k
+ * (kv_0 = v, ka_k from the stage positions); stage k+1 starts at
+ * x + dt sum_{j<=k} a[k*(rk_steps-1)+j] kv_j, v + dt sum_{j<=k} a[...] ka_j;
+ * the step ends at x + dt sum_j b_j kv_j, v + dt sum_j b_j ka_j.  Only the
+ * lower triangle of a is read.  1 <= rk_steps <= 8; the stage buffers take
+ * 2 * rk_steps + 1 records of 32 bytes per particle in HBM. */
+NB_API nb_status nb_step_rk(nb_ctx* ctx, size_t time_steps, double dt, const double* a, const double* b,
+                            size_t rk_steps, int form, double eps);
+
 /* Kinetic and potential energy of the resident state:
  * T = 1/2 sum m v^2 (CPU/force_calculation.cpp:1740), U = -G sum_{i<j} m_i m_j /
  * sqrt(r^2+eps^2) (:1880, softened as FPGA/host/accuracy_evaluation.cpp:35).
@@ -179,6 +203,40 @@ NB_API nb_status nb_snapshot_read(const char* path, size_t n, size_t frame, doub
 
 /* Wait for all work queued on the context's stream. */
 NB_API nb_status nb_synchronize(nb_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Accuracy diagnostics of the reference's omp-accuracy / omp-ratio
+ * experiments (CPU/omp_solver.cpp:899-1243).  Host-only (no context), on
+ * host planes such as those of nb_get_state / nb_compute_forces.
+ * ------------------------------------------------------------------------- */
+
+/* Force-error metrics of calculate_force_error (CPU/nbody_system.cpp:458-567,
+ * double-precision branch) of calc against ref:
+ *   out[0] mean |F - F_ref|          out[1] max |F - F_ref|
+ *   out[2] sum of xy-plane angles between F and F_ref over particles whose
+ *          reference x and y components are both non-zero, / (2 pi N)
+ *   out[3] mean ||F| - |F_ref|| / L0  out[4] max such angle / (2 pi)
+ *   out[5] max ||F| - |F_ref|| / L0
+ * L0 is the length of particle 0's reference force as the reference reads
+ * it, from its single-precision copy (components rounded to float, the
+ * squares summed and rooted in float); L0 = 0 (particle 0 is the ring's
+ * centre) gives infinities, as in the reference. */
+NB_API nb_status nb_force_error(size_t n, const double* const calc[3], const double* const ref[3], double out[6]);
+
+/* calculate_deviation (CPU/nbody_system.cpp:437-456): out[0] mean |calc - ref|,
+ * out[1] max ||calc| - |ref|| / |ref| over particles with |ref| > 0. */
+NB_API nb_status nb_deviation(size_t n, const double* const calc[3], const double* const ref[3], double out[2]);
+
+/* Moment of inertia sum m |x|^2 in index order (compute_inertia,
+ * CPU/force_calculation.cpp:1911-1919). */
+NB_API nb_status nb_inertia(size_t n, const double* mass, const double* const pos[3], double* I);
+
+/* Rigid-rotation state of the NB_IC_RING system after step/steps_per_rev of
+ * a revolution (predict_stable_orbiting_particles, CPU/nbody_system.cpp:302-324):
+ * ring particle i at angle (i/(N-1) + step/steps_per_rev) 2 pi, the centre
+ * at rest in the origin.  Writes pos and vel (either may be NULL). */
+NB_API nb_status nb_ring_predict(size_t n, double R, double omega, size_t step, size_t steps_per_rev,
+                                 double* const pos[3], double* const vel[3]);
 
 /* ---------------------------------------------------------------------------
  * Learned acceleration extrapolator (NB_FORM_LEARNED).  No counterpart in the

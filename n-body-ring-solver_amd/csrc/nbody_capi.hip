@@ -73,6 +73,7 @@ struct nb_ctx {
     std::vector<double> mass_h;
     DevBuf pm, vel, acc, part, rows, eblk;
     DevBuf bi, bj, bout;
+    DevBuf rk;   // Runge-Kutta stage records: x0, kv[s], ka[s]
     // timing
     bool timing = false;
     std::vector<hipEvent_t> events;   // pairs (start, stop) per force launch
@@ -337,7 +338,8 @@ nb_status nb_destroy(nb_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     nbl::destroy(c->learned);
     c->learned = nullptr;
-    for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->sym.items})
+    for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->rk,
+                      &c->sym.items})
         b->release();
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->step_ev)
@@ -599,6 +601,7 @@ nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, con
     if (kf < 0) return fail(NB_ERR_ARG, "nb_step: unknown form");
     if (c->n == 0 || time_steps == 0) return NB_OK;
     NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    nbl::on_new_state(c->learned);   // the learned form's history no longer describes the state
     const int n = int(c->n);
     P4* pm = (P4*)c->pm.p;
     P4* vel = (P4*)c->vel.p;
@@ -625,6 +628,92 @@ nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, con
         if (hipEventElapsedTime(&ms, c->step_ev[0], c->step_ev[1]) == hipSuccess) c->step_ms += ms;
     }
     return NB_OK;
+}
+
+// Shared argument checks of the Euler and Runge-Kutta drivers.
+static nb_status driver_args(nb_ctx* c, double dt, int form, double eps, const char* fn, int& kf) {
+    if (!c) return fail(NB_ERR_ARG, std::string(fn) + ": null context");
+    if (!std::isfinite(dt)) return fail(NB_ERR_ARG, std::string(fn) + ": dt must be finite");
+    if (nb_status s = check_eps(eps, fn)) return s;
+    kf = kernel_form(form);
+    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
+    return NB_OK;
+}
+
+static void step_timer_begin(nb_ctx* c) {
+    if (c->timing) (void)hipEventRecord(c->step_ev[0], c->stream);
+}
+
+static nb_status step_timer_end(nb_ctx* c) {
+    if (!c->timing) return NB_OK;
+    (void)hipEventRecord(c->step_ev[1], c->stream);
+    NB_HIP(hipEventSynchronize(c->step_ev[1]), "hipEventSynchronize");
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->step_ev[0], c->step_ev[1]) == hipSuccess) c->step_ms += ms;
+    return NB_OK;
+}
+
+nb_status nb_step_euler(nb_ctx* c, size_t time_steps, double dt, int form, double eps) {
+    int kf;
+    if (nb_status s = driver_args(c, dt, form, eps, "nb_step_euler", kf)) return s;
+    if (c->n == 0 || time_steps == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    nbl::on_new_state(c->learned);
+    const int n = int(c->n);
+    step_timer_begin(c);
+    for (size_t t = 0; t < time_steps; t++) {
+        NB_HIP(accel_to_acc(c, kf, eps), "force kernel");
+        NB_HIP(nbk::launch_euler(n, (P4*)c->pm.p, (P4*)c->vel.p, (const P4*)c->acc.p, dt, c->stream), "Euler kernel");
+    }
+    return step_timer_end(c);
+}
+
+nb_status nb_runge_kutta(int order, double* a, double* b, size_t* rk_steps) {
+    if (!a || !b || !rk_steps) return fail(NB_ERR_ARG, "nb_runge_kutta: null argument");
+    *rk_steps = runge_kutta(order, a, b);
+    return NB_OK;
+}
+
+nb_status nb_step_rk(nb_ctx* c, size_t time_steps, double dt, const double* a, const double* b, size_t rk_steps,
+                     int form, double eps) {
+    int kf;
+    if (nb_status s = driver_args(c, dt, form, eps, "nb_step_rk", kf)) return s;
+    if (rk_steps < 1 || rk_steps > size_t(nbk::kMaxRK)) return fail(NB_ERR_ARG, "nb_step_rk: rk_steps must be in [1, 8]");
+    if (!b || (rk_steps > 1 && !a)) return fail(NB_ERR_ARG, "nb_step_rk: null coefficient array");
+    if (c->n == 0 || time_steps == 0) return NB_OK;
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    nbl::on_new_state(c->learned);
+    const long n = long(c->n);
+    const size_t s = rk_steps;
+    NB_HIP(c->rk.ensure((2 * s + 1) * size_t(n) * sizeof(P4)), "allocate Runge-Kutta stages");
+    P4* x0 = (P4*)c->rk.p;
+    P4* kv = x0 + n;
+    P4* ka = kv + s * n;
+    P4* pm = (P4*)c->pm.p;
+    P4* vel = (P4*)c->vel.p;
+    const size_t bytes = size_t(n) * sizeof(P4);
+    const size_t ad = s - 1;   // row stride of a (CPU/omp_solver.cpp:514)
+    step_timer_begin(c);
+    for (size_t t = 0; t < time_steps; t++) {
+        NB_HIP(hipMemcpyAsync(x0, pm, bytes, hipMemcpyDeviceToDevice, c->stream), "save positions");
+        NB_HIP(hipMemcpyAsync(kv, vel, bytes, hipMemcpyDeviceToDevice, c->stream), "save velocities");
+        for (size_t k = 0; k < s; k++) {
+            // stage k: acceleration at the stage positions now in pm
+            NB_HIP(accel_to_acc(c, kf, eps), "force kernel");
+            NB_HIP(hipMemcpyAsync(ka + k * n, c->acc.p, bytes, hipMemcpyDeviceToDevice, c->stream), "keep stage");
+            if (k + 1 == s) break;
+            nbk::RKTerms T;
+            T.count = int(k + 1);
+            for (size_t j = 0; j <= k; j++) T.w[j] = a[k * ad + j];
+            NB_HIP(nbk::launch_rk_combine(int(n), x0, kv, ka, n, T, dt, pm, kv + (k + 1) * n, c->stream),
+                   "Runge-Kutta stage kernel");
+        }
+        nbk::RKTerms T;
+        T.count = int(s);
+        for (size_t j = 0; j < s; j++) T.w[j] = b[j];
+        NB_HIP(nbk::launch_rk_combine(int(n), x0, kv, ka, n, T, dt, pm, vel, c->stream), "Runge-Kutta step kernel");
+    }
+    return step_timer_end(c);
 }
 
 nb_status nb_energy(nb_ctx* c, double eps, double* T, double* U) {

@@ -23,6 +23,7 @@ void ic_plummer(size_t n, uint64_t seed, double a, double rmax, double* m, doubl
 void ic_ring(size_t n_total, double R, double omega, double ratio, double* m, double* const pos[3],
              double* const vel[3], double* const force[3]);
 size_t yoshida(int order, double* c, double* d);
+size_t runge_kutta(int order, double* a, double* b);
 
 // Grow-only device buffer.
 struct DevBuf {

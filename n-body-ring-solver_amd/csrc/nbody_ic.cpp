@@ -186,4 +186,16 @@ size_t yoshida(int order, double* c, double* d) {
     return 2 * s;
 }
 
+// Classic fourth-order Runge-Kutta (Kutta 1901) in the reference's layout
+// (CPU/omp_solver.cpp:772-797): a is the (s-1) x (s-1) row-major lower
+// triangle of the Butcher tableau without its zero first row, b the weights.
+// Order 4 is the only tableau; every order maps to it.
+size_t runge_kutta(int /*order*/, double* a, double* b) {
+    const double a4[9] = {0.5, 0.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 1.0};
+    const double b4[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    for (int i = 0; i < 9; i++) a[i] = a4[i];
+    for (int i = 0; i < 4; i++) b[i] = b4[i];
+    return 4;
+}
+
 }  // namespace nbh

@@ -56,6 +56,19 @@ hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hip
 hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
                              double kdt, double c1dt, double c2dt, hipStream_t st);
 hipError_t launch_drift(int n, P4* pm, const P4* vel, double cdt, hipStream_t st);
+// Euler update: x += v dt (old v), then v += a dt.
+hipError_t launch_euler(int n, P4* pm, P4* vel, const P4* acc, double dt, hipStream_t st);
+
+// Runge-Kutta stage combination: x_out = x0 + dt sum_j w_j kv_j (mass from
+// x0), v_out = kv_0 + dt sum_j w_j ka_j, j < count; stage j's records sit at
+// j * stride.
+constexpr int kMaxRK = 8;
+struct RKTerms {
+    int count = 0;
+    double w[kMaxRK] = {};
+};
+hipError_t launch_rk_combine(int n, const P4* x0, const P4* kv, const P4* ka, long stride, const RKTerms& T,
+                             double dt, P4* x_out, P4* v_out, hipStream_t st);
 // rows: splits * n * 16 bytes; blocks_out: 4 doubles per 256-particle block.
 hipError_t launch_energy(const P4* pm, const P4* vel, int n, int splits, int jchunk, double eps2,
                          void* rows, double* blocks_out, hipStream_t st);

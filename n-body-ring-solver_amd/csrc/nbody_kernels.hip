@@ -286,6 +286,55 @@ __global__ __launch_bounds__(BLOCK) void k_drift(int n, P4* __restrict__ pm, con
     pm[i] = p;
 }
 
+// Explicit Euler update of the reference's first-order driver
+// (CPU/omp_solver.cpp:124-137): the position moves with the velocity it had
+// before the kick, then v += a dt.
+__global__ __launch_bounds__(BLOCK) void k_euler(int n, P4* __restrict__ pm, P4* __restrict__ vel,
+                                                 const P4* __restrict__ acc, double dt) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    P4 p = pm[i];
+    P4 v = vel[i];
+    const P4 a = acc[i];
+    p.x = __builtin_fma(v.x, dt, p.x);
+    p.y = __builtin_fma(v.y, dt, p.y);
+    p.z = __builtin_fma(v.z, dt, p.z);
+    v.x = __builtin_fma(a.x, dt, v.x);
+    v.y = __builtin_fma(a.y, dt, v.y);
+    v.z = __builtin_fma(a.z, dt, v.z);
+    pm[i] = p;
+    vel[i] = v;
+}
+
+// One Runge-Kutta combination (CPU/omp_solver.cpp:544-572, 576-605):
+//   x = x0 + dt * sum_j w_j kv_j,   v = kv_0 + dt * sum_j w_j ka_j
+// over stages j = 0..count-1, kv_j / ka_j being stage j's velocity and
+// acceleration (records j*stride + i; kv_0 is the step's starting velocity).
+// The sums run in stage order from zero, as the reference's do; the mass
+// travels with x0.
+__global__ __launch_bounds__(BLOCK) void k_rk_combine(int n, const P4* __restrict__ x0, const P4* __restrict__ kv,
+                                                      const P4* __restrict__ ka, long stride, RKTerms T, double dt,
+                                                      P4* __restrict__ x_out, P4* __restrict__ v_out) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+    for (int j = 0; j < T.count; j++) {
+        const double w = T.w[j];
+        const P4 u = kv[j * stride + i];
+        const P4 a = ka[j * stride + i];
+        sx = __builtin_fma(w, u.x, sx);
+        sy = __builtin_fma(w, u.y, sy);
+        sz = __builtin_fma(w, u.z, sz);
+        tx = __builtin_fma(w, a.x, tx);
+        ty = __builtin_fma(w, a.y, ty);
+        tz = __builtin_fma(w, a.z, tz);
+    }
+    const P4 p = x0[i];
+    const P4 v = kv[i];
+    x_out[i] = P4{__builtin_fma(sx, dt, p.x), __builtin_fma(sy, dt, p.y), __builtin_fma(sz, dt, p.z), p.m};
+    v_out[i] = P4{__builtin_fma(tx, dt, v.x), __builtin_fma(ty, dt, v.y), __builtin_fma(tz, dt, v.z), 0.0};
+}
+
 // ---------------------------------------------------------------------------
 // Energy diagnostic
 // ---------------------------------------------------------------------------
@@ -447,6 +496,21 @@ hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, 
 hipError_t launch_drift(int n, P4* pm, const P4* vel, double cdt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_drift, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, n, pm, vel, cdt);
+    return hipGetLastError();
+}
+
+hipError_t launch_euler(int n, P4* pm, P4* vel, const P4* acc, double dt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_euler, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, n, pm, vel, acc, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_rk_combine(int n, const P4* x0, const P4* kv, const P4* ka, long stride, const RKTerms& T,
+                             double dt, P4* x_out, P4* v_out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (T.count < 1 || T.count > kMaxRK) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rk_combine, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, n, x0, kv, ka, stride, T, dt,
+                       x_out, v_out);
     return hipGetLastError();
 }
 

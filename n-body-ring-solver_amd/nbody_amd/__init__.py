@@ -40,6 +40,8 @@ EXPORTS = (
     "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
     "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
     "nb_near_accel", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
+    "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
+    "nb_ring_predict",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -103,6 +105,13 @@ def load_library(path: str = LIB_PATH):
     L.nb_snapshot_write.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_size_t, _D, _P3, _P3, _P3]
     L.nb_snapshot_frames.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.nb_snapshot_read.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, _D, _D, _P3, _P3, _P3]
+    L.nb_step_euler.argtypes = [_VP, _SZ, C.c_double, C.c_int, C.c_double]
+    L.nb_runge_kutta.argtypes = [C.c_int, _D, _D, C.POINTER(_SZ)]
+    L.nb_step_rk.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
+    L.nb_force_error.argtypes = [_SZ, _P3, _P3, _D]
+    L.nb_deviation.argtypes = [_SZ, _P3, _P3, _D]
+    L.nb_inertia.argtypes = [_SZ, _D, _P3, _D]
+    L.nb_ring_predict.argtypes = [_SZ, C.c_double, C.c_double, _SZ, _SZ, _P3, _P3]
     L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
     L.nb_learned_train.argtypes = [_VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]
     L.nb_learned_stats_read.argtypes = [_VP, C.POINTER(LearnedStats)]
@@ -186,6 +195,52 @@ def yoshida(order: int):
     lf = _SZ()
     _check(L.nb_yoshida(order, _p(c), _p(d), C.byref(lf)), "nb_yoshida")
     return c[:lf.value].copy(), d[:lf.value - 1].copy(), lf.value
+
+
+def runge_kutta(order: int = 4):
+    """The reference's Runge-Kutta tableau: (a ((s-1)^2,), b (s,), s)."""
+    L = load_library()
+    a, b = np.zeros(49), np.zeros(8)
+    s = _SZ()
+    _check(L.nb_runge_kutta(order, _p(a), _p(b), C.byref(s)), "nb_runge_kutta")
+    return a[:(s.value - 1) ** 2].copy(), b[:s.value].copy(), s.value
+
+
+def force_error(calc, ref) -> np.ndarray:
+    """The reference's six force-error metrics of calc against ref ((3, N) each)."""
+    L = load_library()
+    calc = np.ascontiguousarray(calc, dtype=np.float64)
+    ref = np.ascontiguousarray(ref, dtype=np.float64)
+    out = np.empty(6)
+    _check(L.nb_force_error(calc.shape[1], _planes(calc), _planes(ref), _p(out)), "nb_force_error")
+    return out
+
+
+def deviation(calc, ref) -> np.ndarray:
+    """(mean |calc - ref|, max relative length deviation)."""
+    L = load_library()
+    calc = np.ascontiguousarray(calc, dtype=np.float64)
+    ref = np.ascontiguousarray(ref, dtype=np.float64)
+    out = np.empty(2)
+    _check(L.nb_deviation(calc.shape[1], _planes(calc), _planes(ref), _p(out)), "nb_deviation")
+    return out
+
+
+def inertia(mass, pos) -> float:
+    L = load_library()
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    out = C.c_double()
+    _check(L.nb_inertia(mass.shape[0], _p(mass), _planes(pos), C.byref(out)), "nb_inertia")
+    return out.value
+
+
+def ring_predict(n: int, R: float, omega: float, step: int, steps_per_rev: int):
+    """Rigid-rotation positions and velocities of the ring after step/steps_per_rev revolutions."""
+    L = load_library()
+    pos, vel = np.empty((3, n)), np.empty((3, n))
+    _check(L.nb_ring_predict(n, R, omega, step, steps_per_rev, _planes(pos), _planes(vel)), "nb_ring_predict")
+    return pos, vel
 
 
 def mlp_param_count() -> int:
@@ -275,6 +330,15 @@ class Solver:
         c = np.ascontiguousarray(c, dtype=np.float64)
         d = np.ascontiguousarray(d, dtype=np.float64)
         _check(self.L.nb_step(self.h, steps, dt, _p(c), _p(d), lf, _form(form), eps), "nb_step")
+
+    def step_euler(self, steps: int, dt: float, form="fp64", eps: float = 0.0):
+        _check(self.L.nb_step_euler(self.h, steps, dt, _form(form), eps), "nb_step_euler")
+
+    def step_rk(self, steps: int, dt: float, form="fp64", eps: float = 0.0, tableau=None):
+        a, b, s = tableau if tableau is not None else runge_kutta(4)
+        a = np.ascontiguousarray(a if len(a) else np.zeros(1), dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        _check(self.L.nb_step_rk(self.h, steps, dt, _p(a), _p(b), s, _form(form), eps), "nb_step_rk")
 
     def energy(self, eps: float = 0.0):
         T, U = C.c_double(), C.c_double()

@@ -18,6 +18,9 @@
  *   - reduced (Newton's 3rd law) evaluation: each unordered pair once, +f on
  *     k and -f on q   CPU/force_calculation.cpp:477-690
  *   - DKD Yoshida leapfrog   CPU/omp_solver.cpp:305-421
+ *   - explicit Euler and Runge-Kutta drivers   CPU/omp_solver.cpp:60-137, 512-606
+ *   - force-error / deviation / inertia metrics of the accuracy experiments
+ *       CPU/nbody_system.cpp:437-567, CPU/force_calculation.cpp:1911-1919
  *   - Yoshida c/d coefficients   CPU/omp_solver.cpp:677-726
  *   - kinetic / potential energy   CPU/force_calculation.cpp:1740-1748, 1880-1904
  *   - ring + centre initial condition and its analytic force
@@ -441,9 +444,160 @@ ORC_API void orc_leapfrog(size_t n, const double *m, double *x, double *y, doubl
     free(ax);
 }
 
+/* Explicit Euler, restating reduced_solver_omp (CPU/omp_solver.cpp:60-137):
+ * per step, accelerations of the current positions, then x += v dt with the
+ * velocity before the kick, then v += a dt (the reference: v += F dt / m). */
+ORC_API void orc_euler(size_t n, const double *m, double *x, double *y, double *z,
+                       double *vx, double *vy, double *vz, size_t steps, double dt,
+                       double eps, double G)
+{
+    double *ax = (double *)malloc(3 * (n ? n : 1) * sizeof(double));
+    double *ay = ax + n, *az = ax + 2 * n;
+    for (size_t t = 0; t < steps; t++) {
+        orc_accel(n, m, x, y, z, eps, G, ax, ay, az);
+        for (size_t i = 0; i < n; i++) {
+            x[i] += vx[i] * dt;
+            y[i] += vy[i] * dt;
+            z[i] += vz[i] * dt;
+            vx[i] += ax[i] * dt;
+            vy[i] += ay[i] * dt;
+            vz[i] += az[i] * dt;
+        }
+    }
+    free(ax);
+}
+
+/* Explicit Runge-Kutta for x' = v, v' = a(x), restating reduced_solver_omp_rk
+ * (CPU/omp_solver.cpp:512-606): stage k records its velocity kv_k (kv_0 = v)
+ * and the acceleration ka_k of its positions; stage k+1 starts at
+ * x0 + (sum_{j<=k} a[k*(s-1)+j] kv_j) dt and v0 + (sum_j a[...] ka_j) dt; the
+ * step ends at x0 + (sum_j b_j kv_j) dt, v0 + (sum_j b_j ka_j) dt.  Sums run in
+ * stage order from zero, as the reference's.  s <= 8. */
+ORC_API void orc_rk(size_t n, const double *m, double *x, double *y, double *z,
+                    double *vx, double *vy, double *vz, size_t steps, double dt,
+                    const double *a, const double *b, size_t s, double eps, double G)
+{
+    const size_t nn = n ? n : 1;
+    double *buf = (double *)malloc((12 + 6 * s) * nn * sizeof(double));
+    double *x0 = buf;              /* step start: component d at x0 + d nn */
+    double *u0 = x0 + 3 * nn;
+    double *kv = u0 + 3 * nn;      /* stage k, component d: kv + (3k+d) nn */
+    double *ka = kv + 3 * s * nn;
+    double *sx = ka + 3 * s * nn;  /* stage positions */
+    double *sv = sx + 3 * nn;      /* stage velocities */
+    const double *X[3] = {x, y, z};
+    const double *V[3] = {vx, vy, vz};
+    for (size_t t = 0; t < steps; t++) {
+        for (int d = 0; d < 3; d++) {
+            memcpy(x0 + d * nn, X[d], n * sizeof(double));
+            memcpy(u0 + d * nn, V[d], n * sizeof(double));
+            memcpy(sx + d * nn, X[d], n * sizeof(double));
+            memcpy(sv + d * nn, V[d], n * sizeof(double));
+        }
+        for (size_t k = 0; k < s; k++) {
+            for (int d = 0; d < 3; d++) memcpy(kv + (3 * k + d) * nn, sv + d * nn, n * sizeof(double));
+            orc_accel(n, m, sx, sx + nn, sx + 2 * nn, eps, G, ka + 3 * k * nn, ka + (3 * k + 1) * nn,
+                      ka + (3 * k + 2) * nn);
+            if (k + 1 == s) break;
+            for (int d = 0; d < 3; d++)
+                for (size_t i = 0; i < n; i++) {
+                    double px = 0.0, pv = 0.0;
+                    for (size_t j = 0; j <= k; j++) {
+                        px += a[k * (s - 1) + j] * kv[(3 * j + d) * nn + i];
+                        pv += a[k * (s - 1) + j] * ka[(3 * j + d) * nn + i];
+                    }
+                    sx[d * nn + i] = px * dt + x0[d * nn + i];
+                    sv[d * nn + i] = pv * dt + u0[d * nn + i];
+                }
+        }
+        double *Xo[3] = {x, y, z};
+        double *Vo[3] = {vx, vy, vz};
+        for (int d = 0; d < 3; d++)
+            for (size_t i = 0; i < n; i++) {
+                double px = 0.0, pv = 0.0;
+                for (size_t j = 0; j < s; j++) {
+                    px += b[j] * kv[(3 * j + d) * nn + i];
+                    pv += b[j] * ka[(3 * j + d) * nn + i];
+                }
+                Xo[d][i] = px * dt + x0[d * nn + i];
+                Vo[d][i] = pv * dt + u0[d * nn + i];
+            }
+    }
+    free(buf);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Diagnostics                                                               */
 /* ------------------------------------------------------------------------- */
+
+/* Force-error metrics, restating calculate_force_error
+ * (CPU/nbody_system.cpp:458-567, double-precision branch):
+ *   out[0] mean |F-Fr|, out[1] max |F-Fr|,
+ *   out[2] sum of |atan2(det, dot)| of the xy projections over particles
+ *          with Fr_x != 0 and Fr_y != 0, / (2 pi N),
+ *   out[3] sum ||F|-|Fr|| / (L0 N), out[4] max angle / (2 pi),
+ *   out[5] max ||F|-|Fr|| / L0,
+ * L0 = |Fr_0| computed in float from float-rounded components (the reference
+ * reads its single-precision copy there, :566).  The reference's compiled
+ * loop contracts products into fused multiply-adds (|v|^2 as
+ * fma(z, z, fma(x, x, y*y)), dot as fma(fx, rx, fy*ry), det as
+ * fma(fx, ry, -(fy*rx))); they are written out so the values agree bit for bit. */
+static double orc_sq3(double x, double y, double z) { return fma(z, z, fma(x, x, y * y)); }
+
+ORC_API void orc_force_error(size_t n, const double *fx, const double *fy, const double *fz,
+                             const double *rx, const double *ry, const double *rz, double *out)
+{
+    double err = 0.0, lerr = 0.0, lmax = 0.0, aerr = 0.0, amax = 0.0, dmax = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        double dl = sqrt(orc_sq3(fx[i] - rx[i], fy[i] - ry[i], fz[i] - rz[i]));
+        double sl = sqrt(orc_sq3(fx[i], fy[i], fz[i]));
+        double rl = sqrt(orc_sq3(rx[i], ry[i], rz[i]));
+        double ld = fabs(sl - rl);
+        double ang = fabs(atan2(fma(fx[i], ry[i], -(fy[i] * rx[i])), fma(fx[i], rx[i], fy[i] * ry[i])));
+        int xy = rx[i] != 0.0 && ry[i] != 0.0;
+        if (dl > dmax) dmax = dl;
+        if (ld > lmax) lmax = ld;
+        if (xy && ang > amax) amax = ang;
+        err += dl;
+        if (xy) aerr += ang;
+        lerr += ld;
+    }
+    float gx = (float)rx[0], gy = (float)ry[0], gz = (float)rz[0];
+    float l2 = fmaf(gz, gz, fmaf(gx, gx, gy * gy));
+    double L0 = (double)sqrtf(l2);
+    out[0] = err / (double)n;
+    out[1] = dmax;
+    out[2] = aerr / (2 * M_PI * (double)n);
+    out[3] = lerr / (L0 * (double)n);
+    out[4] = amax / (2 * M_PI);
+    out[5] = lmax / L0;
+}
+
+/* calculate_deviation (CPU/nbody_system.cpp:437-456): out[0] mean |c - r|,
+ * out[1] max ||c| - |r|| / |r| over |r| > 0 (|v|^2 contracted as above). */
+ORC_API void orc_deviation(size_t n, const double *cx, const double *cy, const double *cz,
+                           const double *rx, const double *ry, const double *rz, double *out)
+{
+    double mean = 0.0, worst = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        mean += sqrt(orc_sq3(cx[i] - rx[i], cy[i] - ry[i], cz[i] - rz[i]));
+        double lc = sqrt(orc_sq3(cx[i], cy[i], cz[i]));
+        double lr = sqrt(orc_sq3(rx[i], ry[i], rz[i]));
+        if (lr > 0.0 && fabs(lc - lr) / lr > worst) worst = fabs(lc - lr) / lr;
+    }
+    out[0] = mean / (double)n;
+    out[1] = worst;
+}
+
+/* compute_inertia (CPU/force_calculation.cpp:1911-1919): sum m |x|^2 in order;
+ * the compiled loop is vectorised with an in-order sum, so m |x|^2 (|x|^2
+ * contracted as above) is rounded before it is added. */
+ORC_API double orc_inertia(size_t n, const double *m, const double *x, const double *y, const double *z)
+{
+    double I = 0.0;
+    for (size_t i = 0; i < n; i++) I += m[i] * orc_sq3(x[i], y[i], z[i]);
+    return I;
+}
 
 /* T = 1/2 sum m v^2 (CPU/force_calculation.cpp:1740-1748) and
  * U = -G sum_{k<q} m_k m_q / sqrt(r^2 + eps^2) (:1880-1904 with softening, as

@@ -55,6 +55,13 @@ class Oracle:
         L.orc_leapfrog.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _SZ, C.c_double, _D, _D, _SZ,
                                    C.c_double, C.c_double]
         L.orc_energy.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, C.c_double, C.c_double, _D, _D]
+        L.orc_euler.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _SZ, C.c_double, C.c_double, C.c_double]
+        L.orc_rk.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _SZ, C.c_double, _D, _D, _SZ, C.c_double,
+                             C.c_double]
+        L.orc_force_error.argtypes = [_SZ] + [_D] * 7
+        L.orc_deviation.argtypes = [_SZ] + [_D] * 7
+        L.orc_inertia.argtypes = [_SZ, _D, _D, _D, _D]
+        L.orc_inertia.restype = C.c_double
         L.orc_max_rel_err.argtypes = [_SZ] + [_D] * 6 + [_D]
         L.orc_max_rel_err.restype = C.c_double
         L.orc_poly_extrap.argtypes = [_SZ, _D, _D, _D, _D, _D]
@@ -147,6 +154,47 @@ class Oracle:
         self.lib.orc_leapfrog(m.shape[0], _p(m), *[_p_row(pos, k) for k in range(3)],
                               *[_p_row(vel, k) for k in range(3)], steps, dt, _p(c), _p(d), lf, eps, G)
         return pos, vel
+
+    def euler(self, m, pos, vel, steps: int, dt: float, eps: float = 0.0, G: float = 1.0):
+        pos = np.ascontiguousarray(pos.copy())
+        vel = np.ascontiguousarray(vel.copy())
+        self.lib.orc_euler(m.shape[0], _p(m), *[_p_row(pos, k) for k in range(3)],
+                           *[_p_row(vel, k) for k in range(3)], steps, dt, eps, G)
+        return pos, vel
+
+    RK4_A = np.array([0.5, 0.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 1.0])
+    RK4_B = np.array([1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0])
+
+    def rk(self, m, pos, vel, steps: int, dt: float, a=None, b=None, eps: float = 0.0, G: float = 1.0):
+        """Explicit Runge-Kutta with tableau (a, b) in the reference's layout; classic RK4 by default."""
+        a = np.ascontiguousarray(self.RK4_A if a is None else a, dtype=np.float64)
+        b = np.ascontiguousarray(self.RK4_B if b is None else b, dtype=np.float64)
+        s = b.shape[0]
+        assert a.shape[0] >= (s - 1) ** 2
+        pos = np.ascontiguousarray(pos.copy())
+        vel = np.ascontiguousarray(vel.copy())
+        self.lib.orc_rk(m.shape[0], _p(m), *[_p_row(pos, k) for k in range(3)],
+                        *[_p_row(vel, k) for k in range(3)], steps, dt, _p(a if a.size else np.zeros(1)), _p(b),
+                        s, eps, G)
+        return pos, vel
+
+    def force_error(self, calc, ref) -> np.ndarray:
+        calc, ref = np.ascontiguousarray(calc), np.ascontiguousarray(ref)
+        out = np.empty(6)
+        self.lib.orc_force_error(calc.shape[1], *[_p_row(calc, k) for k in range(3)],
+                                 *[_p_row(ref, k) for k in range(3)], _p(out))
+        return out
+
+    def deviation(self, calc, ref) -> np.ndarray:
+        calc, ref = np.ascontiguousarray(calc), np.ascontiguousarray(ref)
+        out = np.empty(2)
+        self.lib.orc_deviation(calc.shape[1], *[_p_row(calc, k) for k in range(3)],
+                               *[_p_row(ref, k) for k in range(3)], _p(out))
+        return out
+
+    def inertia(self, m, pos) -> float:
+        pos = np.ascontiguousarray(pos)
+        return self.lib.orc_inertia(m.shape[0], _p(m), *[_p_row(pos, k) for k in range(3)])
 
     def energy(self, m, pos, vel, eps: float, G: float = 1.0):
         T, U = C.c_double(), C.c_double()
@@ -244,6 +292,13 @@ class Reference:
         L.ref_init_ring.argtypes = [_SZ, C.c_double, C.c_double, C.c_double, _D, P3, P3, P3]
         L.ref_energy.argtypes = [_SZ, _D, P3, P3, _D, _D, _D]
         L.ref_save_particles.argtypes = [_SZ, _D, P3, P3, P3, C.c_double, _SZ]
+        L.ref_euler.argtypes = [_SZ, _D, P3, P3, _SZ, C.c_double]
+        L.ref_rk.argtypes = [_SZ, _D, P3, P3, _SZ, C.c_double, _D, _D, _SZ]
+        L.ref_force_error.argtypes = [_SZ, P3, P3, _D]
+        L.ref_deviation.argtypes = [_SZ, P3, P3, _D]
+        L.ref_inertia.argtypes = [_SZ, _D, P3]
+        L.ref_inertia.restype = C.c_double
+        L.ref_ring_predict.argtypes = [_SZ, C.c_double, C.c_double, _SZ, _SZ, P3, P3]
 
     def forces(self, variant: int, m, pos) -> np.ndarray:
         n = m.shape[0]
@@ -273,6 +328,40 @@ class Reference:
         pos, vel, force = (np.ascontiguousarray(a) for a in (pos, vel, force))
         rc = self.lib.ref_save_particles(m.shape[0], _p(m), _ptr3(pos), _ptr3(vel), _ptr3(force), dt, frames)
         assert rc == 0
+
+    def euler(self, m, pos, vel, steps: int, dt: float):
+        pos = np.ascontiguousarray(pos.copy())
+        vel = np.ascontiguousarray(vel.copy())
+        assert self.lib.ref_euler(m.shape[0], _p(m), _ptr3(pos), _ptr3(vel), steps, dt) == 0
+        return pos, vel
+
+    def rk(self, m, pos, vel, steps: int, dt: float, a, b):
+        pos = np.ascontiguousarray(pos.copy())
+        vel = np.ascontiguousarray(vel.copy())
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert self.lib.ref_rk(m.shape[0], _p(m), _ptr3(pos), _ptr3(vel), steps, dt, _p(a), _p(b), b.shape[0]) == 0
+        return pos, vel
+
+    def force_error(self, calc, ref) -> np.ndarray:
+        calc, ref = np.ascontiguousarray(calc), np.ascontiguousarray(ref)
+        out = np.empty(6)
+        self.lib.ref_force_error(calc.shape[1], _ptr3(calc), _ptr3(ref), _p(out))
+        return out
+
+    def deviation(self, calc, ref) -> np.ndarray:
+        calc, ref = np.ascontiguousarray(calc), np.ascontiguousarray(ref)
+        out = np.empty(2)
+        self.lib.ref_deviation(calc.shape[1], _ptr3(calc), _ptr3(ref), _p(out))
+        return out
+
+    def inertia(self, m, pos) -> float:
+        return self.lib.ref_inertia(m.shape[0], _p(m), _ptr3(np.ascontiguousarray(pos)))
+
+    def ring_predict(self, n: int, R: float, omega: float, step: int, steps_per_rev: int):
+        pos, vel = np.empty((3, n)), np.empty((3, n))
+        self.lib.ref_ring_predict(n, R, omega, step, steps_per_rev, _ptr3(pos), _ptr3(vel))
+        return pos, vel
 
     def init_ring(self, n: int, R: float = 1.0, omega: float = 2 * np.pi, ratio: float = 1e5):
         m = np.empty(n)

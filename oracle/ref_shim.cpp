@@ -12,6 +12,7 @@
 // reference routine, and copies the results out.
 #include <omp.h>
 
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -232,6 +233,98 @@ __attribute__((visibility("default"))) int ref_save_particles(size_t n, const do
         s.save_particles(k, dt);
     }
     s.output_file.close();
+    return 0;
+}
+
+// The reference's explicit Euler driver reduced_solver_omp (CPU/omp_solver.cpp:16)
+// in double precision with compute_forces_avx2_blocked; state in place.
+__attribute__((visibility("default"))) int ref_euler(size_t n, const double* m, double* const* pos,
+                                                     double* const* vel, size_t steps, double dt) {
+    NBody_system s;
+    load_system(s, n, m, (const double* const*)pos, (const double* const*)vel);
+    reduced_solver_omp(s, steps, dt, true, Vectorization_type::AVX2, Data_type::DOUBLE_PRECISION);
+    for (int k = 0; k < DIM; k++)
+        for (size_t i = 0; i < n; i++) {
+            pos[k][i] = s.pos[k][i];
+            vel[k][i] = s.vel[k][i];
+        }
+    return 0;
+}
+
+// The reference's Runge-Kutta driver reduced_solver_omp_rk (CPU/omp_solver.cpp:458)
+// with its a[(rk_steps-1)^2] / b[rk_steps]; state in place.  That routine
+// pins OpenMP to one thread (:460); the previous count is restored after it.
+__attribute__((visibility("default"))) int ref_rk(size_t n, const double* m, double* const* pos,
+                                                  double* const* vel, size_t steps, double dt,
+                                                  const double* a, const double* b, size_t rk_steps) {
+    if (rk_steps < 1 || rk_steps > 8) return 1;
+    NBody_system s;
+    load_system(s, n, m, (const double* const*)pos, (const double* const*)vel);
+    double aa[64] = {0}, bb[8] = {0};
+    memcpy(aa, a, (rk_steps - 1) * (rk_steps - 1) * sizeof(double));
+    memcpy(bb, b, rk_steps * sizeof(double));
+    const int threads = omp_get_max_threads();
+    reduced_solver_omp_rk(s, steps, dt, aa, bb, rk_steps, true);
+    omp_set_num_threads(threads);
+    for (int k = 0; k < DIM; k++)
+        for (size_t i = 0; i < n; i++) {
+            pos[k][i] = s.pos[k][i];
+            vel[k][i] = s.vel[k][i];
+        }
+    return 0;
+}
+
+// calculate_force_error (CPU/nbody_system.cpp:458) in double precision:
+// sys.force = calc, ref.force = ref and ref's single-precision copy rounded
+// from it, as the ring initialisation leaves it (:262-275).
+__attribute__((visibility("default"))) int ref_force_error(size_t n, const double* const* calc,
+                                                           const double* const* ref, double* out) {
+    NBody_system a, b;
+    a.alloc_particles(n, n);
+    b.alloc_particles(n, n);
+    for (int d = 0; d < DIM; d++)
+        for (size_t i = 0; i < n; i++) {
+            a.force[d][i] = calc[d][i];
+            a.force_sp[d][i] = float(calc[d][i]);
+            b.force[d][i] = ref[d][i];
+            b.force_sp[d][i] = float(ref[d][i]);
+        }
+    const std::array<double, 6> e = calculate_force_error(a, b, Data_type::DOUBLE_PRECISION, false);
+    for (int k = 0; k < 6; k++) out[k] = e[k];
+    return 0;
+}
+
+// calculate_deviation (CPU/nbody_system.cpp:437).
+__attribute__((visibility("default"))) int ref_deviation(size_t n, double* const* calc, double* const* ref,
+                                                         double* out) {
+    double* c[DIM] = {calc[0], calc[1], calc[2]};
+    double* r[DIM] = {ref[0], ref[1], ref[2]};
+    const std::array<double, 2> e = calculate_deviation(c, r, n);
+    out[0] = e[0];
+    out[1] = e[1];
+    return 0;
+}
+
+// compute_inertia (CPU/force_calculation.cpp:1911).
+__attribute__((visibility("default"))) double ref_inertia(size_t n, const double* m, const double* const* pos) {
+    NBody_system s;
+    load_system(s, n, m, pos, nullptr);
+    return compute_inertia(s);
+}
+
+// NBody_system::predict_stable_orbiting_particles (CPU/nbody_system.cpp:302)
+// applied to a ring of n particles set up by init_stable_orbiting_particles.
+__attribute__((visibility("default"))) int ref_ring_predict(size_t n, double R, double omega, size_t step,
+                                                            size_t steps_per_rev, double* const* pos,
+                                                            double* const* vel) {
+    NBody_system s;
+    s.init_stable_orbiting_particles(n, R, omega, 1.0);
+    s.predict_stable_orbiting_particles(R, omega, step, steps_per_rev);
+    for (int d = 0; d < DIM; d++)
+        for (size_t i = 0; i < n; i++) {
+            pos[d][i] = s.pos[d][i];
+            vel[d][i] = s.vel[d][i];
+        }
     return 0;
 }
 

@@ -77,7 +77,27 @@ def test_argument_errors():
     assert L.nb_set_state(None, 0, None, None, None) == 1
     assert L.nb_step(None, 1, 1e-3, None, None, 2, 0, 0.0) == 1
     assert L.nb_energy(None, 0.0, None, None) == 1
+    assert L.nb_step_euler(None, 1, 1e-3, 0, 0.0) == 1
+    assert L.nb_step_rk(None, 1, 1e-3, None, None, 4, 0, 0.0) == 1
+    assert L.nb_runge_kutta(4, None, None, None) == 1
     assert L.nb_destroy(None) == 0
+
+
+def test_cli_refuses_run_types_it_does_not_provide():
+    # the reference's cache/scaling/MPI/hybrid run types end with a message
+    # before any device is touched
+    cli = os.path.join(ROOT, "n-body-ring-solver_amd", "bin", "nbody")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    for t in ("omp-weak", "hybrid", "mpi-bandwidth"):
+        r = subprocess.run([cli, "-t", t, "-N", "16"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "not provided" in r.stdout
+    r = subprocess.run([cli, "-t", "bogus", "-i", "bogus", "-S", "x", "-p", "x", "-V", "x", "-N", "16", "-device", "99"],
+                       capture_output=True, text=True, timeout=60)
+    for msg in ("Unknown Execution type: bogus", "Defaulting to omp", "Unknown Integration kind: bogus",
+                "Defaulting to euler", "Unknown Solver type: x", "Unknown Data type: x", "Unknown Vectorization type: x"):
+        assert msg in r.stdout, msg
+    assert r.returncode != 0
 
 
 def _gpu_present() -> bool:
