@@ -29,7 +29,8 @@
 // each 64-particle batch of a run only the particles within rc of the chunk's
 // bounding box are kept (ballot + prefix count, order preserved), so far
 // corners of the cell box cost one distance test instead of a pair each.
-// Pairs beyond rc get S = 0, so the pair code is branch-free.
+// Pairs beyond rc get S = 0, so the pair code is branch-free; the FP32
+// variant stages the masses already rounded to FP32.
 //
 // FP64 throughout, or (fp32) FP64 differences with FP32 pair arithmetic and
 // FP32 sums over each 64-particle batch, flushed to FP64 -- the precision of
@@ -181,49 +182,75 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Taper argument: with s2 = r^2 + eps^2, u = (q - 1/4) (4/3) with
+// q = r^2 / rc^2 is s2 k1 + k0, k1 = 4 / (3 rc^2), k0 = -(1/3 + k1 eps^2), so
+// the pair needs r^2 + eps^2 only (one fused chain seeded with eps^2).
+struct Taper {
+    double k1, k0;
+};
+
+__host__ __device__ inline Taper make_taper(double rc, double eps) {
+    const double k1 = 4.0 / (3.0 * rc * rc);
+    return Taper{k1, -(1.0 / 3.0 + k1 * eps * eps)};
+}
+
 // One ordered pair: a += m S(q) w d, w = (r^2 + eps^2)^(-3/2) from v_rsq_f64
 // and the correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2 (e = 1 - s2 y^2).
-__device__ __forceinline__ void near_pair(const nbk::P4& me, const nbk::P4& o, double inv_rc2, double eps2,
+// For eps > 0 a zero separation needs no test: w stays finite and d = 0
+// makes the term vanish.  With eps = 0 (EPS0) such pairs are selected out.
+template <bool EPS0>
+__device__ __forceinline__ void near_pair(const nbk::P4& me, const nbk::P4& o, double k1, double k0, double eps2,
                                           double& ax, double& ay, double& az) {
     const double dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
-    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    const double u = fmin(fmax((r2 * inv_rc2 - 0.25) * (4.0 / 3.0), 0.0), 1.0);
+    const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+    const double u = fmin(fmax(__builtin_fma(s2, k1, k0), 0.0), 1.0);
     const double S = __builtin_fma(-u * u * u, __builtin_fma(u, __builtin_fma(6.0, u, -15.0), 10.0), 1.0);
-    const double s2 = r2 + eps2;
     const double y = __builtin_amdgcn_rsq(s2);
     const double e = __builtin_fma(-s2, y * y, 1.0);
     const double w = (y * y) * y * __builtin_fma(e, __builtin_fma(1.875, e, 1.5), 1.0);
-    const double f = (r2 > 0.0) ? o.m * S * w : 0.0;
+    double f = o.m * S * w;
+    if (EPS0) f = (s2 > 0.0) ? f : 0.0;
     ax = __builtin_fma(f, dx, ax);
     ay = __builtin_fma(f, dy, ay);
     az = __builtin_fma(f, dz, az);
 }
 
+// Staged record of the FP32 pair: FP64 position, the mass already rounded to
+// FP32 and kept in the low word of the fourth slot.
+__device__ __forceinline__ nbk::P4 stage32(const nbk::P4& p) {
+    return nbk::P4{p.x, p.y, p.z, __builtin_bit_cast(double, make_int2(__builtin_bit_cast(int, (float)p.m), 0))};
+}
+
+__device__ __forceinline__ float staged_mass(const nbk::P4& o) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(int2, o.m).x);
+}
+
 // The same pair with FP64 differences rounded to FP32 and FP32 arithmetic:
 // FP32 taper, v_rsq_f32 with the first-order correction of the cube,
-// w = y^3 (1 + 3/2 (1 - s2 y^2)).
-__device__ __forceinline__ void near_pair32(const nbk::P4& me, const nbk::P4& o, float inv_rc2, float eps2,
+// w = y^3 (1 + 3/2 (1 - s2 y^2)); o is a stage32 record.
+template <bool EPS0>
+__device__ __forceinline__ void near_pair32(const nbk::P4& me, const nbk::P4& o, float k1, float k0, float eps2,
                                             float& ax, float& ay, float& az) {
     const float dx = (float)(o.x - me.x), dy = (float)(o.y - me.y), dz = (float)(o.z - me.z);
-    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-    const float u = fminf(fmaxf((r2 * inv_rc2 - 0.25f) * (4.0f / 3.0f), 0.0f), 1.0f);
+    const float s2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    const float u = fminf(fmaxf(__builtin_fmaf(s2, k1, k0), 0.0f), 1.0f);
     const float S = __builtin_fmaf(-u * u * u, __builtin_fmaf(u, __builtin_fmaf(6.0f, u, -15.0f), 10.0f), 1.0f);
-    const float s2 = r2 + eps2;
     const float y = __builtin_amdgcn_rsqf(s2);
     const float t = y * y;
     const float w = t * y * __builtin_fmaf(__builtin_fmaf(-s2, t, 1.0f), 1.5f, 1.0f);
-    const float f = (r2 > 0.0f) ? (float)o.m * S * w : 0.0f;
+    float f = staged_mass(o) * S * w;
+    if (EPS0) f = (s2 > 0.0f) ? f : 0.0f;
     ax = __builtin_fmaf(f, dx, ax);
     ay = __builtin_fmaf(f, dy, ay);
     az = __builtin_fmaf(f, dz, az);
 }
 
-template <bool F32>
+template <bool F32, bool EPS0>
 __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict__ sorted,
                                                       const unsigned long long* __restrict__ keys,
                                                       const int* __restrict__ order, const int2* __restrict__ cols,
-                                                      const int* __restrict__ chunks, int n, double inv_rc2,
-                                                      double eps2, nbk::P4* __restrict__ out) {
+                                                      const int* __restrict__ chunks, int n, double rc2,
+                                                      Taper tp, double eps2, nbk::P4* __restrict__ out) {
     __shared__ nbk::P4 stage[kWaves][64];
     __shared__ nbk::P4 part[kWaves][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -251,7 +278,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
     // bounding box of the chunk's particles (lanes past the chunk repeat its first)
     const double3 lo = make_double3(wave_min(me.x), wave_min(me.y), wave_min(me.z));
     const double3 hi = make_double3(wave_max(me.x), wave_max(me.y), wave_max(me.z));
-    const double rc2 = 1.0 / inv_rc2;
     nbk::P4* L = stage[w];
     for (int k = 0; k < ny; k++) {
         const int ak = __shfl(a, k, 64), bk = __shfl(b, k, 64);
@@ -272,25 +298,31 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
             const int nkeep = __popcll(keep);
             if (nkeep == 0) continue;   // wave-uniform
             const int m4 = (nkeep + 3) & ~3;   // fillers pad to a multiple of 4
-            if (use) L[__popcll(keep & ((1ull << lane) - 1ull))] = J;
+            if (use) L[__popcll(keep & ((1ull << lane) - 1ull))] = F32 ? stage32(J) : J;
             // filler: this lane's position with zero mass, so it adds nothing
-            // to any lane (r^2 = 0 for this one, m = 0 for the others)
+            // to any lane (d = 0 for this one, m = 0 for the others)
             if (lane >= nkeep && lane < m4) L[lane] = nbk::P4{me.x, me.y, me.z, 0.0};
             wave_sync();
             if constexpr (F32) {
-                const float ir = (float)inv_rc2, e2 = (float)eps2;
+                const float k1 = (float)tp.k1, k0 = (float)tp.k0, e2 = (float)eps2;
                 float fx = 0.f, fy = 0.f, fz = 0.f;
                 for (int t = 0; t < m4; t += 4) {
+                    nbk::P4 o[4];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) near_pair32(me, L[t + q], ir, e2, fx, fy, fz);
+                    for (int q = 0; q < 4; q++) o[q] = L[t + q];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) near_pair32<EPS0>(me, o[q], k1, k0, e2, fx, fy, fz);
                 }
                 ax += fx;
                 ay += fy;
                 az += fz;
             } else {
                 for (int t = 0; t < m4; t += 4) {
+                    nbk::P4 o[4];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) near_pair(me, L[t + q], inv_rc2, eps2, ax, ay, az);
+                    for (int q = 0; q < 4; q++) o[q] = L[t + q];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) near_pair<EPS0>(me, o[q], tp.k1, tp.k0, eps2, ax, ay, az);
                 }
             }
             wave_sync();
@@ -360,14 +392,11 @@ hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk
     if ((e = hipMemcpyAsync(&count, nchunks, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
     if (count <= 0 || count > n) return hipErrorUnknown;
-    if (fp32)
-        hipLaunchKernelGGL(k_near<true>, dim3(count), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
-                           (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, n, 1.0 / (rc * rc),
-                           eps * eps, out);
-    else
-        hipLaunchKernelGGL(k_near<false>, dim3(count), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
-                           (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, n, 1.0 / (rc * rc),
-                           eps * eps, out);
+    auto kernel = fp32 ? (eps > 0.0 ? k_near<true, false> : k_near<true, true>)
+                       : (eps > 0.0 ? k_near<false, false> : k_near<false, true>);
+    hipLaunchKernelGGL(kernel, dim3(count), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
+                       (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, n, rc * rc,
+                       make_taper(rc, eps), eps * eps, out);
     return hipGetLastError();
 }
 
