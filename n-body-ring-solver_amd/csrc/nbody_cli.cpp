@@ -243,11 +243,18 @@ std::vector<size_t> seeded_permutation(size_t n, uint64_t seed) {
     return perm;
 }
 
-double mean_deviation(const Planes& a, const Planes& b, size_t n) {
+// Mean |a - b| over the particles; with verbose, every particle's values and
+// difference as print_deviation writes them (CPU/nbody_system.cpp:416-435).
+double mean_deviation(const Planes& a, const Planes& b, size_t n, const char* kind, bool verbose) {
     double s = 0.0;
     for (size_t i = 0; i < n; i++) {
         const double dx = a.p[0][i] - b.p[0][i], dy = a.p[1][i] - b.p[1][i], dz = a.p[2][i] - b.p[2][i];
         s += std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (verbose) {
+            printf("calc_%s[%zu]=[%f,%f,%f]\n", kind, i, a.p[0][i], a.p[1][i], a.p[2][i]);
+            printf("ref_%s[%zu] =[%f,%f,%f]\n", kind, i, b.p[0][i], b.p[1][i], b.p[2][i]);
+            printf("%s_diff[%zu] =[%f,%f,%f]\n\n", kind, i, dx, dy, dz);
+        }
     }
     return n ? s / double(n) : 0.0;
 }
@@ -361,7 +368,8 @@ int run_omp(nb_ctx* ctx, Args& a) {
 
     Planes pos(N), vel(N);
     CHECK(nb_get_state(ctx, pos.p, vel.p));
-    const double pos_dev = mean_deviation(pos, pos0, N), vel_dev = mean_deviation(vel, vel0, N);
+    const double pos_dev = mean_deviation(pos, pos0, N, "position", a.verbose);
+    const double vel_dev = mean_deviation(vel, vel0, N, "velocity", a.verbose);
     printf("Average position deviation from reference: %.13f\n", pos_dev);
     printf("Average velocity deviation from reference: %.13f\n", vel_dev);
     double T1, U1;

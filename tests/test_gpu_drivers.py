@@ -201,3 +201,24 @@ def test_cli_ratio_experiment_vs_oracle(orc, tmp_path):
     assert abs(e_dev - want_e) <= 1e-6 * want_e + 1e-15
     assert abs(pmax - orc.deviation(p, pp)[1]) <= 1e-6 * pmax
     assert abs(vmax - orc.deviation(v, vp)[1]) <= 1e-6 * vmax
+
+
+def test_cli_csv_row_and_verbose(tmp_path):
+    csv = str(tmp_path / "run.csv")
+    out = cli("-t", "omp", "-N", "16", "-s", "3", "-T", "0.003", "-i", "lf", "-order", "2", "-v", "-o", csv)
+    assert "calc_position[15]=" in out and "ref_velocity[0] =" in out and "velocity_diff[0] =" in out
+    lines = open(csv).read().strip().splitlines()
+    assert lines[0].startswith("N, s, T, dt, integrator, order, eps")
+    row = [x.strip() for x in lines[1].split(",")]
+    assert row[0] == "16" and row[1] == "3" and row[4] == "lf" and row[5] == "2" and row[9] == "fp64"
+    cli("-t", "omp", "-N", "16", "-s", "3", "-T", "0.003", "-i", "rk", "-order", "4", "-p", "sp", "-o", csv)
+    row = [x.strip() for x in open(csv).read().strip().splitlines()[2].split(",")]
+    assert row[4] == "rk" and row[5] == "4" and row[9] == "mixed"
+
+
+def test_cli_learned_form():
+    out = cli("-t", "omp", "-N", "4096", "-ic", "plummer", "-eps", "0.01", "-s", "40", "-T", str(40 / 256),
+              "-form", "learned", "-train-steps", "24", "-train-iters", "50")
+    assert "using -i lf" in out and "Training the extrapolator" in out and "final training loss" in out
+    e = float(re.search(r"\|dE/E\| = (\S+)", out).group(1))
+    assert e < 1e-4
