@@ -170,7 +170,9 @@ static hipError_t sym_plan(nb_ctx* c, int K) {
 
 // Which kernel evaluates the resident system: the symmetric one from 32768
 // particles up (it does half the pair work) while its partial-sum buffer
-// stays within 16 GiB, the tile kernel otherwise.
+// (N^2 / 16 bytes at K = 8) fits: always up to 16 GiB, beyond that while the
+// growth takes at most 80 % of the free device memory (N = 1M needs 64 GiB of
+// the 288 GB of an MI355X); the tile kernel otherwise.
 static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     const long n = long(c->n);
     // 8 particles per lane at large N (fewer LDS reads and rotations per
@@ -178,7 +180,13 @@ static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     K = (n >= 131072) ? 8 : 4;
     const long S = 64L * K;
     const long nB = (n + S - 1) / S;
-    const bool fits = double(nB) * nB * S * sizeof(P4) <= 16.0 * (1L << 30);
+    const double need = double(nB) * nB * S * sizeof(P4);
+    bool fits = need <= 16.0 * (1L << 30) || need <= double(c->part.bytes);
+    if (!fits) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            fits = need - double(c->part.bytes) <= 0.8 * double(free_b);
+    }
     if (c->kernel == NB_KERNEL_TILE) return false;
     if (c->kernel == NB_KERNEL_SYMMETRIC) return fits && n >= 2;
     return fits && n >= 32768;

@@ -216,3 +216,24 @@ def test_symmetric_coincident_eps0(solver, orc):
         assert rel_err(solver.accel("fp64", 0.0), orc.accel(m, pos, 0.0)) < FP64_TOL
     finally:
         solver.set_kernel("auto")
+
+
+def test_symmetric_beyond_16_gib_of_partials(solver):
+    """N = 600000 needs 21 GiB of partial sums at K = 8: the symmetric kernel
+    still runs (the buffer is sized against the free HBM) and agrees with the
+    tile kernel; its time shows it did half the pair work."""
+    n = 600000
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 31)
+    solver.set_state(m, pos)
+    timed = {}
+    out = {}
+    for kernel in ("tile", "auto"):
+        solver.set_kernel(kernel)
+        solver.timing(True)
+        solver.timing_reset()
+        out[kernel] = solver.accel("mixed", 0.005)
+        timed[kernel] = solver.timing_read()[0]
+        solver.timing(False)
+    solver.set_kernel("auto")
+    assert rel_err(out["auto"], out["tile"]) < 2e-5
+    assert timed["auto"] < 0.8 * timed["tile"], timed
