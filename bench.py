@@ -25,10 +25,7 @@ GPairs/s, steps/s, |dE/E|, speed-up over FP64 and force error (max relative
 against FP64 for fp64/mixed; RMS and max relative at the last check for the
 learned forms).  `roofline` is the dominant kernel of the headline form,
 timed with HIP events around every launch on the solver's stream;
-`rooflines` adds the others; each also gives the time its inner loop's
-instruction mix needs at the issue rates measured on the device
-(`issue_ceiling_ms`, `issue_ceiling_frac` = that time / the measured one).
-`cpu_baseline` times the reference's own
+`rooflines` adds the others.  `cpu_baseline` times the reference's own
 AVX-512 force kernel (compiled from its sources) on this host's cores on a
 bounded sample.
 """
@@ -53,14 +50,6 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_PAIR = 20          # the conventional count for one softened pair interaction
 DE_BOUND = 1e-6
-# Issue cost of the symmetric kernels' inner loops (gfx950 assembly of
-# nbody_sym.hip, K = 8): CU-cycles per ordered pair at the VALU rates that
-# bin/valu_ceiling measures on MI355X.  FP64: 320 FP64 arithmetic, 16
-# v_rsq_f64 and 12 DPP moves per 1024 unordered pairs at 47.8, 15.1 and 58.1
-# lane-ops per CU-cycle; mixed: 23.4 full-rate wave64 slots per unordered pair.
-ISSUE_CU_CYCLES_PER_PAIR = {"fp64": (320 * 64 / 47.79 + 16 * 64 / 15.06 + 12 * 64 / 58.12) / 2048,
-                            "mixed": 23.4 / 2 / 64}
-CLOCK_GHZ = 2.4
 LEARNED = {"period": 4, "threshold": 3e-3, "rc": 0.25, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
 LEARNED_P8 = {"period": 8, "threshold": 4e-3, "rc": 0.35, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
 HEADLINE_FORMS = ("fp64", "mixed", "learned", "learned-cubic")
@@ -187,18 +176,13 @@ def cpu_baseline(state):
             "sample": f"oracle C port, {ni} i-particles x {n} j-particles"}
 
 
-def roofline_entry(name, kernel_ms, n, peak, traffic=None, pairs="N^2 ordered pairs (20 flops each)",
-                   form="fp64", cus=256):
+def roofline_entry(name, kernel_ms, n, peak, traffic=None, pairs="N^2 ordered pairs (20 flops each)"):
     t = kernel_ms * 1e-3
     achieved = FLOPS_PER_PAIR * float(n) * n / t / 1e12
-    # the time the loop's instruction mix needs at the measured issue rates
-    ceiling_ms = ISSUE_CU_CYCLES_PER_PAIR[form] * float(n) * n / (cus * CLOCK_GHZ * 1e9) * 1e3
     return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": traffic, "kernel": name, "launch_avg_ms": kernel_ms,
             "algorithmic_flops_per_launch": FLOPS_PER_PAIR * float(n) * n, "counted_as": pairs,
-            "algorithmic_bytes_per_launch": 64 * n,
-            "issue_ceiling_ms": ceiling_ms, "issue_ceiling_frac": ceiling_ms / kernel_ms,
-            "issue_ceiling_source": "profiles/valu_ceiling_mi355x.md"}
+            "algorithmic_bytes_per_launch": 64 * n}
 
 
 def main():
@@ -272,19 +256,16 @@ def main():
     hr = forms[head]
 
     rooflines = {}
-    cus = torch.cuda.get_device_properties(0).multi_processor_count
     if forms.get("fp64", {}).get("exact_force_evals"):
         rooflines["fp64"] = roofline_entry("k_accel_sym<FP64, K=8> (symmetric, FP64 VALU)",
                                            forms["fp64"]["force_kernel_ms_avg"], n, FP64_PEAK_TFLOPS,
-                                           profile_traffic("k_accel_sym_plummer_n262144_pmc.md"), form="fp64",
-                                           cus=cus)
+                                           profile_traffic("k_accel_sym_plummer_n262144_pmc.md"))
     if forms.get("mixed", {}).get("exact_force_evals"):
         rooflines["mixed"] = roofline_entry("k_accel_sym<MIXED_NR, K=8> (symmetric, FP32 pair math on FP64 "
                                             "differences; against the FP32 peak, which counts packed FP32)",
                                             forms["mixed"]["force_kernel_ms_avg"], n, FP32_PEAK_TFLOPS,
                                             profile_traffic("k_accel_sym_plummer_n262144_pmc.md",
-                                                            "traffic_bytes_per_launch_mixed"), form="mixed",
-                                            cus=cus)
+                                                            "traffic_bytes_per_launch_mixed"))
     dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
     roofline = rooflines.get(dominant) or rooflines.get("fp64")
 

@@ -147,7 +147,12 @@ static hipError_t sym_plan(nb_ctx* c, int K) {
     const long S = 64L * K;
     const long nB = (long(c->n) + S - 1) / S;
     const long tiles = nB * (nB + 1) / 2;
-    const long target = long(c->num_cus) * 16;   // items: ~16 per CU
+    // ~128 items per CU (~43 per wave slot at 3 waves per SIMD): short items
+    // keep the last round of the dispatch from idling most of the chip.
+    // Measured at N = 262144: 16 items per CU (L = 32) 27.6 / 20.9 ms
+    // (FP64 / mixed), 128 per CU (L = 4) 24.4 / 18.0 ms; the extra action
+    // slots cost the reduction ~0.1 ms.
+    const long target = long(c->num_cus) * 128;
     const long L = std::max(1L, tiles / target);
     std::vector<int4> items;
     for (long A = 0; A < nB; A++)
