@@ -147,13 +147,15 @@ static hipError_t sym_plan(nb_ctx* c, int K) {
     const long S = 64L * K;
     const long nB = (long(c->n) + S - 1) / S;
     const long tiles = nB * (nB + 1) / 2;
-    // ~128 items per CU (~43 per wave slot at 3 waves per SIMD): short items
-    // keep the last round of the dispatch from idling most of the chip.
-    // Measured at N = 262144: 16 items per CU (L = 32) 27.6 / 20.9 ms
-    // (FP64 / mixed), 128 per CU (L = 4) 24.4 / 18.0 ms; the extra action
-    // slots cost the reduction ~0.1 ms.
+    // Items of L >= 2 tiles, ~128 items per CU where there are enough tiles:
+    // short items keep the last round of the dispatch from idling most of
+    // the chip, while single-tile items cost more in per-item loads, action
+    // slots and their reduction than they save.  Measured: N = 262144
+    // (L = 32 / 4) 27.6 / 24.4 ms FP64, 20.9 / 18.0 ms mixed; N = 65536
+    // (L = 8 / 2 / 1) 1.91 / 1.75 / 1.77 ms FP64; N = 32768 steps 450 us at
+    // L = 2 against 495 us at L = 1.
     const long target = long(c->num_cus) * 128;
-    const long L = std::max(1L, tiles / target);
+    const long L = std::max(2L, tiles / target);
     std::vector<int4> items;
     for (long A = 0; A < nB; A++)
         for (long b0 = A; b0 < nB; b0 += L) items.push_back(make_int4(int(A), int(b0), int(std::min(nB, b0 + L)), 0));
@@ -173,7 +175,7 @@ static hipError_t sym_plan(nb_ctx* c, int K) {
     return hipSuccess;
 }
 
-// Which kernel evaluates the resident system: the symmetric one from 32768
+// Which kernel evaluates the resident system: the symmetric one from 16384
 // particles up (it does half the pair work) while its partial-sum buffer
 // (N^2 / 16 bytes at K = 8) fits: always up to 16 GiB, beyond that while the
 // growth takes at most 80 % of the free device memory (N = 1M needs 64 GiB of
@@ -194,7 +196,7 @@ static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     }
     if (c->kernel == NB_KERNEL_TILE) return false;
     if (c->kernel == NB_KERNEL_SYMMETRIC) return fits && n >= 2;
-    return fits && n >= 32768;
+    return fits && n >= 16384;   // measured crossover: N = 8192 tile 57 us / symmetric 74 us, 16384 196 / 164 us
 }
 
 // One force evaluation of the resident state; `out` says where the partial
