@@ -84,12 +84,14 @@ struct nb_ctx {
     hipEvent_t step_ev[2] = {nullptr, nullptr};
     nbl::Learned* learned = nullptr;
     int kernel = NB_KERNEL_AUTO;
-    // symmetric-kernel plan for the resident size
-    struct {
+    // symmetric-kernel plans for the resident size (one per K in use: the
+    // FP64 and mixed forms block differently)
+    struct SymPlan {
         size_t n = 0;
         int K = 0, nB = 0, L = 0, nitems = 0;
         DevBuf items;
-    } sym;
+    } sym[2];
+    int sym_next = 0;
 };
 
 namespace nbh {
@@ -141,9 +143,15 @@ static void timing_end(nb_ctx* c, long idx) {
 }
 
 // Symmetric plan: blocks of S = 64*K, items of L tiles along each row.
-static hipError_t sym_plan(nb_ctx* c, int K) {
-    auto& P = c->sym;
-    if (P.n == c->n && P.K == K) return hipSuccess;
+static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
+    for (const auto& Q : c->sym)
+        if (Q.n == c->n && Q.K == K) {
+            plan = &Q;
+            return hipSuccess;
+        }
+    auto& P = c->sym[c->sym_next];
+    c->sym_next ^= 1;
+    P.n = 0;   // invalid until built
     const long S = 64L * K;
     const long nB = (long(c->n) + S - 1) / S;
     const long tiles = nB * (nB + 1) / 2;
@@ -172,6 +180,7 @@ static hipError_t sym_plan(nb_ctx* c, int K) {
     P.nB = int(nB);
     P.L = int(L);
     P.nitems = int(items.size());
+    plan = &P;
     return hipSuccess;
 }
 
@@ -182,9 +191,11 @@ static hipError_t sym_plan(nb_ctx* c, int K) {
 // the 288 GB of an MI355X); the tile kernel otherwise.
 static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     const long n = long(c->n);
-    // 8 particles per lane at large N (fewer LDS reads and rotations per
-    // pair; the mixed loop is left unrolled once so 8 still fits the VGPRs).
-    K = (n >= 131072) ? 8 : 4;
+    // More particles per lane at large N (fewer LDS reads and rotations per
+    // pair): 8 for FP64, 6 for the mixed forms, whose t-loop is left unrolled
+    // once to fit the VGPRs (measured at N = 262144, mixed: K = 5 / 6 / 7 / 8
+    // 17.9 / 17.2 / 17.6 / 17.8 ms; FP64 K = 6 / 8 24.8 / 24.75 ms).
+    K = (n >= 131072) ? (kform == nbk::FORM_FP64 ? 8 : 6) : 4;
     const long S = 64L * K;
     const long nB = (n + S - 1) / S;
     const double need = double(nB) * nB * S * sizeof(P4);
@@ -205,9 +216,10 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     const long n = long(c->n);
     int K;
     if (use_symmetric(c, kform, K)) {
-        hipError_t e = sym_plan(c, K);
+        const nb_ctx::SymPlan* plan = nullptr;
+        hipError_t e = sym_plan(c, K, plan);
         if (e != hipSuccess) return e;
-        const auto& P = c->sym;
+        const auto& P = *plan;
         e = c->part.ensure(size_t(P.nB) * P.nB * 64 * K * sizeof(P4));
         if (e != hipSuccess) return e;
         nbk::SymLaunch L;
@@ -354,7 +366,7 @@ nb_status nb_destroy(nb_ctx* c) {
     nbl::destroy(c->learned);
     c->learned = nullptr;
     for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->rk,
-                      &c->sym.items})
+                      &c->sym[0].items, &c->sym[1].items})
         b->release();
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->step_ev)

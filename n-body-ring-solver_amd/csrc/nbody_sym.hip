@@ -289,7 +289,12 @@ static hipError_t launch_sym_t(const SymLaunch& L, hipStream_t st) {
 
 template <int FORM, bool EPS0>
 static hipError_t launch_sym_k(const SymLaunch& L, hipStream_t st) {
-    return (L.K == 8) ? launch_sym_t<FORM, EPS0, 8>(L, st) : launch_sym_t<FORM, EPS0, 4>(L, st);
+    switch (L.K) {
+    case 8: return launch_sym_t<FORM, EPS0, 8>(L, st);
+    case 6: return launch_sym_t<FORM, EPS0, 6>(L, st);
+    case 4: return launch_sym_t<FORM, EPS0, 4>(L, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int FORM>
