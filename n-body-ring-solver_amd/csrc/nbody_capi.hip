@@ -184,6 +184,13 @@ static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
     return hipSuccess;
 }
 
+// Partial-sum bytes of the symmetric kernel: nB*nB*S records of 32 B, plus
+// for the mixed forms nB*nB*S FP32 reaction slots of 16 B after them.
+static double sym_part_bytes(long nB, long S, int kform) {
+    const double slots = double(nB) * nB * S;
+    return slots * sizeof(P4) + (kform == nbk::FORM_FP64 ? 0.0 : slots * sizeof(float4));
+}
+
 // Which kernel evaluates the resident system: the symmetric one from 16384
 // particles up (it does half the pair work) while its partial-sum buffer
 // (N^2 / 16 bytes at K = 8) fits: always up to 16 GiB, beyond that while the
@@ -198,7 +205,7 @@ static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     K = (n >= 131072) ? (kform == nbk::FORM_FP64 ? 8 : 6) : 4;
     const long S = 64L * K;
     const long nB = (n + S - 1) / S;
-    const double need = double(nB) * nB * S * sizeof(P4);
+    const double need = sym_part_bytes(nB, S, kform);
     bool fits = need <= 16.0 * (1L << 30) || need <= double(c->part.bytes);
     if (!fits) {
         size_t free_b = 0, total_b = 0;
@@ -220,7 +227,8 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         hipError_t e = sym_plan(c, K, plan);
         if (e != hipSuccess) return e;
         const auto& P = *plan;
-        e = c->part.ensure(size_t(P.nB) * P.nB * 64 * K * sizeof(P4));
+        const long S = 64L * K;
+        e = c->part.ensure(size_t(sym_part_bytes(P.nB, S, kform)));
         if (e != hipSuccess) return e;
         nbk::SymLaunch L;
         L.form = kform;
@@ -232,8 +240,11 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         L.nB = P.nB;
         L.eps2 = eps * eps;
         L.part = (P4*)c->part.p;
+        L.rpart = (kform == nbk::FORM_FP64) ? nullptr
+                                             : (float4*)((char*)c->part.p + size_t(P.nB) * P.nB * S * sizeof(P4));
         out.sym = true;
         out.part = L.part;
+        out.rpart = L.rpart;
         out.nB = P.nB;
         out.S = 64 * K;
         out.L = P.L;
@@ -282,7 +293,7 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
 // acc = sum of the partials of `out`.
 hipError_t finish_accel(nb_ctx* c, const ForceOut& out) {
     const int n = int(c->n);
-    if (out.sym) return nbk::launch_sym_reduce(out.part, out.nB, out.S, out.L, n, (P4*)c->acc.p, c->stream);
+    if (out.sym) return nbk::launch_sym_reduce(out.part, out.rpart, out.nB, out.S, out.L, n, (P4*)c->acc.p, c->stream);
     if (out.S == 1) return hipSuccess;
     return nbk::launch_reduce(out.part, out.S, long(n), n, (P4*)c->acc.p, c->stream);
 }
@@ -293,7 +304,9 @@ hipError_t finish_kick_drift(nb_ctx* c, const ForceOut& out, double kdt, double 
     P4* pm = (P4*)c->pm.p;
     P4* vel = (P4*)c->vel.p;
     P4* acc = (P4*)c->acc.p;
-    if (out.sym) return nbk::launch_sym_kick_drift(out.part, out.nB, out.S, out.L, n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
+    if (out.sym)
+        return nbk::launch_sym_kick_drift(out.part, out.rpart, out.nB, out.S, out.L, n, pm, vel, acc, kdt, c1dt, c2dt,
+                                          c->stream);
     return nbk::launch_kick_drift(out.part, out.S, long(n), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
 }
 

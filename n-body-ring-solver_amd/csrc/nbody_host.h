@@ -37,6 +37,7 @@ struct DevBuf {
 struct ForceOut {
     bool sym = false;
     nbk::P4* part = nullptr;
+    const float4* rpart = nullptr;   // symmetric, mixed forms: FP32 reaction slots
     int S = 1;          // tile kernel: number of j-splits; symmetric: block size
     int nB = 0, L = 1;  // symmetric: blocks, tiles per work item
 };

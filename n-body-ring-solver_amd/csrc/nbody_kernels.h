@@ -46,12 +46,14 @@ struct SymLaunch {
     int nB = 0;
     double eps2 = 0.0;
     P4* part = nullptr;
+    float4* rpart = nullptr;   // mixed forms: FP32 reaction slots, nB*nB*S float4
 };
 
 hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st);
-hipError_t launch_sym_reduce(const P4* part, int nB, int S, int L, int n, P4* acc, hipStream_t st);
-hipError_t launch_sym_kick_drift(const P4* part, int nB, int S, int L, int n, P4* pm, P4* vel, P4* acc,
-                                 double kdt, double c1dt, double c2dt, hipStream_t st);
+hipError_t launch_sym_reduce(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* acc,
+                             hipStream_t st);
+hipError_t launch_sym_kick_drift(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* pm, P4* vel,
+                                 P4* acc, double kdt, double c1dt, double c2dt, hipStream_t st);
 hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st);
 hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
                              double kdt, double c1dt, double c2dt, hipStream_t st);

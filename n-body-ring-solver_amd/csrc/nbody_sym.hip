@@ -17,8 +17,10 @@
 // again.  Diagonal tiles (B == A) mask pairs with i >= j.
 //
 // Outputs are deterministic partial sums, no atomics: reactions of block B
-// from block A go to part[B][A] (slot A < B), the actions of item (A, b0, ..)
-// to part[A][b0] (slot >= A), diagonal reactions fold into the actions.
+// from block A go to slot [B][A] (A < B; FP64 records in part, or the FP32
+// sums of the mixed forms as float4 in rpart), the actions of item
+// (A, b0, ..) to part[A][b0] (slot >= A), diagonal reactions fold into the
+// actions.
 // k_sym_kick_drift / k_sym_reduce add a block's slots in a fixed order.
 #include <hip/hip_runtime.h>
 
@@ -118,7 +120,8 @@ template <int FORM, bool EPS0, int K>
 __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __restrict__ pm, int n,
                                                                    const int4* __restrict__ items, int nitems,
                                                                    int nB, double eps2, double c15,
-                                                                   double c1875, P4* __restrict__ part) {
+                                                                   double c1875, P4* __restrict__ part,
+                                                                   float4* __restrict__ rpart) {
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR);
     constexpr int S = 64 * K;
@@ -220,7 +223,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                         ax[k] += bx; ay[k] += by; az[k] += bz;
                     }
             } else {
-                part[((long)B * nB + A) * S + c * 64 + lane] = P4{bx, by, bz, 0.0};
+                // reactions of the mixed forms are FP32 sums: stored as such
+                // (16 B), the FP64 form's as records (32 B)
+                const long slot = ((long)B * nB + A) * S + c * 64 + lane;
+                if constexpr (MIXED) rpart[slot] = make_float4((float)bx, (float)by, (float)bz, 0.0f);
+                else part[slot] = P4{bx, by, bz, 0.0};
             }
             wave_sync();
         }
@@ -230,39 +237,83 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
 }
 
 // a_i for i in block X = sum of reaction slots Y < X, then action slots
-// b0 = X, X+L, ... < nB, in that order.
-__device__ __forceinline__ void sym_sum(const P4* __restrict__ part, int nB, int S, int L, int i, double& ax,
-                                        double& ay, double& az) {
+// b0 = X, X+L, ... < nB, in that order.  F32R: the reactions are the FP32
+// records of rpart (the mixed forms), else P4 records of part.
+// Slots are loaded eight at a time (the loads in flight together) and added
+// one by one in slot order.
+template <bool F32R>
+__device__ __forceinline__ void sym_sum(const P4* __restrict__ part, const float4* __restrict__ rpart, int nB,
+                                        int S, int L, int i, double& ax, double& ay, double& az) {
+    constexpr int U = 8;
     const int X = i / S, r = i - X * S;
     ax = ay = az = 0.0;
-    const P4* row = part + (long)X * nB * S + r;
-    for (int Y = 0; Y < X; Y++) {
-        const P4 p = row[(long)Y * S];
-        ax += p.x; ay += p.y; az += p.z;
+    const long row = (long)X * nB * S + r;
+    int Y = 0;
+    if constexpr (F32R) {
+        for (; Y + U <= X; Y += U) {
+            float4 q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = rpart[row + (long)(Y + u) * S];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                ax += (double)q[u].x; ay += (double)q[u].y; az += (double)q[u].z;
+            }
+        }
+        for (; Y < X; Y++) {
+            const float4 q = rpart[row + (long)Y * S];
+            ax += (double)q.x; ay += (double)q.y; az += (double)q.z;
+        }
+    } else {
+        for (; Y + U <= X; Y += U) {
+            P4 q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = part[row + (long)(Y + u) * S];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                ax += q[u].x; ay += q[u].y; az += q[u].z;
+            }
+        }
+        for (; Y < X; Y++) {
+            const P4 p = part[row + (long)Y * S];
+            ax += p.x; ay += p.y; az += p.z;
+        }
     }
-    for (int Y = X; Y < nB; Y += L) {
-        const P4 p = row[(long)Y * S];
+    Y = X;
+    for (; Y + (U - 1) * L < nB; Y += U * L) {
+        P4 q[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) q[u] = part[row + (long)(Y + u * L) * S];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ax += q[u].x; ay += q[u].y; az += q[u].z;
+        }
+    }
+    for (; Y < nB; Y += L) {
+        const P4 p = part[row + (long)Y * S];
         ax += p.x; ay += p.y; az += p.z;
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sym_reduce(const P4* __restrict__ part, int nB, int S, int L, int n,
-                                                      P4* __restrict__ acc) {
+template <bool F32R>
+__global__ __launch_bounds__(BLOCK) void k_sym_reduce(const P4* __restrict__ part, const float4* __restrict__ rpart,
+                                                      int nB, int S, int L, int n, P4* __restrict__ acc) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double ax, ay, az;
-    sym_sum(part, nB, S, L, i, ax, ay, az);
+    sym_sum<F32R>(part, rpart, nB, S, L, i, ax, ay, az);
     acc[i] = P4{ax, ay, az, 0.0};
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sym_kick_drift(const P4* __restrict__ part, int nB, int S, int L, int n,
-                                                          P4* __restrict__ pm, P4* __restrict__ vel,
+template <bool F32R>
+__global__ __launch_bounds__(BLOCK) void k_sym_kick_drift(const P4* __restrict__ part,
+                                                          const float4* __restrict__ rpart, int nB, int S, int L,
+                                                          int n, P4* __restrict__ pm, P4* __restrict__ vel,
                                                           P4* __restrict__ acc, double kdt, double c1dt,
                                                           double c2dt) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double ax, ay, az;
-    sym_sum(part, nB, S, L, i, ax, ay, az);
+    sym_sum<F32R>(part, rpart, nB, S, L, i, ax, ay, az);
     acc[i] = P4{ax, ay, az, 0.0};
     P4 v = vel[i];
     v.x = __builtin_fma(kdt, ax, v.x);
@@ -283,7 +334,7 @@ template <int FORM, bool EPS0, int K>
 static hipError_t launch_sym_t(const SymLaunch& L, hipStream_t st) {
     const int groups = (L.nitems + kWavesPerGroup - 1) / kWavesPerGroup;
     hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm, L.n,
-                       L.items, L.nitems, L.nB, L.eps2, 1.5, 1.875, L.part);
+                       L.items, L.nitems, L.nB, L.eps2, 1.5, 1.875, L.part, L.rpart);
     return hipGetLastError();
 }
 
@@ -304,6 +355,7 @@ static hipError_t launch_sym_eps(const SymLaunch& L, hipStream_t st) {
 
 hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st) {
     if (L.nitems <= 0) return hipSuccess;
+    if (L.form != FORM_FP64 && !L.rpart) return hipErrorInvalidValue;   // mixed reactions need their FP32 slots
     switch (L.form) {
     case FORM_FP64: return launch_sym_eps<FORM_FP64>(L, st);
     case FORM_MIXED: return launch_sym_eps<FORM_MIXED>(L, st);
@@ -312,17 +364,27 @@ hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st) {
     }
 }
 
-hipError_t launch_sym_reduce(const P4* part, int nB, int S, int L, int n, P4* acc, hipStream_t st) {
+hipError_t launch_sym_reduce(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* acc,
+                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sym_reduce, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, nB, S, L, n, acc);
+    if (rpart)
+        hipLaunchKernelGGL(k_sym_reduce<true>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB, S,
+                           L, n, acc);
+    else
+        hipLaunchKernelGGL(k_sym_reduce<false>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB, S,
+                           L, n, acc);
     return hipGetLastError();
 }
 
-hipError_t launch_sym_kick_drift(const P4* part, int nB, int S, int L, int n, P4* pm, P4* vel, P4* acc,
-                                 double kdt, double c1dt, double c2dt, hipStream_t st) {
+hipError_t launch_sym_kick_drift(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* pm, P4* vel,
+                                 P4* acc, double kdt, double c1dt, double c2dt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sym_kick_drift, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, nB, S, L, n, pm,
-                       vel, acc, kdt, c1dt, c2dt);
+    if (rpart)
+        hipLaunchKernelGGL(k_sym_kick_drift<true>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB,
+                           S, L, n, pm, vel, acc, kdt, c1dt, c2dt);
+    else
+        hipLaunchKernelGGL(k_sym_kick_drift<false>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB,
+                           S, L, n, pm, vel, acc, kdt, c1dt, c2dt);
     return hipGetLastError();
 }
 
