@@ -185,6 +185,37 @@ def roofline_entry(name, kernel_ms, n, peak, traffic=None, pairs="N^2 ordered pa
             "algorithmic_bytes_per_launch": 64 * n}
 
 
+def ring_share(eps, seed, ni=131072, nj=1048576, j_block=131072, reps=3):
+    """One rank's force share of an 8-rank ring over a Plummer N = 1048576: the
+    first 131072 particles against all of them, the j-set visited in 8 blocks
+    as the ring delivers them (nb_accel_block_device), HIP events around every
+    launch on the solver's stream."""
+    import torch
+    s = nbody_amd.Solver(0)
+    m, pos, _, _ = nbody_amd.init_system("plummer", nj, seed, [1.0, 20.0])
+    pj = torch.tensor(np.stack([pos[0], pos[1], pos[2], m], axis=1), dtype=torch.float64, device="cuda")
+    pi = pj[:ni].contiguous()
+    outs, res = {}, {}
+    for form in ("fp64", "mixed"):
+        out = torch.empty((ni, 4), dtype=torch.float64, device="cuda")
+        s.accel_block_device(form, eps, ni, pi.data_ptr(), nj, pj.data_ptr(), j_block, out.data_ptr())
+        s.synchronize()
+        s.timing(True)
+        s.timing_reset()
+        for _ in range(reps):
+            s.accel_block_device(form, eps, ni, pi.data_ptr(), nj, pj.data_ptr(), j_block, out.data_ptr())
+        s.synchronize()
+        ms, launches, _ = s.timing_read()
+        s.timing(False)
+        outs[form] = out[:, :3].cpu().numpy().T
+        res[form] = {"ms_per_share": ms / reps, "launches_per_share": launches // reps,
+                     "tpairs_per_s": ni * nj / (ms / reps) / 1e9}
+    res["mixed"]["max_rel_force_err_vs_fp64"] = rel_err(outs["mixed"], outs["fp64"])
+    print(json.dumps({"workload": f"ring share: Plummer N={nj}, {ni} i-particles x {nj} j-particles in "
+                                  f"blocks of {j_block}, eps={eps}", "forms": res}), flush=True)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,7 +230,14 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (a known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--ring-share", action="store_true",
+                    help="only time one rank's share of the reference's ring (131072 i-particles against "
+                         "1048576 j-particles in blocks of 131072) for fp64 and mixed, print one JSON line "
+                         "and exit")
     args = ap.parse_args()
+    if args.ring_share:
+        ring_share(args.eps, args.seed)
+        return
     if args.copy_probe:
         import torch
         src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
@@ -261,7 +299,7 @@ def main():
                                            forms["fp64"]["force_kernel_ms_avg"], n, FP64_PEAK_TFLOPS,
                                            profile_traffic("k_accel_sym_plummer_n262144_pmc.md"))
     if forms.get("mixed", {}).get("exact_force_evals"):
-        rooflines["mixed"] = roofline_entry("k_accel_sym<MIXED_NR, K=8> (symmetric, FP32 pair math on FP64 "
+        rooflines["mixed"] = roofline_entry("k_accel_sym<MIXED_NR, K=6> (symmetric, FP32 pair math on FP64 "
                                             "differences; against the FP32 peak, which counts packed FP32)",
                                             forms["mixed"]["force_kernel_ms_avg"], n, FP32_PEAK_TFLOPS,
                                             profile_traffic("k_accel_sym_plummer_n262144_pmc.md",

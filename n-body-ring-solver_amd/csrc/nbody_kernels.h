@@ -20,7 +20,7 @@ enum Form { FORM_FP64 = 0, FORM_MIXED = 1, FORM_MIXED_NR = 2 };
 // blockIdx.y); chunk s writes out[(split_base + s) * out_stride + i].
 struct AccelLaunch {
     int form = FORM_FP64;
-    int ipt = 1;              // i-particles per lane (1 or 2)
+    int ipt = 1;              // i-particles per lane (1 or 4)
     const P4* pi = nullptr;
     int ni = 0;
     const P4* pj = nullptr;

@@ -453,7 +453,11 @@ static hipError_t launch_accel_t(const AccelLaunch& L, hipStream_t st) {
 
 template <int FORM, bool EPS0>
 static hipError_t launch_accel_ipt(const AccelLaunch& L, hipStream_t st) {
-    return (L.ipt == 2) ? launch_accel_t<FORM, EPS0, 2>(L, st) : launch_accel_t<FORM, EPS0, 1>(L, st);
+    switch (L.ipt) {
+    case 4: return launch_accel_t<FORM, EPS0, 4>(L, st);
+    case 1: return launch_accel_t<FORM, EPS0, 1>(L, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int FORM>

@@ -237,3 +237,16 @@ def test_symmetric_beyond_16_gib_of_partials(solver):
     solver.set_kernel("auto")
     assert rel_err(out["auto"], out["tile"]) < 2e-5
     assert timed["auto"] < 0.8 * timed["tile"], timed
+
+
+def test_accel_block_four_rows_per_lane(solver, orc):
+    # 131372 i-rows: on 256 CUs the tile kernel holds 4 i-particles per lane
+    # (from 512 rows per CU) and the last workgroup is partly empty.
+    m, pos, _, _ = nbody_amd.init_system("plummer", 131372, 21)
+    pi = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    pj = pi[::64].copy()
+    ref = orc.accel_block(pi, pj, 0.005)
+    acc = solver.accel_block(pi, pj, "fp64", 0.005, j_block=1024)
+    assert rel_err(acc.T, ref.T) < FP64_TOL
+    accm = solver.accel_block(pi, pj, "mixed", 0.005)
+    assert rel_err(accm.T, ref.T) < MIXED_TOL
