@@ -144,6 +144,19 @@ static void timing_end(nb_ctx* c, long idx) {
 }
 
 // Symmetric plan: blocks of S = 64*K, items of L tiles along each row.
+// Tiles per work item of the symmetric kernel: L >= 2, ~128 items per CU
+// where there are enough tiles.  Short items keep the last round of the
+// dispatch from idling most of the chip, while single-tile items cost more
+// in per-item loads, action slots and their reduction than they save.
+// Measured: N = 262144 (L = 32 / 4) 27.6 / 24.4 ms FP64, 20.9 / 18.0 ms
+// mixed; N = 65536 (L = 8 / 2 / 1) 1.91 / 1.75 / 1.77 ms FP64; N = 32768
+// steps 450 us at L = 2 against 495 us at L = 1.
+static long sym_item_tiles(const nb_ctx* c, long nB) {
+    const long tiles = nB * (nB + 1) / 2;
+    const long target = long(c->num_cus) * 128;
+    return std::max(2L, tiles / target);
+}
+
 static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
     for (const auto& Q : c->sym)
         if (Q.n == c->n && Q.K == K) {
@@ -155,19 +168,12 @@ static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
     P.n = 0;   // invalid until built
     const long S = 64L * K;
     const long nB = (long(c->n) + S - 1) / S;
-    const long tiles = nB * (nB + 1) / 2;
-    // Items of L >= 2 tiles, ~128 items per CU where there are enough tiles:
-    // short items keep the last round of the dispatch from idling most of
-    // the chip, while single-tile items cost more in per-item loads, action
-    // slots and their reduction than they save.  Measured: N = 262144
-    // (L = 32 / 4) 27.6 / 24.4 ms FP64, 20.9 / 18.0 ms mixed; N = 65536
-    // (L = 8 / 2 / 1) 1.91 / 1.75 / 1.77 ms FP64; N = 32768 steps 450 us at
-    // L = 2 against 495 us at L = 1.
-    const long target = long(c->num_cus) * 128;
-    const long L = std::max(2L, tiles / target);
+    const long L = sym_item_tiles(c, nB);
+    const long W = (nB - 1) / L + 1;
     std::vector<int4> items;
     for (long A = 0; A < nB; A++)
-        for (long b0 = A; b0 < nB; b0 += L) items.push_back(make_int4(int(A), int(b0), int(std::min(nB, b0 + L)), 0));
+        for (long b0 = A; b0 < nB; b0 += L)
+            items.push_back(make_int4(int(A), int(b0), int(std::min(nB, b0 + L)), int(A * W + (b0 - A) / L)));
     // longest items first so the short row tails fill in at the end
     std::stable_sort(items.begin(), items.end(), [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
     hipError_t e = P.items.ensure(items.size() * sizeof(int4));
@@ -185,18 +191,26 @@ static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
     return hipSuccess;
 }
 
-// Partial-sum bytes of the symmetric kernel: nB*nB*S records of 32 B, plus
-// for the mixed forms nB*nB*S FP32 reaction slots of 16 B after them.
-static double sym_part_bytes(long nB, long S, int kform) {
-    const double slots = double(nB) * nB * S;
-    return slots * sizeof(P4) + (kform == nbk::FORM_FP64 ? 0.0 : slots * sizeof(float4));
+// Partial sums of the symmetric kernel: the action slots (nB*W slots of S
+// records of 32 B, W = (nB - 1)/L + 1) and after them the reaction slots
+// (nB(nB-1)/2 slots of S records: 32 B for FP64, 16 B FP32 for the mixed
+// forms).  N^2/32 + 32 N W bytes for FP64 at K = 8 (34 GB at N = 1048576),
+// N^2/48 + 32 N W for the mixed forms at K = 6.
+struct SymRegions {
+    size_t act_bytes, react_bytes;
+};
+static SymRegions sym_regions(long nB, long S, long L, int kform) {
+    const long W = (nB - 1) / L + 1;
+    const size_t rec = (kform == nbk::FORM_FP64) ? sizeof(P4) : sizeof(float4);
+    return {size_t(nB) * size_t(W) * size_t(S) * sizeof(P4), size_t(nB) * size_t(nB - 1) / 2 * size_t(S) * rec};
 }
 
 // Which kernel evaluates the resident system: the symmetric one from 16384
 // particles up (it does half the pair work) while its partial-sum buffer
-// (N^2 / 16 bytes at K = 8) fits: always up to 16 GiB, beyond that while the
-// growth takes at most 80 % of the free device memory (N = 1M needs 64 GiB of
-// the 288 GB of an MI355X); the tile kernel otherwise.
+// fits: always up to 16 GiB, beyond that while the growth takes at most 80 %
+// of the free device memory (N = 1M needs 35 GB FP64 / 24 GB mixed of the
+// 288 GB of an MI355X, so one device runs it to N ~ 2.6M FP64); the tile
+// kernel otherwise.
 static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     const long n = long(c->n);
     // More particles per lane at large N (fewer LDS reads and rotations per
@@ -206,7 +220,8 @@ static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     K = (n >= 131072) ? (kform == nbk::FORM_FP64 ? 8 : 6) : 4;
     const long S = 64L * K;
     const long nB = (n + S - 1) / S;
-    const double need = sym_part_bytes(nB, S, kform);
+    const SymRegions R = sym_regions(nB, S, sym_item_tiles(c, nB), kform);
+    const double need = double(R.act_bytes) + double(R.react_bytes);
     bool fits = need <= 16.0 * (1L << 30) || need <= double(c->part.bytes);
     if (!fits) {
         size_t free_b = 0, total_b = 0;
@@ -229,8 +244,10 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         if (e != hipSuccess) return e;
         const auto& P = *plan;
         const long S = 64L * K;
-        e = c->part.ensure(size_t(sym_part_bytes(P.nB, S, kform)));
+        const SymRegions R = sym_regions(P.nB, S, P.L, kform);
+        e = c->part.ensure(R.act_bytes + R.react_bytes);
         if (e != hipSuccess) return e;
+        char* const react = (char*)c->part.p + R.act_bytes;
         nbk::SymLaunch L;
         L.form = kform;
         L.K = K;
@@ -241,11 +258,12 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         L.nB = P.nB;
         L.eps2 = eps * eps;
         L.part = (P4*)c->part.p;
-        L.rpart = (kform == nbk::FORM_FP64) ? nullptr
-                                             : (float4*)((char*)c->part.p + size_t(P.nB) * P.nB * S * sizeof(P4));
+        if (kform == nbk::FORM_FP64) L.react64 = (P4*)react;
+        else L.react32 = (float4*)react;
         out.sym = true;
         out.part = L.part;
-        out.rpart = L.rpart;
+        out.react64 = L.react64;
+        out.react32 = L.react32;
         out.nB = P.nB;
         out.S = 64 * K;
         out.L = P.L;
@@ -291,10 +309,21 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     return e;
 }
 
+static nbk::SymSums sym_sums(const ForceOut& out) {
+    nbk::SymSums P;
+    P.part = out.part;
+    P.react64 = out.react64;
+    P.react32 = out.react32;
+    P.nB = out.nB;
+    P.S = out.S;
+    P.L = out.L;
+    return P;
+}
+
 // acc = sum of the partials of `out`.
 hipError_t finish_accel(nb_ctx* c, const ForceOut& out) {
     const int n = int(c->n);
-    if (out.sym) return nbk::launch_sym_reduce(out.part, out.rpart, out.nB, out.S, out.L, n, (P4*)c->acc.p, c->stream);
+    if (out.sym) return nbk::launch_sym_reduce(sym_sums(out), n, (P4*)c->acc.p, c->stream);
     if (out.S == 1) return hipSuccess;
     return nbk::launch_reduce(out.part, out.S, long(n), n, (P4*)c->acc.p, c->stream);
 }
@@ -306,8 +335,7 @@ hipError_t finish_kick_drift(nb_ctx* c, const ForceOut& out, double kdt, double 
     P4* vel = (P4*)c->vel.p;
     P4* acc = (P4*)c->acc.p;
     if (out.sym)
-        return nbk::launch_sym_kick_drift(out.part, out.rpart, out.nB, out.S, out.L, n, pm, vel, acc, kdt, c1dt, c2dt,
-                                          c->stream);
+        return nbk::launch_sym_kick_drift(sym_sums(out), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
     return nbk::launch_kick_drift(out.part, out.S, long(n), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
 }
 

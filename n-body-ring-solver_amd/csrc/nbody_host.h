@@ -36,8 +36,9 @@ struct DevBuf {
 // Where one force evaluation left its partial sums.
 struct ForceOut {
     bool sym = false;
-    nbk::P4* part = nullptr;
-    const float4* rpart = nullptr;   // symmetric, mixed forms: FP32 reaction slots
+    nbk::P4* part = nullptr;              // tile: split partials; symmetric: action slots
+    const nbk::P4* react64 = nullptr;     // symmetric, FP64 form: reaction slots
+    const float4* react32 = nullptr;      // symmetric, mixed forms: FP32 reaction slots
     int S = 1;          // tile kernel: number of j-splits; symmetric: block size
     int nB = 0, L = 1;  // symmetric: blocks, tiles per work item
 };

@@ -34,8 +34,8 @@ struct AccelLaunch {
 hipError_t launch_accel(const AccelLaunch& L, hipStream_t st);
 
 // Symmetric kernel (nbody_sym.hip): blocks of S = 64*K particles, work items
-// (A, b0, b1) = one wave sweeping j-blocks [b0, b1) against i-block A;
-// part holds nB*nB*S records (slot layout described in nbody_sym.hip).
+// (A, b0, b1, action slot) = one wave sweeping j-blocks [b0, b1) against
+// i-block A (slot layout described in nbody_sym.hip).
 struct SymLaunch {
     int form = FORM_FP64;
     int K = 8;
@@ -45,15 +45,24 @@ struct SymLaunch {
     int nitems = 0;
     int nB = 0;
     double eps2 = 0.0;
-    P4* part = nullptr;
-    float4* rpart = nullptr;   // mixed forms: FP32 reaction slots, nB*nB*S float4
+    P4* part = nullptr;          // action slots: nB*W slots of S records, W = (nB - 1)/L + 1
+    P4* react64 = nullptr;       // FP64 form: reaction slots, nB(nB-1)/2 slots of S records
+    float4* react32 = nullptr;   // mixed forms: FP32 reaction slots, nB(nB-1)/2 slots of S float4
 };
 
 hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st);
-hipError_t launch_sym_reduce(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* acc,
-                             hipStream_t st);
-hipError_t launch_sym_kick_drift(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* pm, P4* vel,
-                                 P4* acc, double kdt, double c1dt, double c2dt, hipStream_t st);
+
+// The partial sums one symmetric evaluation leaves (exactly one of react64 /
+// react32 set) and the item length L they were cut with.
+struct SymSums {
+    const P4* part = nullptr;
+    const P4* react64 = nullptr;
+    const float4* react32 = nullptr;
+    int nB = 0, S = 0, L = 1;
+};
+hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st);
+hipError_t launch_sym_kick_drift(const SymSums& P, int n, P4* pm, P4* vel, P4* acc, double kdt, double c1dt,
+                                 double c2dt, hipStream_t st);
 hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st);
 hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
                              double kdt, double c1dt, double c2dt, hipStream_t st);

@@ -16,11 +16,13 @@
 // lane l receives lane l+1), so after 64 steps each sits in its home lane
 // again.  Diagonal tiles (B == A) mask pairs with i >= j.
 //
-// Outputs are deterministic partial sums, no atomics: reactions of block B
-// from block A go to slot [B][A] (A < B; FP64 records in part, or the FP32
-// sums of the mixed forms as float4 in rpart), the actions of item
-// (A, b0, ..) to part[A][b0] (slot >= A), diagonal reactions fold into the
-// actions.
+// Outputs are deterministic partial sums, no atomics.  Reactions of block B
+// from block A (A < B) go to the triangular slot B(B-1)/2 + A of the reaction
+// region (FP64 records in react64, or the FP32 sums of the mixed forms as
+// float4 in react32); the actions of item (A, b0, ..) go to action slot
+// A*W + (b0 - A)/L of part, W = (nB - 1)/L + 1 slots per block (the item
+// carries that index); diagonal reactions fold into the actions.  Each slot
+// holds S records, one per particle of the block.
 // k_sym_kick_drift / k_sym_reduce add a block's slots in a fixed order.
 #include <hip/hip_runtime.h>
 
@@ -121,7 +123,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                                                                    const int4* __restrict__ items, int nitems,
                                                                    int nB, double eps2, double c15,
                                                                    double c1875, P4* __restrict__ part,
-                                                                   float4* __restrict__ rpart) {
+                                                                   P4* __restrict__ react64,
+                                                                   float4* __restrict__ react32) {
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR);
     constexpr int S = 64 * K;
@@ -132,7 +135,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
     const int item = blockIdx.x * kWavesPerGroup + wave;
     if (item >= nitems) return;   // the whole wave leaves; no workgroup barrier is used below
     const int4 it = items[item];
-    const int A = it.x, b0 = it.y, b1 = it.z;
+    const int A = it.x, b0 = it.y, b1 = it.z, act = it.w;
     P4* __restrict__ lds = jt[wave];
 
     double xi[K], yi[K], zi[K], mi[K], ax[K], ay[K], az[K];
@@ -225,95 +228,101 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
             } else {
                 // reactions of the mixed forms are FP32 sums: stored as such
                 // (16 B), the FP64 form's as records (32 B)
-                const long slot = ((long)B * nB + A) * S + c * 64 + lane;
-                if constexpr (MIXED) rpart[slot] = make_float4((float)bx, (float)by, (float)bz, 0.0f);
-                else part[slot] = P4{bx, by, bz, 0.0};
+                const long slot = ((long)B * (B - 1) / 2 + A) * S + c * 64 + lane;
+                if constexpr (MIXED) react32[slot] = make_float4((float)bx, (float)by, (float)bz, 0.0f);
+                else react64[slot] = P4{bx, by, bz, 0.0};
             }
             wave_sync();
         }
     }
 #pragma unroll
-    for (int k = 0; k < K; k++) part[((long)A * nB + b0) * S + k * 64 + lane] = P4{ax[k], ay[k], az[k], 0.0};
+    for (int k = 0; k < K; k++) part[(long)act * S + k * 64 + lane] = P4{ax[k], ay[k], az[k], 0.0};
 }
 
-// a_i for i in block X = sum of reaction slots Y < X, then action slots
-// b0 = X, X+L, ... < nB, in that order.  F32R: the reactions are the FP32
-// records of rpart (the mixed forms), else P4 records of part.
-// Slots are loaded eight at a time (the loads in flight together) and added
-// one by one in slot order.
+// a_i for i in block X = sum of the reactions from blocks Y = 0 .. X-1, then
+// of the actions of the items (X, b0 = X, X+L, ..) of row X, in that order.
+// F32R: the reactions are the FP32 records of react32 (the mixed forms), else
+// the P4 records of react64.  Slots are loaded eight at a time (the loads in
+// flight together) and added one by one in slot order.
 template <bool F32R>
-__device__ __forceinline__ void sym_sum(const P4* __restrict__ part, const float4* __restrict__ rpart, int nB,
-                                        int S, int L, int i, double& ax, double& ay, double& az) {
+__device__ __forceinline__ void sym_sum(const P4* __restrict__ part, const P4* __restrict__ react64,
+                                        const float4* __restrict__ react32, int nB, int S, int L, int i,
+                                        double& ax, double& ay, double& az) {
     constexpr int U = 8;
     const int X = i / S, r = i - X * S;
     ax = ay = az = 0.0;
-    const long row = (long)X * nB * S + r;
+    const long rrow = (long)X * (X - 1) / 2 * S + r;
     int Y = 0;
     if constexpr (F32R) {
         for (; Y + U <= X; Y += U) {
             float4 q[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) q[u] = rpart[row + (long)(Y + u) * S];
+            for (int u = 0; u < U; u++) q[u] = react32[rrow + (long)(Y + u) * S];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 ax += (double)q[u].x; ay += (double)q[u].y; az += (double)q[u].z;
             }
         }
         for (; Y < X; Y++) {
-            const float4 q = rpart[row + (long)Y * S];
+            const float4 q = react32[rrow + (long)Y * S];
             ax += (double)q.x; ay += (double)q.y; az += (double)q.z;
         }
     } else {
         for (; Y + U <= X; Y += U) {
             P4 q[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) q[u] = part[row + (long)(Y + u) * S];
+            for (int u = 0; u < U; u++) q[u] = react64[rrow + (long)(Y + u) * S];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 ax += q[u].x; ay += q[u].y; az += q[u].z;
             }
         }
         for (; Y < X; Y++) {
-            const P4 p = part[row + (long)Y * S];
+            const P4 p = react64[rrow + (long)Y * S];
             ax += p.x; ay += p.y; az += p.z;
         }
     }
-    Y = X;
-    for (; Y + (U - 1) * L < nB; Y += U * L) {
+    const int W = (nB - 1) / L + 1;
+    const int cnt = (nB - 1 - X) / L + 1;   // items of row X
+    const long arow = (long)X * W * S + r;
+    int j = 0;
+    for (; j + U <= cnt; j += U) {
         P4 q[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) q[u] = part[row + (long)(Y + u * L) * S];
+        for (int u = 0; u < U; u++) q[u] = part[arow + (long)(j + u) * S];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             ax += q[u].x; ay += q[u].y; az += q[u].z;
         }
     }
-    for (; Y < nB; Y += L) {
-        const P4 p = part[row + (long)Y * S];
+    for (; j < cnt; j++) {
+        const P4 p = part[arow + (long)j * S];
         ax += p.x; ay += p.y; az += p.z;
     }
 }
 
 template <bool F32R>
-__global__ __launch_bounds__(BLOCK) void k_sym_reduce(const P4* __restrict__ part, const float4* __restrict__ rpart,
-                                                      int nB, int S, int L, int n, P4* __restrict__ acc) {
+__global__ __launch_bounds__(BLOCK) void k_sym_reduce(const P4* __restrict__ part, const P4* __restrict__ react64,
+                                                      const float4* __restrict__ react32, int nB, int S, int L,
+                                                      int n, P4* __restrict__ acc) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double ax, ay, az;
-    sym_sum<F32R>(part, rpart, nB, S, L, i, ax, ay, az);
+    sym_sum<F32R>(part, react64, react32, nB, S, L, i, ax, ay, az);
     acc[i] = P4{ax, ay, az, 0.0};
 }
 
 template <bool F32R>
 __global__ __launch_bounds__(BLOCK) void k_sym_kick_drift(const P4* __restrict__ part,
-                                                          const float4* __restrict__ rpart, int nB, int S, int L,
+                                                          const P4* __restrict__ react64,
+                                                          const float4* __restrict__ react32, int nB, int S, int L,
                                                           int n, P4* __restrict__ pm, P4* __restrict__ vel,
                                                           P4* __restrict__ acc, double kdt, double c1dt,
                                                           double c2dt) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double ax, ay, az;
-    sym_sum<F32R>(part, rpart, nB, S, L, i, ax, ay, az);
+    sym_sum<F32R>(part, react64, react32, nB, S, L, i, ax, ay, az);
     acc[i] = P4{ax, ay, az, 0.0};
     P4 v = vel[i];
     v.x = __builtin_fma(kdt, ax, v.x);
@@ -334,7 +343,7 @@ template <int FORM, bool EPS0, int K>
 static hipError_t launch_sym_t(const SymLaunch& L, hipStream_t st) {
     const int groups = (L.nitems + kWavesPerGroup - 1) / kWavesPerGroup;
     hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm, L.n,
-                       L.items, L.nitems, L.nB, L.eps2, 1.5, 1.875, L.part, L.rpart);
+                       L.items, L.nitems, L.nB, L.eps2, 1.5, 1.875, L.part, L.react64, L.react32);
     return hipGetLastError();
 }
 
@@ -355,7 +364,8 @@ static hipError_t launch_sym_eps(const SymLaunch& L, hipStream_t st) {
 
 hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st) {
     if (L.nitems <= 0) return hipSuccess;
-    if (L.form != FORM_FP64 && !L.rpart) return hipErrorInvalidValue;   // mixed reactions need their FP32 slots
+    // FP64 reactions need their record slots, the mixed forms' their FP32 slots
+    if (L.form == FORM_FP64 ? !L.react64 : !L.react32) return hipErrorInvalidValue;
     switch (L.form) {
     case FORM_FP64: return launch_sym_eps<FORM_FP64>(L, st);
     case FORM_MIXED: return launch_sym_eps<FORM_MIXED>(L, st);
@@ -364,27 +374,30 @@ hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st) {
     }
 }
 
-hipError_t launch_sym_reduce(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* acc,
-                             hipStream_t st) {
+hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (rpart)
-        hipLaunchKernelGGL(k_sym_reduce<true>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB, S,
-                           L, n, acc);
+    if (!P.react64 == !P.react32) return hipErrorInvalidValue;   // exactly one reaction region
+    const dim3 grid((n + BLOCK - 1) / BLOCK);
+    if (P.react32)
+        hipLaunchKernelGGL(k_sym_reduce<true>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB, P.S,
+                           P.L, n, acc);
     else
-        hipLaunchKernelGGL(k_sym_reduce<false>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB, S,
-                           L, n, acc);
+        hipLaunchKernelGGL(k_sym_reduce<false>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB, P.S,
+                           P.L, n, acc);
     return hipGetLastError();
 }
 
-hipError_t launch_sym_kick_drift(const P4* part, const float4* rpart, int nB, int S, int L, int n, P4* pm, P4* vel,
-                                 P4* acc, double kdt, double c1dt, double c2dt, hipStream_t st) {
+hipError_t launch_sym_kick_drift(const SymSums& P, int n, P4* pm, P4* vel, P4* acc, double kdt, double c1dt,
+                                 double c2dt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (rpart)
-        hipLaunchKernelGGL(k_sym_kick_drift<true>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB,
-                           S, L, n, pm, vel, acc, kdt, c1dt, c2dt);
+    if (!P.react64 == !P.react32) return hipErrorInvalidValue;
+    const dim3 grid((n + BLOCK - 1) / BLOCK);
+    if (P.react32)
+        hipLaunchKernelGGL(k_sym_kick_drift<true>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB,
+                           P.S, P.L, n, pm, vel, acc, kdt, c1dt, c2dt);
     else
-        hipLaunchKernelGGL(k_sym_kick_drift<false>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, part, rpart, nB,
-                           S, L, n, pm, vel, acc, kdt, c1dt, c2dt);
+        hipLaunchKernelGGL(k_sym_kick_drift<false>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB,
+                           P.S, P.L, n, pm, vel, acc, kdt, c1dt, c2dt);
     return hipGetLastError();
 }
 

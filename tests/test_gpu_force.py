@@ -219,24 +219,26 @@ def test_symmetric_coincident_eps0(solver, orc):
 
 
 def test_symmetric_beyond_16_gib_of_partials(solver):
-    """N = 600000 needs 21 GiB of partial sums at K = 8: the symmetric kernel
-    still runs (the buffer is sized against the free HBM) and agrees with the
-    tile kernel; its time shows it did half the pair work."""
-    n = 600000
+    """N = 1048576 needs 35 GB (FP64) / 24 GB (mixed) of partial sums: the
+    symmetric kernel still runs (the buffer is sized against the free HBM)
+    and agrees with the tile kernel; its time shows it did half the pair
+    work."""
+    n = 1048576
     m, pos, _, _ = nbody_amd.init_system("plummer", n, 31)
     solver.set_state(m, pos)
-    timed = {}
-    out = {}
-    for kernel in ("tile", "auto"):
-        solver.set_kernel(kernel)
-        solver.timing(True)
-        solver.timing_reset()
-        out[kernel] = solver.accel("mixed", 0.005)
-        timed[kernel] = solver.timing_read()[0]
-        solver.timing(False)
-    solver.set_kernel("auto")
-    assert rel_err(out["auto"], out["tile"]) < 2e-5
-    assert timed["auto"] < 0.8 * timed["tile"], timed
+    for form, tol in (("fp64", FP64_TOL), ("mixed", 2e-5)):
+        timed = {}
+        out = {}
+        for kernel in ("tile", "auto"):
+            solver.set_kernel(kernel)
+            solver.timing(True)
+            solver.timing_reset()
+            out[kernel] = solver.accel(form, 0.005)
+            timed[kernel] = solver.timing_read()[0]
+            solver.timing(False)
+        solver.set_kernel("auto")
+        assert rel_err(out["auto"], out["tile"]) < tol, form
+        assert timed["auto"] < 0.8 * timed["tile"], (form, timed)
 
 
 def test_accel_block_four_rows_per_lane(solver, orc):
