@@ -794,14 +794,13 @@ nb_status nb_energy(nb_ctx* c, double eps, double* T, double* U) {
     const long n = long(c->n);
     if (n == 0) return NB_OK;
     NB_HIP(hipSetDevice(c->device), "hipSetDevice");
-    int S = choose_splits(c, n, n, 1);
-    long jchunk = (n + S - 1) / S;
-    jchunk = (jchunk + 63) & ~63L;
-    S = int((n + jchunk - 1) / jchunk);
+    // Workgroups of paired tile rows, split so the grid holds ~8 per CU.
     const long nblk = (n + nbk::BLOCK - 1) / nbk::BLOCK;
+    const long pairs = (nblk + 1) / 2;
+    const long S = std::min(nblk + 1, std::max(1L, (long(c->num_cus) * 8 + pairs - 1) / pairs));
     NB_HIP(c->rows.ensure(size_t(S) * n * 16), "allocate energy rows");
     NB_HIP(c->eblk.ensure(size_t(nblk) * 4 * sizeof(double)), "allocate energy blocks");
-    NB_HIP(nbk::launch_energy((const P4*)c->pm.p, (const P4*)c->vel.p, int(n), S, int(jchunk), eps * eps, c->rows.p,
+    NB_HIP(nbk::launch_energy((const P4*)c->pm.p, (const P4*)c->vel.p, int(n), int(S), eps * eps, c->rows.p,
                               (double*)c->eblk.p, c->stream),
            "energy kernels");
     std::vector<double> h(size_t(nblk) * 4);
@@ -820,7 +819,7 @@ nb_status nb_energy(nb_ctx* c, double eps, double* T, double* U) {
             }
         return s + comp;
     };
-    *U = -0.5 * nsum(0);   // sum_i m_i row_i counts every pair twice; G = 1
+    *U = -nsum(0);   // sum_i m_i row_i counts every pair once; G = 1
     *T = 0.5 * nsum(2);
     return NB_OK;
 }

@@ -80,8 +80,11 @@ struct RKTerms {
 };
 hipError_t launch_rk_combine(int n, const P4* x0, const P4* kv, const P4* ka, long stride, const RKTerms& T,
                              double dt, P4* x_out, P4* v_out, hipStream_t st);
-// rows: splits * n * 16 bytes; blocks_out: 4 doubles per 256-particle block.
-hipError_t launch_energy(const P4* pm, const P4* vel, int n, int splits, int jchunk, double eps2,
-                         void* rows, double* blocks_out, hipStream_t st);
+// Potential rows over the upper triangle (every pair once) in `splits`
+// chunks (1 <= splits <= ceil(n/BLOCK) + 1) and the per-block sums of
+// m_i row_i and m_i |v_i|^2.  rows: splits * n * 16 bytes; blocks_out: 4
+// doubles per 256-particle block.
+hipError_t launch_energy(const P4* pm, const P4* vel, int n, int splits, double eps2, void* rows,
+                         double* blocks_out, hipStream_t st);
 
 }  // namespace nbk

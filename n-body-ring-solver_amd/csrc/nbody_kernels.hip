@@ -362,41 +362,73 @@ __device__ __forceinline__ void dd_merge(DD& a, const DD& b) {
     a.c += e + b.c;
 }
 
-// row_i (split part) = sum_j m_j / sqrt(|x_j - x_i|^2 + eps^2), zero-separation
-// pairs skipped; FP64 with a compensated per-lane sum.
-__global__ __launch_bounds__(BLOCK) void k_potential(const P4* __restrict__ pm, int n, int jchunk,
+// m_j / sqrt(d2 + eps2) with the FP64 rsq and its quadratic correction
+// (1-e)^(-1/2) = 1 + e/2 + 3/8 e^2; zero-separation pairs (d2 == 0) give 0.
+__device__ __forceinline__ double pot_term(const P4& me, const P4& q, double eps2) {
+    const double dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+    const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    const double s2 = d2 + eps2;
+    const double y = __builtin_amdgcn_rsq(s2);
+    const double e = __builtin_fma(-s2, y * y, 1.0);
+    const double inv = y * __builtin_fma(e, __builtin_fma(e, 0.375, 0.5), 1.0);
+    return (d2 > 0.0) ? q.m * inv : 0.0;
+}
+
+// Upper-triangle potential rows: row_i = sum_{j > i} m_j / sqrt(|x_j - x_i|^2
+// + eps^2), so that U = -sum_i m_i row_i counts every pair once.  Tiles of
+// BLOCK particles; workgroup (g, s) owns the rows of tiles I1 = g and
+// I2 = nT-1-g, whose tile lists J = I..nT-1 together hold nT+1 tiles (every
+// workgroup row has the same work), and the s-th of `splits` chunks of that
+// concatenated list.  It writes its partial of each of its particles' rows to
+// out[s*n + i], zero where its chunk holds no tile of that row, so every
+// (s, i) is written once.  Terms are summed in FP64 per 64 j (all positive:
+// relative error <= 63 u) and the 64-term sums added with TwoSum.
+__global__ __launch_bounds__(BLOCK) void k_potential(const P4* __restrict__ pm, int n, int nT, int splits,
                                                      double eps2, DD* __restrict__ out) {
-    __shared__ P4 tile[2][BLOCK];
+    __shared__ P4 tile[BLOCK];
     const int t = threadIdx.x;
-    const int split = blockIdx.y;
-    const int jbeg = split * jchunk;
-    const int jend = min(n, jbeg + jchunk);
-    const int i = blockIdx.x * BLOCK + t;
-    const P4 me = (i < n) ? pm[i] : P4{0.0, 0.0, 0.0, 0.0};
-    DD acc{0.0, 0.0};
-    const int ntiles = (jend > jbeg) ? (jend - jbeg + BLOCK - 1) / BLOCK : 0;
-    P4 nxt = load_p4(pm, jbeg + t, jend);
-    for (int tt = 0; tt < ntiles; tt++) {
-        const int buf = tt & 1;
-        tile[buf][t] = nxt;
-        __syncthreads();
-        const int j0 = jbeg + tt * BLOCK;
-        if (tt + 1 < ntiles) nxt = load_p4(pm, j0 + BLOCK + t, jend);
-        const int cnt = min(BLOCK, jend - j0);
-        for (int jj = 0; jj < cnt; jj++) {
-            const P4 q = tile[buf][jj];
-            const double dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-            const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-            if (d2 == 0.0) continue;
-            const double s2 = d2 + eps2;
-            const double y = __builtin_amdgcn_rsq(s2);
-            const double e = __builtin_fma(-s2, y * y, 1.0);
-            // (1-e)^(-1/2) = 1 + e/2 + 3/8 e^2 + O(e^3)
-            const double inv = y * __builtin_fma(e, __builtin_fma(e, 0.375, 0.5), 1.0);
-            dd_add(acc, q.m * inv);
+    const int g = blockIdx.x, s = blockIdx.y;
+    const int I1 = g, I2 = nT - 1 - g;
+    const int len1 = nT - I1;
+    const int len2 = (I2 != I1) ? nT - I2 : 0;
+    const int total = len1 + len2;
+    const int per = (total + splits - 1) / splits;
+    const int c0 = min(total, s * per), c1 = min(total, c0 + per);
+    for (int seg = 0; seg < 2; seg++) {
+        if (seg == 1 && len2 == 0) break;
+        const int I = seg ? I2 : I1;
+        const int base = seg ? len1 : 0;
+        const int len = seg ? len2 : len1;
+        const int ka = max(c0, base) - base, kb = min(c1, base + len) - base;   // tiles I+ka .. I+kb-1
+        const int i = I * BLOCK + t;
+        const P4 me = (i < n) ? pm[i] : P4{0.0, 0.0, 0.0, 0.0};
+        DD acc{0.0, 0.0};
+        P4 nxt = (ka < kb) ? load_p4(pm, (I + ka) * BLOCK + t, n) : P4{0.0, 0.0, 0.0, 0.0};
+        for (int k = ka; k < kb; k++) {
+            tile[t] = nxt;
+            __syncthreads();
+            if (k + 1 < kb) nxt = load_p4(pm, (I + k + 1) * BLOCK + t, n);
+            if (k == 0) {   // the diagonal tile: j > i only
+                for (int j0 = 0; j0 < BLOCK; j0 += 64) {
+                    double part = 0.0;
+                    for (int jj = j0; jj < j0 + 64; jj++) {
+                        const double v = pot_term(me, tile[jj], eps2);
+                        part += (jj > t) ? v : 0.0;
+                    }
+                    dd_add(acc, part);
+                }
+            } else {
+                for (int j0 = 0; j0 < BLOCK; j0 += 64) {
+                    double part = 0.0;
+#pragma unroll 8
+                    for (int jj = j0; jj < j0 + 64; jj++) part += pot_term(me, tile[jj], eps2);
+                    dd_add(acc, part);
+                }
+            }
+            __syncthreads();
         }
+        if (i < n) out[(long)s * n + i] = acc;
     }
-    if (i < n) out[(long)split * n + i] = acc;
 }
 
 // Per block: U_b = sum_i m_i row_i (row over all splits), T_b = sum_i m_i |v_i|^2,
@@ -518,11 +550,13 @@ hipError_t launch_rk_combine(int n, const P4* x0, const P4* kv, const P4* ka, lo
     return hipGetLastError();
 }
 
-hipError_t launch_energy(const P4* pm, const P4* vel, int n, int splits, int jchunk, double eps2,
-                         void* rows, double* blocks_out, hipStream_t st) {
+hipError_t launch_energy(const P4* pm, const P4* vel, int n, int splits, double eps2, void* rows,
+                         double* blocks_out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const dim3 grid((n + BLOCK - 1) / BLOCK, splits);
-    hipLaunchKernelGGL(k_potential, grid, dim3(BLOCK), 0, st, pm, n, jchunk, eps2, (DD*)rows);
+    const int nT = (n + BLOCK - 1) / BLOCK;
+    if (splits < 1 || splits > nT + 1) return hipErrorInvalidValue;
+    const dim3 grid((nT + 1) / 2, splits);
+    hipLaunchKernelGGL(k_potential, grid, dim3(BLOCK), 0, st, pm, n, nT, splits, eps2, (DD*)rows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_energy_reduce, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const DD*)rows,
