@@ -164,7 +164,8 @@ NB_API nb_status nb_step_rk(nb_ctx* ctx, size_t time_steps, double dt, const dou
 
 /* Kinetic and potential energy of the resident state:
  * T = 1/2 sum m v^2 (CPU/force_calculation.cpp:1740), U = -G sum_{i<j} m_i m_j /
- * sqrt(r^2+eps^2) (:1880, softened as FPGA/host/accuracy_evaluation.cpp:35).
+ * sqrt(r^2+eps^2) (:1880, softened as FPGA/host/accuracy_evaluation.cpp:35;
+ * a coincident pair adds m_i m_j / eps, and is skipped when eps = 0).
  * FP64 with compensated, fixed-order sums (bitwise reproducible). */
 NB_API nb_status nb_energy(nb_ctx* ctx, double eps, double* T, double* U);
 

@@ -363,19 +363,21 @@ __device__ __forceinline__ void dd_merge(DD& a, const DD& b) {
 }
 
 // m_j / sqrt(d2 + eps2) with the FP64 rsq and its quadratic correction
-// (1-e)^(-1/2) = 1 + e/2 + 3/8 e^2; zero-separation pairs (d2 == 0) give 0.
+// (1-e)^(-1/2) = 1 + e/2 + 3/8 e^2.  A pair with d2 + eps2 == 0 (coincident,
+// unsoftened) gives 0; with eps > 0 coincident particles give m_j / eps, as
+// the softened potential of FPGA/host/accuracy_evaluation.cpp:35 does.
 __device__ __forceinline__ double pot_term(const P4& me, const P4& q, double eps2) {
     const double dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-    const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    const double s2 = d2 + eps2;
+    const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
     const double y = __builtin_amdgcn_rsq(s2);
     const double e = __builtin_fma(-s2, y * y, 1.0);
     const double inv = y * __builtin_fma(e, __builtin_fma(e, 0.375, 0.5), 1.0);
-    return (d2 > 0.0) ? q.m * inv : 0.0;
+    return (s2 > 0.0) ? q.m * inv : 0.0;
 }
 
 // Upper-triangle potential rows: row_i = sum_{j > i} m_j / sqrt(|x_j - x_i|^2
-// + eps^2), so that U = -sum_i m_i row_i counts every pair once.  Tiles of
+// + eps^2) (coincident pairs skipped only when eps = 0), so that
+// U = -sum_i m_i row_i counts every pair once.  Tiles of
 // BLOCK particles; workgroup (g, s) owns the rows of tiles I1 = g and
 // I2 = nT-1-g, whose tile lists J = I..nT-1 together hold nT+1 tiles (every
 // workgroup row has the same work), and the s-th of `splits` chunks of that

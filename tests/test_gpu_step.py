@@ -76,6 +76,26 @@ def test_energy_vs_oracle(solver, orc, eps):
     assert solver.energy(eps) == (T, U)   # bitwise reproducible
 
 
+@pytest.mark.parametrize("n", [1, 2, 255, 257, 700])
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_energy_ragged_and_coincident(solver, orc, n, eps):
+    # tile counts 1..3 with partial last tiles, varied masses and (n >= 4)
+    # coincident particles: skipped when eps = 0, m_i m_j / eps otherwise
+    m, pos, vel, _ = nbody_amd.init_system("plummer", n, 40 + n)
+    m = m * np.linspace(0.5, 1.5, n)
+    if n >= 4:
+        pos[:, 3] = pos[:, 1]
+        pos[:, n - 1] = pos[:, 0]
+    solver.set_state(m, pos, vel)
+    T, U = solver.energy(eps)
+    To, Uo = orc.energy(m, pos, vel, eps)
+    assert abs(T - To) <= 1e-15 * abs(To)
+    if n == 1:
+        assert U == 0.0 and Uo == 0.0
+    else:
+        assert abs(U - Uo) <= 1e-14 * abs(Uo)
+
+
 def test_energy_fixture(solver):
     g = golden("jitter_300.npz")
     solver.set_state(g["m"], g["pos"], g["vel"])
