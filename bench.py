@@ -144,7 +144,7 @@ def run_form(s, torch, state, form, steps, warmup, dt, eps):
     }
 
 
-def cpu_baseline(state):
+def cpu_baseline(state, ic_text):
     """The reference's compute_forces_avx512_rsqrt_4i_blocked on this host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     try:
@@ -160,7 +160,7 @@ def cpu_baseline(state):
         threads = ref.lib.ref_max_threads()
         return {"value": n * (n - 1) / t / 1e9, "unit": "GPairs/s (effective, N(N-1) per evaluation)",
                 "cores": threads, "kind": "reference",
-                "sample": f"best of {reps} force evaluations, N={n} Plummer, AVX512-RSQRT-4I blocked kernel "
+                "sample": f"best of {reps} force evaluations, {ic_text}, AVX512-RSQRT-4I blocked kernel "
                           f"(CPU/force_calculation.cpp:1137), unsoftened (the reference has no eps; cost per "
                           f"pair is the same), OMP_NUM_THREADS={threads}",
                 "steps_per_s": 1.0 / t}
@@ -174,6 +174,15 @@ def cpu_baseline(state):
     t = time.perf_counter() - t0
     return {"value": ni * n / t / 1e9, "unit": "GPairs/s", "cores": os.cpu_count(), "kind": "port",
             "sample": f"oracle C port, {ni} i-particles x {n} j-particles"}
+
+
+def force_kernel_name(form, n):
+    """The force kernel the library picks for a resident system of n particles."""
+    kf = "FP64" if form == "fp64" else "MIXED_NR"
+    if n < 16384:
+        return f"k_accel<{kf}, IPT=1> (tile)"
+    k = (8 if form == "fp64" else 6) if n >= 131072 else 4
+    return f"k_accel_sym<{kf}, K={k}> (symmetric)"
 
 
 def roofline_entry(name, kernel_ms, n, peak, traffic=None, pairs="N^2 ordered pairs (20 flops each)"):
@@ -225,6 +234,8 @@ def main():
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=1.0 / 256)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--ic", choices=("plummer", "sphere"), default="plummer",
+                    help="initial conditions: Plummer sphere (a = 1, r < 20) or uniform unit sphere at rest")
     ap.add_argument("--forms", default="fp64,mixed,learned,learned-cubic,learned-p8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -255,7 +266,12 @@ def main():
     import torch  # only for the device-wide synchronisation around the timed region
 
     n = args.n
-    m, pos, vel, _ = nbody_amd.init_system("plummer", n, args.seed, [1.0, 20.0])
+    if args.ic == "plummer":
+        m, pos, vel, _ = nbody_amd.init_system("plummer", n, args.seed, [1.0, 20.0])
+        ic_text = "Plummer N={} a=1 r<20".format(n)
+    else:
+        m, pos, vel, _ = nbody_amd.init_system("sphere", n, args.seed)
+        ic_text = "uniform unit sphere N={} at rest".format(n)
     state = (m, pos, vel)
     s = nbody_amd.Solver(0)
     s.set_state(m, pos, vel)
@@ -294,20 +310,24 @@ def main():
     hr = forms[head]
 
     rooflines = {}
+    # HBM traffic was counted for the default workload only
+    profiled = args.ic == "plummer" and n == 262144
     if forms.get("fp64", {}).get("exact_force_evals"):
-        rooflines["fp64"] = roofline_entry("k_accel_sym<FP64, K=8> (symmetric, FP64 VALU)",
+        rooflines["fp64"] = roofline_entry(force_kernel_name("fp64", n) + " (FP64 VALU)",
                                            forms["fp64"]["force_kernel_ms_avg"], n, FP64_PEAK_TFLOPS,
-                                           profile_traffic("k_accel_sym_plummer_n262144_pmc.md"))
+                                           profile_traffic("k_accel_sym_plummer_n262144_pmc.md") if profiled
+                                           else None)
     if forms.get("mixed", {}).get("exact_force_evals"):
-        rooflines["mixed"] = roofline_entry("k_accel_sym<MIXED_NR, K=6> (symmetric, FP32 pair math on FP64 "
+        rooflines["mixed"] = roofline_entry(force_kernel_name("mixed", n) + " (FP32 pair math on FP64 "
                                             "differences; against the FP32 peak, which counts packed FP32)",
                                             forms["mixed"]["force_kernel_ms_avg"], n, FP32_PEAK_TFLOPS,
                                             profile_traffic("k_accel_sym_plummer_n262144_pmc.md",
-                                                            "traffic_bytes_per_launch_mixed"))
+                                                            "traffic_bytes_per_launch_mixed") if profiled
+                                            else None)
     dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
     roofline = rooflines.get(dominant) or rooflines.get("fp64")
 
-    cpu = None if args.no_cpu else cpu_baseline(state)
+    cpu = None if args.no_cpu else cpu_baseline(state, ic_text)
     line = {
         "metric": METRIC,
         "value": hr["gpairs_per_s"],
@@ -320,9 +340,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f64+f32",
-        "data": "synthetic (seeded Plummer sphere)",
-        "config": {"workload": f"Plummer N={n} a=1 r<20, eps={args.eps}, dt=1/{round(1 / args.dt)}, "
-                               f"2nd-order leapfrog, {args.steps} steps", "n": n, "eps": args.eps, "dt": args.dt,
+        "data": "synthetic (seeded {})".format("Plummer sphere" if args.ic == "plummer" else "uniform sphere"),
+        "config": {"workload": f"{ic_text}, eps={args.eps}, dt={args.dt:g}, 2nd-order leapfrog, "
+                               f"{args.steps} steps", "n": n, "eps": args.eps, "dt": args.dt,
                    "headline_form": head, "energy_bound_met": head in ok},
         "steps_per_s": hr["steps_per_s"],
         "dE_E": hr["dE_E"],
