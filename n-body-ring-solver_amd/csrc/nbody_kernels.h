@@ -15,6 +15,24 @@ struct __attribute__((aligned(32))) P4 {
 
 enum Form { FORM_FP64 = 0, FORM_MIXED = 1, FORM_MIXED_NR = 2 };
 
+// Correction of the hardware reciprocal square root, shared by the pair
+// functions.  With y ~ r2^(-1/2) and s = r2 y^2 = 1 - e, the exact
+// r2^(-3/2) = y^3 (1 - e)^(-3/2).
+//  FP64, to second order: 1 + 3/2 e + 15/8 e^2 = 15/8 ((s - 7/5)^2 + 28/75),
+//    so w = y^3 ((s + kCubicShift)^2 + kCubicOffset) takes two FMAs and the
+//    factor 15/8 rides on the masses (kCubicMass), which the pair multiplies
+//    in anyway.
+//  FP32, to first order (one Newton-Raphson step on the cube):
+//    1 + 3/2 e = 3/2 (5/3 - s).  With F = fl32(5/3) the factor
+//    c = 1/(F - 1) (kNewtonMass, on the masses) makes c (F - s) exactly 1 at
+//    s = 1, so w = y^3 (F - s) takes one FMA; c differs from 3/2 by 6e-8
+//    relative, a change of the first-order term far below FP32 rounding.
+constexpr double kCubicShift = -1.4;
+constexpr double kCubicOffset = 28.0 / 75.0;
+constexpr double kCubicMass = 1.875;
+constexpr float kNewtonOffset = 5.0f / 3.0f;
+constexpr double kNewtonMass = 1.0 / ((double)kNewtonOffset - 1.0);
+
 // Launch description of the force kernel: accelerations of pi[0..ni) from
 // pj[jbeg..jend), the j-range cut into `splits` chunks of `jchunk` (one per
 // blockIdx.y); chunk s writes out[(split_base + s) * out_stride + i].

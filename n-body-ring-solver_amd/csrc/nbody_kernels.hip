@@ -41,20 +41,22 @@ __device__ __forceinline__ P4 load_p4(const P4* __restrict__ p, int j, int jend)
 
 // FP64: v_rsq_f64 gives y ~ r2^(-1/2) to about 2^-22 relative; with
 // e = 1 - r2 y^2 the exact y_true^3 = y^3 (1-e)^(-3/2) = y^3 (1 + 3/2 e + 15/8 e^2 + O(e^3)),
-// so two FMAs restore full FP64 accuracy (|e|^3 ~ 2^-66).  The constants
-// c15 = 1.5 and c1875 = 1.875 arrive as kernel arguments so they stay in
-// registers (as literals the compiler rebuilds them with two moves per pair).
+// so two FMAs restore full FP64 accuracy (|e|^3 ~ 2^-66): the polynomial is
+// evaluated as (s + cshift)^2 + coff, s = r2 y^2 (kCubicShift, kCubicOffset),
+// and mj carries its factor 15/8 (staged with kCubicMass).  The constants
+// arrive as kernel arguments so they stay in registers (as literals the
+// compiler rebuilds them with two moves per pair).
 template <bool EPS0>
 __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, double xj, double yj,
-                                          double zj, double mj, double eps2, double c15, double c1875,
+                                          double zj, double mj, double eps2, double cshift, double coff,
                                           double& ax, double& ay, double& az) {
     const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
     const double y = __builtin_amdgcn_rsq(r2);
     const double t = y * y;
-    const double e = __builtin_fma(-r2, t, 1.0);
+    const double u = __builtin_fma(r2, t, cshift);
     const double y3 = t * y;
-    const double p = __builtin_fma(e, __builtin_fma(e, c1875, c15), 1.0);
+    const double p = __builtin_fma(u, u, coff);
     double s = (mj * y3) * p;
     if (EPS0) s = (r2 > 0.0) ? s : 0.0;   // zero separation (self pair) contributes 0
     ax = __builtin_fma(s, dx, ax);
@@ -63,20 +65,19 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
 }
 
 // Mixed: FP64 differences rounded to FP32, FP32 rsqrt (1 ulp) plus one
-// Newton-Raphson step y <- y (3/2 - r2/2 y^2), FP32 pair arithmetic.
+// Newton-Raphson step applied to the cube, y^3 (1 + 3/2 (1 - r2 y^2))
+// = c y^3 (F - r2 y^2) (F = kNewtonOffset; mj carries c = kNewtonMass),
+// FP32 pair arithmetic.
 template <bool EPS0, bool NR>
 __device__ __forceinline__ void pair_mixed(double xi, double yi, double zi, double xj, double yj,
                                            double zj, float mj, float eps2, float& ax, float& ay,
                                            float& az) {
     const float dx = (float)(xj - xi), dy = (float)(yj - yi), dz = (float)(zj - zi);
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-    float y = __builtin_amdgcn_rsqf(r2);
-    if (NR) {
-        const float h = 0.5f * r2;
-        y = y * __builtin_fmaf(-h, y * y, 1.5f);
-    }
+    const float y = __builtin_amdgcn_rsqf(r2);
     const float y2 = y * y;
     float s = (mj * y) * y2;
+    if (NR) s = s * __builtin_fmaf(-r2, y2, kNewtonOffset);
     if (EPS0) s = (r2 > 0.0f) ? s : 0.0f;
     ax = __builtin_fmaf(s, dx, ax);
     ay = __builtin_fmaf(s, dy, ay);
@@ -90,10 +91,12 @@ template <int FORM, bool EPS0, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int ni,
                                                  const P4* __restrict__ pj, int jbeg_all,
                                                  int jend_all, int jchunk, double eps2, double G,
-                                                 double c15, double c1875, P4* __restrict__ out,
+                                                 double cshift, double coff, P4* __restrict__ out,
                                                  long out_stride, int split_base) {
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR);
+    // the factor of the correction polynomial carried by the staged masses
+    constexpr double MSCALE = MIXED ? (NR ? kNewtonMass : 1.0) : kCubicMass;
     using Rec = typename std::conditional<MIXED, P4m, P4>::type;
     __shared__ Rec tile[2][BLOCK];
 
@@ -120,9 +123,9 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
     for (int tt = 0; tt < ntiles; tt++) {
         const int buf = tt & 1;
         if constexpr (MIXED) {
-            tile[buf][t] = P4m{nxt.x, nxt.y, nxt.z, (float)nxt.m, 0.0f};
+            tile[buf][t] = P4m{nxt.x, nxt.y, nxt.z, (float)(nxt.m * MSCALE), 0.0f};
         } else {
-            tile[buf][t] = nxt;
+            tile[buf][t] = P4{nxt.x, nxt.y, nxt.z, nxt.m * MSCALE};
         }
         __syncthreads();
         const int j0 = jbeg + tt * BLOCK;
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
                     const P4 q = tl[jj + u];
 #pragma unroll
                     for (int k = 0; k < IPT; k++)
-                        pair_fp64<EPS0>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2, c15, c1875,
+                        pair_fp64<EPS0>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2, cshift, coff,
                                         ax[k], ay[k], az[k]);
                 }
             }
@@ -480,7 +483,7 @@ template <int FORM, bool EPS0, int IPT>
 static hipError_t launch_accel_t(const AccelLaunch& L, hipStream_t st) {
     const dim3 grid((L.ni + BLOCK * IPT - 1) / (BLOCK * IPT), L.splits);
     hipLaunchKernelGGL((k_accel<FORM, EPS0, IPT>), grid, dim3(BLOCK), 0, st, L.pi, L.ni, L.pj,
-                       L.jbeg, L.jend, L.jchunk, L.eps2, L.G, 1.5, 1.875, L.out, L.out_stride,
+                       L.jbeg, L.jend, L.jchunk, L.eps2, L.G, kCubicShift, kCubicOffset, L.out, L.out_stride,
                        L.split_base);
     return hipGetLastError();
 }

@@ -195,7 +195,9 @@ __host__ __device__ inline Taper make_taper(double rc, double eps) {
 }
 
 // One ordered pair: a += m S(q) w d, w = (r^2 + eps^2)^(-3/2) from v_rsq_f64
-// and the correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2 (e = 1 - s2 y^2).
+// and the correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2 (e = 1 - s2 y^2),
+// evaluated as (s2 y^2 + kCubicShift)^2 + kCubicOffset with the factor 15/8
+// in the staged mass (stage64).
 // For eps > 0 a zero separation needs no test: w stays finite and d = 0
 // makes the term vanish.  With eps = 0 (EPS0) such pairs are selected out.
 template <bool EPS0>
@@ -206,8 +208,9 @@ __device__ __forceinline__ void near_pair(const nbk::P4& me, const nbk::P4& o, d
     const double u = fmin(fmax(__builtin_fma(s2, k1, k0), 0.0), 1.0);
     const double S = __builtin_fma(-u * u * u, __builtin_fma(u, __builtin_fma(6.0, u, -15.0), 10.0), 1.0);
     const double y = __builtin_amdgcn_rsq(s2);
-    const double e = __builtin_fma(-s2, y * y, 1.0);
-    const double w = (y * y) * y * __builtin_fma(e, __builtin_fma(1.875, e, 1.5), 1.0);
+    const double t = y * y;
+    const double v = __builtin_fma(s2, t, nbk::kCubicShift);
+    const double w = t * y * __builtin_fma(v, v, nbk::kCubicOffset);
     double f = o.m * S * w;
     if (EPS0) f = (s2 > 0.0) ? f : 0.0;
     ax = __builtin_fma(f, dx, ax);
@@ -215,10 +218,17 @@ __device__ __forceinline__ void near_pair(const nbk::P4& me, const nbk::P4& o, d
     az = __builtin_fma(f, dz, az);
 }
 
-// Staged record of the FP32 pair: FP64 position, the mass already rounded to
-// FP32 and kept in the low word of the fourth slot.
+// Staged record of the FP64 pair: the mass times the correction's factor 15/8.
+__device__ __forceinline__ nbk::P4 stage64(const nbk::P4& p) {
+    return nbk::P4{p.x, p.y, p.z, p.m * nbk::kCubicMass};
+}
+
+// Staged record of the FP32 pair: FP64 position, the mass times the Newton
+// step's factor kNewtonMass, rounded to FP32 and kept in the low word of the
+// fourth slot.
 __device__ __forceinline__ nbk::P4 stage32(const nbk::P4& p) {
-    return nbk::P4{p.x, p.y, p.z, __builtin_bit_cast(double, make_int2(__builtin_bit_cast(int, (float)p.m), 0))};
+    return nbk::P4{p.x, p.y, p.z,
+                   __builtin_bit_cast(double, make_int2(__builtin_bit_cast(int, (float)(p.m * nbk::kNewtonMass)), 0))};
 }
 
 __device__ __forceinline__ float staged_mass(const nbk::P4& o) {
@@ -227,7 +237,8 @@ __device__ __forceinline__ float staged_mass(const nbk::P4& o) {
 
 // The same pair with FP64 differences rounded to FP32 and FP32 arithmetic:
 // FP32 taper, v_rsq_f32 with the first-order correction of the cube,
-// w = y^3 (1 + 3/2 (1 - s2 y^2)); o is a stage32 record.
+// w = y^3 (1 + 3/2 (1 - s2 y^2)) = c y^3 (kNewtonOffset - s2 y^2) with c in
+// the staged mass; o is a stage32 record.
 template <bool EPS0>
 __device__ __forceinline__ void near_pair32(const nbk::P4& me, const nbk::P4& o, float k1, float k0, float eps2,
                                             float& ax, float& ay, float& az) {
@@ -237,7 +248,7 @@ __device__ __forceinline__ void near_pair32(const nbk::P4& me, const nbk::P4& o,
     const float S = __builtin_fmaf(-u * u * u, __builtin_fmaf(u, __builtin_fmaf(6.0f, u, -15.0f), 10.0f), 1.0f);
     const float y = __builtin_amdgcn_rsqf(s2);
     const float t = y * y;
-    const float w = t * y * __builtin_fmaf(__builtin_fmaf(-s2, t, 1.0f), 1.5f, 1.0f);
+    const float w = t * y * __builtin_fmaf(-s2, t, nbk::kNewtonOffset);
     float f = staged_mass(o) * S * w;
     if (EPS0) f = (s2 > 0.0f) ? f : 0.0f;
     ax = __builtin_fmaf(f, dx, ax);
@@ -298,7 +309,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
             const int nkeep = __popcll(keep);
             if (nkeep == 0) continue;   // wave-uniform
             const int m4 = (nkeep + 3) & ~3;   // fillers pad to a multiple of 4
-            if (use) L[__popcll(keep & ((1ull << lane) - 1ull))] = F32 ? stage32(J) : J;
+            if (use) L[__popcll(keep & ((1ull << lane) - 1ull))] = F32 ? stage32(J) : stage64(J);
             // filler: this lane's position with zero mass, so it adds nothing
             // to any lane (d = 0 for this one, m = 0 for the others)
             if (lane >= nkeep && lane < m4) L[lane] = nbk::P4{me.x, me.y, me.z, 0.0};

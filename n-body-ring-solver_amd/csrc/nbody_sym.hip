@@ -52,19 +52,21 @@ __device__ __forceinline__ float rotate_down(float v) {
 }
 
 // One unordered pair, FP64.  d = x_j - x_i; w = 1/(|d|^2+eps^2)^(3/2) from
-// v_rsq_f64 and the cubic correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2;
-// a_i += m_j w d, b_j -= m_i w d.  `keep` zeroes masked pairs.
+// v_rsq_f64 and the second-order correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2
+// written as ((s + cshift)^2 + coff), s = r2 y^2 (kCubicShift, kCubicOffset),
+// with the 15/8 already in both masses; a_i += m_j w d, b_j -= m_i w d.
+// `keep` zeroes masked pairs.
 template <bool EPS0, bool MASK>
 __device__ __forceinline__ void sym64(double xi, double yi, double zi, double mi, double xj, double yj,
-                                      double zj, double mj, double eps2, double c15, double c1875, bool keep,
+                                      double zj, double mj, double eps2, double cshift, double coff, bool keep,
                                       double& ax, double& ay, double& az, double& bx, double& by,
                                       double& bz) {
     const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
     const double y = __builtin_amdgcn_rsq(r2);
     const double t = y * y;
-    const double e = __builtin_fma(-r2, t, 1.0);
-    const double p = __builtin_fma(e, __builtin_fma(e, c1875, c15), 1.0);
+    const double u = __builtin_fma(r2, t, cshift);
+    const double p = __builtin_fma(u, u, coff);
     double w = (t * y) * p;
     if (EPS0) w = (r2 > 0.0) ? w : 0.0;
     if (MASK) w = keep ? w : 0.0;
@@ -81,7 +83,9 @@ __device__ __forceinline__ void sym64(double xi, double yi, double zi, double mi
 // FP32 sums (flushed to FP64 per 64-particle chunk).  With NR the Newton-Raphson
 // step is applied to the cube: y = rsq(r2) (1 ulp), e = 1 - r2 y^2, and
 // r^-3 = y^3 (1 - e)^(-3/2) = y^3 (1 + 3/2 e) + O(e^2) -- the same first-order
-// gain as refining y by y (3 - r2 y^2)/2 and cubing, for two FMAs fewer.
+// gain as refining y by y (3 - r2 y^2)/2 and cubing, for two instructions
+// fewer: 1 + 3/2 e = c (F - r2 y^2) with F = kNewtonOffset and c = kNewtonMass
+// already in both masses.
 template <bool EPS0, bool MASK, bool NR>
 __device__ __forceinline__ void sym32(double xi, double yi, double zi, float mi, double xj, double yj,
                                       double zj, float mj, float eps2, bool keep, float& ax, float& ay,
@@ -91,7 +95,7 @@ __device__ __forceinline__ void sym32(double xi, double yi, double zi, float mi,
     const float y = __builtin_amdgcn_rsqf(r2);
     const float t = y * y;
     float w = t * y;
-    if (NR) w = w * __builtin_fmaf(__builtin_fmaf(-r2, t, 1.0f), 1.5f, 1.0f);
+    if (NR) w = w * __builtin_fmaf(-r2, t, kNewtonOffset);
     if (EPS0) w = (r2 > 0.0f) ? w : 0.0f;
     if (MASK) w = keep ? w : 0.0f;
     const float si = mj * w, sj = mi * w;
@@ -121,13 +125,15 @@ __device__ __forceinline__ P4 load_or_zero(const P4* __restrict__ p, int j, int 
 template <int FORM, bool EPS0, int K>
 __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __restrict__ pm, int n,
                                                                    const int4* __restrict__ items, int nitems,
-                                                                   int nB, double eps2, double c15,
-                                                                   double c1875, P4* __restrict__ part,
+                                                                   int nB, double eps2, double cshift,
+                                                                   double coff, P4* __restrict__ part,
                                                                    P4* __restrict__ react64,
                                                                    float4* __restrict__ react32) {
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR);
     constexpr int S = 64 * K;
+    // the factor of the correction polynomial carried by the masses
+    constexpr double MSCALE = MIXED ? (NR ? kNewtonMass : 1.0) : kCubicMass;
     __shared__ P4 jt[kWavesPerGroup][128];   // each wave's chunk, stored twice
 
     const int wave = threadIdx.x >> 6;
@@ -142,7 +148,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const P4 p = load_or_zero(pm, A * S + k * 64 + lane, n);
-        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.m;
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.m * MSCALE;
         ax[k] = ay[k] = az[k] = 0.0;
     }
     const float eps2f = (float)eps2;
@@ -153,7 +159,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
     for (int B = b0; B < b1; B++) {
         const bool diag = (B == A);
         for (int c = 0; c < K; c++) {
-            const P4 q = load_or_zero(pm, B * S + c * 64 + lane, n);
+            P4 q = load_or_zero(pm, B * S + c * 64 + lane, n);
+            q.m *= MSCALE;
             lds[lane] = q;
             lds[lane + 64] = q;
             wave_sync();
@@ -166,7 +173,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                         const P4 J = lds[lane + t];
 #pragma unroll
                         for (int k = 0; k < K; k++)
-                            sym64<EPS0, false>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, c15, c1875,
+                            sym64<EPS0, false>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
                                                true, ax[k], ay[k], az[k], bx, by, bz);
                         bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
                     }
@@ -176,7 +183,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                         const bool same_ok = (t >= 1) && (lane + t < 64);   // i = c*64+lane < j = c*64+lane+t
 #pragma unroll
                         for (int k = 0; k < K; k++)
-                            sym64<EPS0, true>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, c15, c1875,
+                            sym64<EPS0, true>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
                                               (k < c) || (k == c && same_ok), ax[k], ay[k], az[k], bx, by, bz);
                         bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
                     }
@@ -343,7 +350,7 @@ template <int FORM, bool EPS0, int K>
 static hipError_t launch_sym_t(const SymLaunch& L, hipStream_t st) {
     const int groups = (L.nitems + kWavesPerGroup - 1) / kWavesPerGroup;
     hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm, L.n,
-                       L.items, L.nitems, L.nB, L.eps2, 1.5, 1.875, L.part, L.react64, L.react32);
+                       L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, L.part, L.react64, L.react32);
     return hipGetLastError();
 }
 
