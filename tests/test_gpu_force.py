@@ -186,6 +186,27 @@ def test_kernels_vs_oracle(solver, orc, kernel, n, eps):
         solver.set_kernel("auto")
 
 
+@pytest.mark.parametrize("kernel", ["tile", "symmetric"])
+def test_mixed_has_no_scale_bias(solver, orc, kernel):
+    """The Newton step's factor rides on the FP32 masses (c = 1/(fl32(5/3) - 1),
+    not 3/2): the mean projected error (a_mixed - a_ref).a_ref / |a_ref|^2 over
+    20000 particles stays at the level the FP32 rounding of m c leaves (m = 2^-15
+    makes that rounding the one of c alone, +2e-8), while the factor 3/2 would
+    shrink every force by 6e-8."""
+    n = 20000
+    _, pos, _, _ = nbody_amd.init_system("plummer", n, 12)
+    m = np.full(n, 2.0 ** -15)
+    ref = orc.accel(m, pos, 0.01)
+    solver.set_kernel(kernel)
+    try:
+        solver.set_state(m, pos)
+        am = solver.accel("mixed", 0.01)
+    finally:
+        solver.set_kernel("auto")
+    bias = float(np.mean(np.sum((am - ref) * ref, axis=0) / np.sum(ref * ref, axis=0)))
+    assert abs(bias) < 4e-8, bias
+
+
 @pytest.mark.parametrize("n", [40000, 131072 + 77])
 def test_symmetric_matches_tile_large(solver, n):
     # K = 4 and K = 8 blocks with a ragged last block; both kernels agree to the FP64 bar
