@@ -405,6 +405,9 @@ struct Learned {
     bool net_ok = false;       // the network passed the last check: use it until the next
     bool clear_held = true;    // held to be cleared before the next substep
     int nflag = 0;
+    int* nflag_h = nullptr;          // pinned host copy of the flag count
+    hipEvent_t nflag_ev = nullptr;   // recorded after that copy
+    bool nflag_wait = false;         // a copy is in flight: take_flag_count() waits for it
     std::vector<double> chk_h;
     nb_learned_stats st{};
 };
@@ -502,6 +505,8 @@ nb_status ensure_particles(Learned& L, size_t n) {
     LH(L.nearacc.ensure(m * sizeof(P4)), "allocate near-field accelerations");
     LH(L.flags.ensure(m * sizeof(int)), "allocate flags");
     LH(L.nflags.ensure(sizeof(int)), "allocate flag counter");
+    if (!L.nflag_h) LH(hipHostMalloc((void**)&L.nflag_h, sizeof(int), hipHostMallocDefault), "allocate pinned count");
+    if (!L.nflag_ev) LH(hipEventCreateWithFlags(&L.nflag_ev, hipEventDisableTiming), "create event");
     L.n = n;
     L.count = 0;
     L.head = 3;
@@ -556,7 +561,9 @@ inline int4 ring_slots(const Learned& L) {
 // for the current substep into `out` (the cubic alone stays in L.cub); marks
 // and lists the particles to be summed exactly.  The network is applied when
 // `allow_net`, it is on and trained, and the history is evenly spaced (noted
-// in L.pred_net).  With `count`, waits for and records how many are marked.
+// in L.pred_net).  With `count`, queues the copy of how many are marked to the
+// host; take_flag_count() waits for it, so work queued in between keeps the
+// device busy while the host waits.
 nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
@@ -575,11 +582,19 @@ nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
                        out);
     LH(hipGetLastError(), "combine kernel");
     if (count) {
-        int nf = 0;
-        LH(hipMemcpyAsync(&nf, L.nflags.p, sizeof(int), hipMemcpyDeviceToHost, st), "download flag count");
-        LH(hipStreamSynchronize(st), "hipStreamSynchronize");
-        L.nflag = std::min<int>(std::max(nf, 0), (int)n);
+        LH(hipMemcpyAsync(L.nflag_h, L.nflags.p, sizeof(int), hipMemcpyDeviceToHost, st), "download flag count");
+        LH(hipEventRecord(L.nflag_ev, st), "record event");
+        L.nflag_wait = true;
     }
+    return NB_OK;
+}
+
+// The flag count of the last predict(count = true), into L.nflag.
+nb_status take_flag_count(Learned& L) {
+    if (!L.nflag_wait) return NB_OK;
+    LH(hipEventSynchronize(L.nflag_ev), "hipEventSynchronize");
+    L.nflag_wait = false;
+    L.nflag = std::min<int>(std::max(*L.nflag_h, 0), (int)L.n);
     return NB_OK;
 }
 
@@ -759,6 +774,8 @@ nb_status kick_drift(nb_ctx* c, const Schedule& q) {
 void destroy(Learned* l) {
     if (!l) return;
     l->near.release();
+    if (l->nflag_ev) (void)hipEventDestroy(l->nflag_ev);
+    if (l->nflag_h) (void)hipHostFree(l->nflag_h);
     for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
                       &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub, &l->frame,
                       &l->pred, &l->chk, &l->mask, &l->held, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc,
@@ -819,15 +836,18 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 if (nb_status e = push_history(c, L)) return e;
                 L.since_exact = 0;
             } else {
+                // the extrapolation first: its flag count travels to the host
+                // while the near field runs
                 if (L.rc > 0) {
-                    if (nb_status e = near_field(c, L, eps)) return e;
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, true, L.net_ok)) return e;
+                    if (nb_status e = near_field(c, L, eps)) return e;
                     hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.pred.p,
                                        (const P4*)L.nearacc.p, (long)n, nbh::ctx_acc(c));
                     LH(hipGetLastError(), "add kernel");
                 } else {
                     if (nb_status e = predict(c, L, nbh::ctx_acc(c), true, L.net_ok)) return e;
                 }
+                if (nb_status e = take_flag_count(L)) return e;
                 if (nb_status e = exact_subset(c, L, eps, L.exact_kform)) return e;
                 L.st.predicted_evals++;
                 if (L.pred_net) L.st.net_evals++;

@@ -11,9 +11,10 @@ namespace nbn {
 
 struct Near {
     nbh::DevBuf keys_in, keys_out, idx_in, idx_out, sorted, temp, scan_temp, cols, chunks;
+    int resident[4] = {0, 0, 0, 0};   // persistent grid of each k_near variant (0: not yet queried)
 
     // out[i] = sum over j with 0 < r_ij < rc of m_j S(r_ij) d_ij / (r_ij^2 + eps^2)^(3/2);
-    // synchronises the stream once (to size the launch)
+    // enqueued on st without synchronising it
     // fp32: FP32 pair arithmetic on FP64 differences (the mixed form's precision)
     hipError_t near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk::P4* out, hipStream_t st,
                           bool fp32 = false);

@@ -21,7 +21,9 @@
 // 64 particles of one dense column (kDense or more particles), or the
 // particles of consecutive sparser columns of one group of kGroup columns
 // (same x, y / kGroup equal), merged greedily up to 64, so sparse regions do
-// not cost one launch slot per column.  A block of five waves
+// not cost one launch slot per column.  The blocks the device holds at once
+// take chunks from a work counter (no chunk count travels to the host); a
+// block of five waves
 // takes a chunk spanning columns (x0, ys..ye) and z cells zmin..zmax: wave w
 // sums over the columns x0 - 2 + w, ys - 2..ye + 2, each restricted by binary
 // search to the z cells zmin - 2..zmax + 2, streaming those runs through LDS;
@@ -41,6 +43,8 @@
 // 4.19 rc^3 of the cut-off sphere, so at most 27% of the examined pairs
 // contribute.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -133,13 +137,17 @@ __global__ __launch_bounds__(BLK) void k_chunk_flags(const int2* __restrict__ co
     }
 }
 
+// The chunk list, its length and the work counter k_near takes chunks from
+// (reset here, so no fill is needed).
 __global__ __launch_bounds__(BLK) void k_chunk_list(const int* __restrict__ flag, const int* __restrict__ pos, int n,
-                                                    int* __restrict__ chunks, int* __restrict__ nchunks) {
+                                                    int* __restrict__ chunks, int* __restrict__ nchunks,
+                                                    int* __restrict__ next) {
     const int p = blockIdx.x * BLK + threadIdx.x;
     if (p >= n) return;
     if (flag[p]) chunks[pos[p]] = p;
     if (p == n - 1) {
         *nchunks = pos[p] + flag[p];
+        *next = 0;
         chunks[pos[p] + flag[p]] = n;   // sentinel: the end of the last chunk
     }
 }
@@ -256,99 +264,116 @@ __device__ __forceinline__ void near_pair32(const nbk::P4& me, const nbk::P4& o,
     az = __builtin_fmaf(f, dz, az);
 }
 
+// A resident grid of blocks takes the chunks in list order from the work
+// counter `next` until the count `*nchunks` is reached (every block leaves
+// then), so the launch needs no count from the host; a chunk's sums do not
+// depend on which block takes it.
 template <bool F32, bool EPS0>
 __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict__ sorted,
                                                       const unsigned long long* __restrict__ keys,
                                                       const int* __restrict__ order, const int2* __restrict__ cols,
-                                                      const int* __restrict__ chunks, int n, double rc2,
-                                                      Taper tp, double eps2, nbk::P4* __restrict__ out) {
+                                                      const int* __restrict__ chunks,
+                                                      const int* __restrict__ nchunks, int* __restrict__ next,
+                                                      int n, double rc2, Taper tp, double eps2,
+                                                      nbk::P4* __restrict__ out) {
     __shared__ nbk::P4 stage[kWaves][64];
     __shared__ nbk::P4 part[kWaves][64];
+    __shared__ int taken;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int begin = chunks[blockIdx.x];
-    const int cnt = chunks[blockIdx.x + 1] - begin;   // 1..64
-    const int i = begin + (lane < cnt ? lane : 0);
-    const nbk::P4 me = sorted[i];
-    const unsigned long long ki = keys[i];
-    const int x0 = (int)(ki >> 32);   // one x for the whole chunk
-    const int ys = wave_min((int)((ki >> 21) & (kColSide - 1))), ye = wave_max((int)((ki >> 21) & (kColSide - 1)));
-    const int zlo = max(wave_min(z_of(ki)) - kReach, 0), zhi = min(wave_max(z_of(ki)) + kReach, kZMax);
-    const int X = x0 - kReach + w;
-    const int ny = ye - ys + 1 + 2 * kReach;   // <= kGroup + 2 kReach <= 64
-    // lanes 0..ny-1 locate the z-runs of columns (X, ys - 2 .. ye + 2)
-    int a = 0, b = 0;
-    if (lane < ny && X >= 0 && X < kColSide) {
-        const int Y = ys - kReach + lane;
-        if (Y >= 0 && Y < kColSide) {
-            const int2 run = cols[X * kColSide + Y];
-            a = lower_bound(keys, run.x, run.y, pack(X, Y, zlo));
-            b = lower_bound(keys, a, run.y, pack(X, Y, zhi) + 1);
-        }
-    }
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    // bounding box of the chunk's particles (lanes past the chunk repeat its first)
-    const double3 lo = make_double3(wave_min(me.x), wave_min(me.y), wave_min(me.z));
-    const double3 hi = make_double3(wave_max(me.x), wave_max(me.y), wave_max(me.z));
-    nbk::P4* L = stage[w];
-    for (int k = 0; k < ny; k++) {
-        const int ak = __shfl(a, k, 64), bk = __shfl(b, k, 64);
-        for (int j0 = ak; j0 < bk; j0 += 64) {
-            // keep only the particles within rc of the chunk's bounding box,
-            // packed in order to the front of the stage
-            const int jj = j0 + lane;
-            nbk::P4 J = nbk::P4{0.0, 0.0, 0.0, 0.0};
-            bool use = false;
-            if (jj < bk) {
-                J = sorted[jj];
-                const double ex = fmax(fmax(lo.x - J.x, J.x - hi.x), 0.0);
-                const double ey = fmax(fmax(lo.y - J.y, J.y - hi.y), 0.0);
-                const double ez = fmax(fmax(lo.z - J.z, J.z - hi.z), 0.0);
-                use = ex * ex + ey * ey + ez * ez < rc2;
+    const int nch = *nchunks;
+    for (;;) {
+        if (threadIdx.x == 0) taken = atomicAdd(next, 1);
+        __syncthreads();
+        const int ch = taken;   // the same for the whole block
+        if (ch >= nch) break;
+        const int begin = chunks[ch];
+        const int cnt = chunks[ch + 1] - begin;   // 1..64
+        const int i = begin + (lane < cnt ? lane : 0);
+        const nbk::P4 me = sorted[i];
+        const unsigned long long ki = keys[i];
+        const int x0 = (int)(ki >> 32);   // one x for the whole chunk
+        const int ys = wave_min((int)((ki >> 21) & (kColSide - 1)));
+        const int ye = wave_max((int)((ki >> 21) & (kColSide - 1)));
+        const int zlo = max(wave_min(z_of(ki)) - kReach, 0), zhi = min(wave_max(z_of(ki)) + kReach, kZMax);
+        const int X = x0 - kReach + w;
+        const int ny = ye - ys + 1 + 2 * kReach;   // <= kGroup + 2 kReach <= 64
+        // lanes 0..ny-1 locate the z-runs of columns (X, ys - 2 .. ye + 2)
+        int a = 0, b = 0;
+        if (lane < ny && X >= 0 && X < kColSide) {
+            const int Y = ys - kReach + lane;
+            if (Y >= 0 && Y < kColSide) {
+                const int2 run = cols[X * kColSide + Y];
+                a = lower_bound(keys, run.x, run.y, pack(X, Y, zlo));
+                b = lower_bound(keys, a, run.y, pack(X, Y, zhi) + 1);
             }
-            const unsigned long long keep = __ballot(use);
-            const int nkeep = __popcll(keep);
-            if (nkeep == 0) continue;   // wave-uniform
-            const int m4 = (nkeep + 3) & ~3;   // fillers pad to a multiple of 4
-            if (use) L[__popcll(keep & ((1ull << lane) - 1ull))] = F32 ? stage32(J) : stage64(J);
-            // filler: this lane's position with zero mass, so it adds nothing
-            // to any lane (d = 0 for this one, m = 0 for the others)
-            if (lane >= nkeep && lane < m4) L[lane] = nbk::P4{me.x, me.y, me.z, 0.0};
-            wave_sync();
-            if constexpr (F32) {
-                const float k1 = (float)tp.k1, k0 = (float)tp.k0, e2 = (float)eps2;
-                float fx = 0.f, fy = 0.f, fz = 0.f;
-                for (int t = 0; t < m4; t += 4) {
-                    nbk::P4 o[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) o[q] = L[t + q];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) near_pair32<EPS0>(me, o[q], k1, k0, e2, fx, fy, fz);
+        }
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        // bounding box of the chunk's particles (lanes past the chunk repeat its first)
+        const double3 lo = make_double3(wave_min(me.x), wave_min(me.y), wave_min(me.z));
+        const double3 hi = make_double3(wave_max(me.x), wave_max(me.y), wave_max(me.z));
+        nbk::P4* L = stage[w];
+        for (int k = 0; k < ny; k++) {
+            const int ak = __shfl(a, k, 64), bk = __shfl(b, k, 64);
+            for (int j0 = ak; j0 < bk; j0 += 64) {
+                // keep only the particles within rc of the chunk's bounding box,
+                // packed in order to the front of the stage
+                const int jj = j0 + lane;
+                nbk::P4 J = nbk::P4{0.0, 0.0, 0.0, 0.0};
+                bool use = false;
+                if (jj < bk) {
+                    J = sorted[jj];
+                    const double ex = fmax(fmax(lo.x - J.x, J.x - hi.x), 0.0);
+                    const double ey = fmax(fmax(lo.y - J.y, J.y - hi.y), 0.0);
+                    const double ez = fmax(fmax(lo.z - J.z, J.z - hi.z), 0.0);
+                    use = ex * ex + ey * ey + ez * ez < rc2;
                 }
-                ax += fx;
-                ay += fy;
-                az += fz;
-            } else {
-                for (int t = 0; t < m4; t += 4) {
-                    nbk::P4 o[4];
+                const unsigned long long keep = __ballot(use);
+                const int nkeep = __popcll(keep);
+                if (nkeep == 0) continue;   // wave-uniform
+                const int m4 = (nkeep + 3) & ~3;   // fillers pad to a multiple of 4
+                if (use) L[__popcll(keep & ((1ull << lane) - 1ull))] = F32 ? stage32(J) : stage64(J);
+                // filler: this lane's position with zero mass, so it adds nothing
+                // to any lane (d = 0 for this one, m = 0 for the others)
+                if (lane >= nkeep && lane < m4) L[lane] = nbk::P4{me.x, me.y, me.z, 0.0};
+                wave_sync();
+                if constexpr (F32) {
+                    const float k1 = (float)tp.k1, k0 = (float)tp.k0, e2 = (float)eps2;
+                    float fx = 0.f, fy = 0.f, fz = 0.f;
+                    for (int t = 0; t < m4; t += 4) {
+                        nbk::P4 o[4];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) o[q] = L[t + q];
+                        for (int q = 0; q < 4; q++) o[q] = L[t + q];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) near_pair<EPS0>(me, o[q], tp.k1, tp.k0, eps2, ax, ay, az);
+                        for (int q = 0; q < 4; q++) near_pair32<EPS0>(me, o[q], k1, k0, e2, fx, fy, fz);
+                    }
+                    ax += fx;
+                    ay += fy;
+                    az += fz;
+                } else {
+                    for (int t = 0; t < m4; t += 4) {
+                        nbk::P4 o[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) o[q] = L[t + q];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) near_pair<EPS0>(me, o[q], tp.k1, tp.k0, eps2, ax, ay, az);
+                    }
                 }
+                wave_sync();
             }
-            wave_sync();
         }
-    }
-    part[w][lane] = nbk::P4{ax, ay, az, 0.0};
-    __syncthreads();
-    if (w == 0 && lane < cnt) {
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        for (int q = 0; q < kWaves; q++) {
-            sx += part[q][lane].x;
-            sy += part[q][lane].y;
-            sz += part[q][lane].z;
+        part[w][lane] = nbk::P4{ax, ay, az, 0.0};
+        __syncthreads();
+        if (w == 0 && lane < cnt) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int q = 0; q < kWaves; q++) {
+                sx += part[q][lane].x;
+                sy += part[q][lane].y;
+                sz += part[q][lane].z;
+            }
+            out[order[begin + lane]] = nbk::P4{sx, sy, sz, 0.0};
         }
-        out[order[begin + lane]] = nbk::P4{sx, sy, sz, 0.0};
+        // the next chunk's part[] and `taken` are written only after every wave
+        // has passed the barrier above and wave 0 has read part[]
     }
 }
 
@@ -379,10 +404,11 @@ hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk
                                   (const int*)idx_in.p, (int*)idx_out.p, (unsigned int)n, 0, 43, st);
     if (e != hipSuccess) return e;
     const unsigned long long* keys = (const unsigned long long*)keys_out.p;
-    int* flag = (int*)chunks.p;              // n flags, then n + 1 scan results, the list (n + 1) and the count
+    int* flag = (int*)chunks.p;   // n flags, then n + 1 scan results, the list (n + 1), the count, the counter
     int* pos = flag + n;
     int* list = pos + n + 1;
     int* nchunks = list + n + 1;
+    int* next = nchunks + 1;
     hipLaunchKernelGGL(k_gather_sorted, grid, dim3(BLK), 0, st, pm, (const int*)idx_out.p, n, (nbk::P4*)sorted.p);
     hipLaunchKernelGGL(k_clear_cols, dim3((ncols + BLK - 1) / BLK), dim3(BLK), 0, st, (int2*)cols.p, ncols);
     hipLaunchKernelGGL(k_col_runs, grid, dim3(BLK), 0, st, keys, n, (int2*)cols.p);
@@ -397,17 +423,24 @@ hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk
     if ((e = scan_temp.ensure(sbytes + 16)) != hipSuccess) return e;
     e = rocprim::exclusive_scan(scan_temp.p, sbytes, (const int*)flag, pos, 0, (size_t)n, rocprim::plus<int>(), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_chunk_list, grid, dim3(BLK), 0, st, (const int*)flag, (const int*)pos, n, list, nchunks);
+    hipLaunchKernelGGL(k_chunk_list, grid, dim3(BLK), 0, st, (const int*)flag, (const int*)pos, n, list, nchunks,
+                       next);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    int count = 0;
-    if ((e = hipMemcpyAsync(&count, nchunks, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-    if (count <= 0 || count > n) return hipErrorUnknown;
+    const int variant = (fp32 ? 2 : 0) + (eps > 0.0 ? 0 : 1);
     auto kernel = fp32 ? (eps > 0.0 ? k_near<true, false> : k_near<true, true>)
                        : (eps > 0.0 ? k_near<false, false> : k_near<false, true>);
-    hipLaunchKernelGGL(kernel, dim3(count), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
-                       (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, n, rc * rc,
-                       make_taper(rc, eps), eps * eps, out);
+    if (resident[variant] <= 0) {   // blocks the device holds at once: the persistent grid
+        int dev = 0, ncu = 0, per_cu = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kWaves, 0)) != hipSuccess)
+            return e;
+        resident[variant] = std::max(1, ncu * std::max(per_cu, 1));
+    }
+    const int blocks = std::min(resident[variant], n);   // never more chunks than particles
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
+                       (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, (const int*)nchunks, next, n,
+                       rc * rc, make_taper(rc, eps), eps * eps, out);
     return hipGetLastError();
 }
 
