@@ -23,11 +23,10 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ unsigned short to_bf16(float f) {   // round to nearest even
-    unsigned int u = __builtin_bit_cast(unsigned int, f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0x007fffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+// Round to nearest even: the conversion compiles to gfx950's
+// v_cvt_pk_bf16_f32 (NaN stays NaN).
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
@@ -47,7 +46,9 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// v_exp_f32 and v_rcp_f32 (1 ulp) instead of an IEEE division: every value
+// computed from it enters a matrix product as a bf16 operand (8 bits).
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_d(float z) {
     const float s = sigmoid(z);
@@ -125,25 +126,29 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(Net net, const float* __restric
 // at LDS latency, instead of from memory for every MFMA (144 dependent-latency
 // loads per 16-particle tile at one wave per SIMD).  An empty asm with a
 // memory clobber at the top of the tile loop keeps the compiler from hoisting
-// all 144 tiles into registers (which would cost the occupancy).
+// all 144 tiles into registers (which would cost the occupancy).  Workgroups
+// of kInferWaves = 6 waves: two per CU fill the three waves per SIMD that the
+// kernel's 132 registers allow.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mlp_infer(Net net, const float* __restrict__ X, int rows,
-                                                   float* __restrict__ Y) {
+constexpr int kInferWaves = 6;
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(Net net, const float* __restrict__ X, int rows,
+                                                                float* __restrict__ Y) {
     __shared__ uint2 wt[TILES * 64];
     __shared__ float bias[3 * 128 + 16];
     {
         const uint4* src = (const uint4*)net.Wtile;
         uint4* dst = (uint4*)wt;
-        for (int k = threadIdx.x; k < TILES * 32; k += 256) dst[k] = src[k];
-        for (int k = threadIdx.x; k < 3 * 128 + 16; k += 256)
+        for (int k = threadIdx.x; k < TILES * 32; k += 64 * kInferWaves) dst[k] = src[k];
+        for (int k = threadIdx.x; k < 3 * 128 + 16; k += 64 * kInferWaves)
             bias[k] = k < 128 ? net.b1[k] : k < 256 ? net.b2[k - 128] : k < 384 ? net.b3[k - 256] : net.b4[k - 384];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
     const int tiles = (rows + 15) / 16;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * 4;
+    const int wave = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * kInferWaves;
     for (int tile = wave; tile < tiles; tile += nwaves) {
         asm volatile("" ::: "memory");
         auto A = [&](int t) { return __builtin_bit_cast(bf16x4, wt[t * 64 + lane]); };
@@ -391,8 +396,8 @@ hipError_t forward(const Net& net, const float* X, int rows, float* Y, bool save
     if (rows <= 0) return hipSuccess;
     if (!save) {
         const int tiles = (rows + 15) / 16;
-        hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min(512, (tiles + 3) / 4))), dim3(256), 0, st, net, X,
-                           rows, Y);
+        const int blocks = std::max(1, std::min(512, (tiles + kInferWaves - 1) / kInferWaves));
+        hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * kInferWaves), 0, st, net, X, rows, Y);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_mlp_fwd, dim3(grid_for(rows)), dim3(256), 0, st, net, X, rows, Y, 1, Z, H, ld);

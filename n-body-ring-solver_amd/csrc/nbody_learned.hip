@@ -498,7 +498,7 @@ nb_status ensure_particles(Learned& L, size_t n) {
     LH(L.cub.ensure(m * sizeof(P4)), "allocate extrapolation");
     LH(L.frame.ensure(m * 3 * sizeof(float4)), "allocate output frames");
     LH(L.pred.ensure(m * sizeof(P4)), "allocate prediction");
-    LH(L.chk.ensure(((m + BLK - 1) / BLK) * 3 * sizeof(double)), "allocate check sums");
+    LH(L.chk.ensure(((m + BLK - 1) / BLK) * 6 * sizeof(double)), "allocate check sums");
     LH(L.mask.ensure(m), "allocate roughness mask");
     LH(L.held.ensure(m), "allocate holds");
     L.clear_held = true;
@@ -636,26 +636,37 @@ nb_status exact_subset(nb_ctx* c, Learned& L, double eps, int kform) {
     return NB_OK;
 }
 
-// Compare the extrapolation in L.pred with the exact acc over the smooth
-// particles; update stats and the fallback state.  Synchronises the stream.
-nb_status check_sums(nb_ctx* c, Learned& L, const P4* pred, double& e2, double& a2, double& mx) {
+// Compare the extrapolations pred[0..sets) with the exact acc over the
+// smooth particles: per set, the squared error, the squared exact value and
+// the largest relative error, summed on the host from per-block sums.  One
+// synchronisation of the stream for all sets.
+struct CheckSums {
+    double e2 = 0.0, a2 = 0.0, mx = 0.0;
+};
+
+nb_status check_sums(nb_ctx* c, Learned& L, const P4* const* pred, int sets, CheckSums* out) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
     const long nb = (n + BLK - 1) / BLK;
-    hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, pred, (const P4*)nbh::ctx_acc(c),
-                       L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, (const unsigned char*)L.mask.p, n,
-                       (double*)L.chk.p);
-    LH(hipGetLastError(), "check kernel");
-    L.chk_h.resize(3 * nb);
-    LH(hipMemcpyAsync(L.chk_h.data(), L.chk.p, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, st), "download check");
+    for (int k = 0; k < sets; k++) {
+        hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, pred[k], (const P4*)nbh::ctx_acc(c),
+                           L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, (const unsigned char*)L.mask.p, n,
+                           (double*)L.chk.p + 3 * nb * k);
+        LH(hipGetLastError(), "check kernel");
+    }
+    L.chk_h.resize(3 * nb * sets);
+    LH(hipMemcpyAsync(L.chk_h.data(), L.chk.p, 3 * nb * sets * sizeof(double), hipMemcpyDeviceToHost, st),
+       "download check");
     LH(hipStreamSynchronize(st), "hipStreamSynchronize");
-    e2 = 0.0;
-    a2 = 0.0;
-    mx = 0.0;
-    for (long b = 0; b < nb; b++) {
-        e2 += L.chk_h[3 * b];
-        a2 += L.chk_h[3 * b + 1];
-        mx = std::max(mx, L.chk_h[3 * b + 2]);
+    for (int k = 0; k < sets; k++) {
+        const double* h = L.chk_h.data() + 3 * nb * k;
+        CheckSums r;
+        for (long b = 0; b < nb; b++) {
+            r.e2 += h[3 * b];
+            r.a2 += h[3 * b + 1];
+            r.mx = std::max(r.mx, h[3 * b + 2]);
+        }
+        out[k] = r;
     }
     return NB_OK;
 }
@@ -669,14 +680,15 @@ nb_status check(nb_ctx* c, Learned& L) {
     const long n = (long)L.n;
     // nothing checked (every particle rough or held, or all at rest) validates nothing
     auto rel = [](double e2, double a2) { return a2 > 0 ? std::sqrt(e2 / a2) : INFINITY; };
-    double e2c, a2, mxc, mxn = INFINITY;
-    if (nb_status e = check_sums(c, L, (const P4*)L.cub.p, e2c, a2, mxc)) return e;
-    const double err_c = rel(e2c, a2);
-    double err_n = INFINITY;
+    const P4* sets[2] = {(const P4*)L.cub.p, (const P4*)L.pred.p};
+    CheckSums r[2];
+    if (nb_status e = check_sums(c, L, sets, L.pred_net ? 2 : 1, r)) return e;
+    const double a2 = r[0].a2, mxc = r[0].mx;
+    const double err_c = rel(r[0].e2, a2);
+    double err_n = INFINITY, mxn = INFINITY;
     if (L.pred_net) {
-        double e2n, a2n;
-        if (nb_status e = check_sums(c, L, (const P4*)L.pred.p, e2n, a2n, mxn)) return e;
-        err_n = rel(e2n, a2n);
+        err_n = rel(r[1].e2, r[1].a2);
+        mxn = r[1].mx;
     }
     L.net_ok = L.pred_net && (err_n <= L.threshold);   // NaN fails
     const double err = L.net_ok ? err_n : err_c;
