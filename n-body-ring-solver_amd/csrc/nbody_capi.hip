@@ -22,12 +22,7 @@ using nbk::P4;
 
 namespace {
 
-thread_local std::string g_err;
-
-nb_status fail(nb_status s, const std::string& msg) {
-    g_err = msg;
-    return s;
-}
+nb_status fail(nb_status s, const std::string& msg) { return nbh::set_error(s, msg); }
 
 nb_status hip_fail(hipError_t e, const char* what) {
     return fail(e == hipErrorOutOfMemory ? NB_ERR_ALLOC : NB_ERR_DEVICE,
@@ -369,8 +364,6 @@ using namespace nbh;
 
 extern "C" {
 
-const char* nb_last_error(void) { return g_err.c_str(); }
-
 const char* nb_version(void) { return "nbody-mi355x 0.1 (gfx950)"; }
 
 nb_status nb_create(nb_ctx** out, int device) {
@@ -397,7 +390,7 @@ nb_status nb_create(nb_ctx** out, int device) {
     }
     for (auto& ev : c->step_ev) (void)hipEventCreate(&ev);
     *out = c;
-    g_err.clear();
+    nbh::clear_error();
     return NB_OK;
 }
 
@@ -873,6 +866,3 @@ nb_status nb_synchronize(nb_ctx* c) {
 
 }  // extern "C"
 
-namespace nbh {
-nb_status set_error(nb_status s, const std::string& msg) { return fail(s, msg); }
-}  // namespace nbh

@@ -44,7 +44,8 @@ struct ForceOut {
 };
 
 // Context internals used by the learned-form module.
-nb_status set_error(nb_status s, const std::string& msg);
+nb_status set_error(nb_status s, const std::string& msg);   // nbody_status.cpp: text for nb_last_error()
+void clear_error();
 int choose_splits(const nb_ctx* c, long ni, long nj, int ipt);
 int kernel_form(int form);
 hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out);
