@@ -12,8 +12,11 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("kind,n,rc,eps", [("plummer", 3000, 0.1, 0.01), ("plummer", 3000, 0.4, 0.0),
-                                           ("sphere", 2049, 0.25, 1e-3), ("plummer", 700, 100.0, 0.01)])
+                                           ("sphere", 2049, 0.25, 1e-3), ("plummer", 700, 100.0, 0.01),
+                                           ("plummer", 20000, 0.05, 0.01)])
 def test_near_accel_vs_oracle(solver, orc, kind, n, rc, eps):
+    # (20000, rc = 0.05) forms ~4100 chunks, more than the resident grid holds
+    # (4 blocks per CU on 256 CUs), so blocks take several chunks each
     m, pos, vel, _ = nbody_amd.init_system(kind, n, 3)
     solver.set_state(m, pos, vel)
     a = solver.near_accel(rc, eps)
@@ -26,11 +29,12 @@ def test_near_accel_vs_oracle(solver, orc, kind, n, rc, eps):
         assert np.max(np.abs(a - full)) < 1e-12 * np.max(np.abs(full))
 
 
-@pytest.mark.parametrize("rc", [0.1, 0.3])
-def test_near_accel_mixed_vs_oracle(solver, orc, rc):
+@pytest.mark.parametrize("rc,n", [(0.1, 4000), (0.3, 4000), (0.1, 40000)])
+def test_near_accel_mixed_vs_oracle(solver, orc, rc, n):
     """FP32 pair arithmetic on FP64 differences: within 1e-5 of the largest
-    near-field acceleration (FP32 rounding of ~1e3 terms)."""
-    m, pos, vel, _ = nbody_amd.init_system("plummer", 4000, 8)
+    near-field acceleration (FP32 rounding of ~1e3 terms).  N = 40000 at
+    rc = 0.1 forms ~3300 chunks (several per resident block)."""
+    m, pos, vel, _ = nbody_amd.init_system("plummer", n, 8)
     solver.set_state(m, pos, vel)
     a = solver.near_accel(rc, 0.005, form="mixed")
     ao = orc.near_accel(m, pos, rc, 0.005)
