@@ -8,14 +8,18 @@
 // time and is what the learned form extrapolates.  A polynomial in r^2 needs
 // no square root per pair.
 //
-// Layout: cells of edge rc/2.  A cell (x, y, z) has the key
+// Layout: columns of cross-section (rc/2)^2, cut along z into slabs of
+// height rc/8 (kZFine = 4 slabs per rc/2).  A cell (x, y, z) has the key
 // x << 32 | y << 21 | z (x, y biased by 1024 and clamped to [0, 2048), z by
 // 2^20 and clamped to [0, 2^21)), so the 2^22 (x, y) columns are the keys'
 // top bits and each column's cells are contiguous in key order.  Particles
 // are radix-sorted by key (stable: inside a cell, index order); a table gives
-// every column's run [start, end) of the sorted array.  Clamping only merges
-// far cells, and two particles closer than rc stay within two cells of each
-// other after it, so results stay exact (only slower for far-flung systems).
+// every column's run [start, end) of the sorted array.  The thin z slabs
+// order a dense column's particles by height, so its 64-particle chunks are
+// compact (about 10 % fewer examined pairs than index order inside rc/2
+// cells).  Clamping only merges far cells, and two particles closer than rc
+// stay within two columns and eight slabs of each other after it, so results
+// stay exact (only slower for far-flung systems).
 //
 // Work: a chunk is a run of consecutive sorted particles, one per lane: up to
 // 64 particles of one dense column (kDense or more particles), or the
@@ -24,9 +28,9 @@
 // not cost one launch slot per column.  The blocks the device holds at once
 // take chunks from a work counter (no chunk count travels to the host); a
 // block of five waves
-// takes a chunk spanning columns (x0, ys..ye) and z cells zmin..zmax: wave w
+// takes a chunk spanning columns (x0, ys..ye) and z slabs zmin..zmax: wave w
 // sums over the columns x0 - 2 + w, ys - 2..ye + 2, each restricted by binary
-// search to the z cells zmin - 2..zmax + 2, streaming those runs through LDS;
+// search to the z slabs zmin - 8..zmax + 8, streaming those runs through LDS;
 // the five partial sums are added in wave order (bitwise reproducible).  Of
 // each 64-particle batch of a run only the particles within rc of the chunk's
 // bounding box are kept (ballot + prefix count, order preserved), so far
@@ -61,7 +65,9 @@ constexpr int kColSide = 1 << 11;         // columns per side of the (x, y) tabl
 constexpr int kColBias = kColSide / 2;
 constexpr int kZBias = 1 << 20;
 constexpr int kZMax = (1 << 21) - 1;
-constexpr int kReach = 2;                 // neighbour cells each way (cell edge rc/2)
+constexpr int kReach = 2;                 // neighbour columns each way (column edge rc/2)
+constexpr int kZFine = 4;                 // z slabs per rc/2: slab height rc/8
+constexpr int kZReach = kReach * kZFine;  // neighbour slabs each way
 constexpr int kDense = 32;                // particles from which a column gets chunks of its own
 constexpr int kGroup = 16;                // sparser columns are merged within groups of 16 along y
 
@@ -71,7 +77,7 @@ __device__ __forceinline__ int cell_xy(double x, double inv_h) {
 }
 
 __device__ __forceinline__ int cell_z(double z, double inv_h) {
-    const double c = floor(z * inv_h) + (double)kZBias;
+    const double c = floor(z * (inv_h * kZFine)) + (double)kZBias;
     return (int)fmin(fmax(c, 0.0), (double)kZMax);
 }
 
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
         const int x0 = (int)(ki >> 32);   // one x for the whole chunk
         const int ys = wave_min((int)((ki >> 21) & (kColSide - 1)));
         const int ye = wave_max((int)((ki >> 21) & (kColSide - 1)));
-        const int zlo = max(wave_min(z_of(ki)) - kReach, 0), zhi = min(wave_max(z_of(ki)) + kReach, kZMax);
+        const int zlo = max(wave_min(z_of(ki)) - kZReach, 0), zhi = min(wave_max(z_of(ki)) + kZReach, kZMax);
         const int X = x0 - kReach + w;
         const int ny = ye - ys + 1 + 2 * kReach;   // <= kGroup + 2 kReach <= 64
         // lanes 0..ny-1 locate the z-runs of columns (X, ys - 2 .. ye + 2)
