@@ -327,6 +327,20 @@ def main():
     dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
     roofline = rooflines.get(dominant) or rooflines.get("fp64")
 
+    # the boundary takes host planes: time the upload and download around a
+    # run (not part of `value`, which starts with the state resident in HBM)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.set_state(m, pos, vel)
+    s.synchronize()
+    t_up = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.get_state()
+    t_down = time.perf_counter() - t0
+    host_io = {"upload_ms": 1e3 * t_up, "download_ms": 1e3 * t_down,
+               "bytes": int(n) * 7 * 8 + int(n) * 6 * 8,
+               "steps_per_s_with_host_io": args.steps / (hr["wall_s"] + t_up + t_down)}
+
     cpu = None if args.no_cpu else cpu_baseline(state, ic_text)
     line = {
         "metric": METRIC,
@@ -348,6 +362,7 @@ def main():
         "dE_E": hr["dE_E"],
         "forms": forms,
         "training": training,
+        "host_io": host_io,
         "roofline": roofline,
         "rooflines": rooflines,
         "cpu_baseline": cpu,
