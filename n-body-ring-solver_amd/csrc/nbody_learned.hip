@@ -408,6 +408,8 @@ struct Learned {
     int* nflag_h = nullptr;          // pinned host copy of the flag count
     hipEvent_t nflag_ev = nullptr;   // recorded after that copy
     bool nflag_wait = false;         // a copy is in flight: take_flag_count() waits for it
+    hipStream_t side = nullptr;      // the exact substeps' near field runs here, beside the force kernel
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<double> chk_h;
     nb_learned_stats st{};
 };
@@ -507,6 +509,9 @@ nb_status ensure_particles(Learned& L, size_t n) {
     LH(L.nflags.ensure(sizeof(int)), "allocate flag counter");
     if (!L.nflag_h) LH(hipHostMalloc((void**)&L.nflag_h, sizeof(int), hipHostMallocDefault), "allocate pinned count");
     if (!L.nflag_ev) LH(hipEventCreateWithFlags(&L.nflag_ev, hipEventDisableTiming), "create event");
+    if (!L.side) LH(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking), "create stream");
+    if (!L.ev_fork) LH(hipEventCreateWithFlags(&L.ev_fork, hipEventDisableTiming), "create event");
+    if (!L.ev_join) LH(hipEventCreateWithFlags(&L.ev_join, hipEventDisableTiming), "create event");
     L.n = n;
     L.count = 0;
     L.head = 3;
@@ -729,6 +734,27 @@ nb_status near_field(nb_ctx* c, Learned& L, double eps) {
     return NB_OK;
 }
 
+// The near field of the current positions on the side stream, after all work
+// queued so far on the main stream; near_field_join() makes the main stream
+// wait for it.  On an exact substep it runs beside the all-pairs kernel and
+// fills the CUs that kernel's last waves leave idle.
+nb_status near_field_fork(nb_ctx* c, Learned& L, double eps) {
+    if (!(L.rc > 0)) return NB_OK;
+    LH(hipEventRecord(L.ev_fork, nbh::ctx_stream(c)), "record event");
+    LH(hipStreamWaitEvent(L.side, L.ev_fork, 0), "stream wait");
+    LH(L.near.near_accel(nbh::ctx_pm(c), (int)L.n, L.rc, eps, (P4*)L.nearacc.p, L.side,
+                         L.exact_kform != nbk::FORM_FP64),
+       "near-field kernels");
+    LH(hipEventRecord(L.ev_join, L.side), "record event");
+    return NB_OK;
+}
+
+nb_status near_field_join(nb_ctx* c, Learned& L) {
+    if (!(L.rc > 0)) return NB_OK;
+    LH(hipStreamWaitEvent(nbh::ctx_stream(c), L.ev_join, 0), "stream wait");
+    return NB_OK;
+}
+
 // Append this substep's (features, target) pairs: features from the exact
 // samples one to four periods back in the collection ring (R slots, sample t
 // in slot t mod R), target the exact remainder now.
@@ -785,8 +811,12 @@ nb_status kick_drift(nb_ctx* c, const Schedule& q) {
 
 void destroy(Learned* l) {
     if (!l) return;
+    if (l->side) (void)hipStreamSynchronize(l->side);   // its near-field work uses the buffers below
     l->near.release();
     if (l->nflag_ev) (void)hipEventDestroy(l->nflag_ev);
+    if (l->ev_fork) (void)hipEventDestroy(l->ev_fork);
+    if (l->ev_join) (void)hipEventDestroy(l->ev_join);
+    if (l->side) (void)hipStreamDestroy(l->side);
     if (l->nflag_h) (void)hipHostFree(l->nflag_h);
     for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
                       &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub, &l->frame,
@@ -837,8 +867,9 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 if (full) {
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, false, true)) return e;
                 }
+                if (nb_status e = near_field_fork(c, L, eps)) return e;
                 if (nb_status e = exact_force(c, L.exact_kform, eps)) return e;
-                if (nb_status e = near_field(c, L, eps)) return e;
+                if (nb_status e = near_field_join(c, L)) return e;
                 L.st.exact_evals++;
                 // extrapolation starts only after a check has passed, so a
                 // network that produces garbage never reaches the integrator
