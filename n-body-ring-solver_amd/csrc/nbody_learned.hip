@@ -604,7 +604,9 @@ nb_status take_flag_count(Learned& L) {
 }
 
 // Exact sums for the flagged particles only, written into acc.
-nb_status exact_subset(nb_ctx* c, Learned& L, double eps, int kform) {
+// Exact sums of the flagged particles into L.sub_acc; exact_subset_scatter()
+// writes them into acc.
+nb_status exact_subset_compute(nb_ctx* c, Learned& L, double eps, int kform) {
     if (L.nflag <= 0) return NB_OK;
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n, m = L.nflag;
@@ -635,8 +637,14 @@ nb_status exact_subset(nb_ctx* c, Learned& L, double eps, int kform) {
     A.out_stride = m;
     LH(nbk::launch_accel(A, st), "subset force kernel");
     LH(nbk::launch_reduce((const P4*)L.sub_part.p, S, m, (int)m, (P4*)L.sub_acc.p, st), "subset reduce kernel");
-    hipLaunchKernelGGL(k_scatter, blocks(m), dim3(BLK), 0, st, (const P4*)L.sub_acc.p, (const int*)L.flags.p, (int)m,
-                       nbh::ctx_acc(c));
+    return NB_OK;
+}
+
+nb_status exact_subset_scatter(nb_ctx* c, Learned& L) {
+    if (L.nflag <= 0) return NB_OK;
+    const long m = L.nflag;
+    hipLaunchKernelGGL(k_scatter, blocks(m), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.sub_acc.p,
+                       (const int*)L.flags.p, (int)m, nbh::ctx_acc(c));
     LH(hipGetLastError(), "scatter kernel");
     return NB_OK;
 }
@@ -737,7 +745,8 @@ nb_status near_field(nb_ctx* c, Learned& L, double eps) {
 // The near field of the current positions on the side stream, after all work
 // queued so far on the main stream; near_field_join() makes the main stream
 // wait for it.  On an exact substep it runs beside the all-pairs kernel and
-// fills the CUs that kernel's last waves leave idle.
+// fills the CUs that kernel's waves leave idle; on an extrapolated one beside
+// the extrapolation and the exact sums of the flagged particles.
 nb_status near_field_fork(nb_ctx* c, Learned& L, double eps) {
     if (!(L.rc > 0)) return NB_OK;
     LH(hipEventRecord(L.ev_fork, nbh::ctx_stream(c)), "record event");
@@ -879,19 +888,24 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 if (nb_status e = push_history(c, L)) return e;
                 L.since_exact = 0;
             } else {
-                // the extrapolation first: its flag count travels to the host
-                // while the near field runs
+                // the near field runs on the side stream while the main
+                // stream extrapolates, reads the flag count back and sums the
+                // flagged particles exactly
                 if (L.rc > 0) {
+                    if (nb_status e = near_field_fork(c, L, eps)) return e;
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, true, L.net_ok)) return e;
-                    if (nb_status e = near_field(c, L, eps)) return e;
+                    if (nb_status e = take_flag_count(L)) return e;
+                    if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
+                    if (nb_status e = near_field_join(c, L)) return e;
                     hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.pred.p,
                                        (const P4*)L.nearacc.p, (long)n, nbh::ctx_acc(c));
                     LH(hipGetLastError(), "add kernel");
                 } else {
                     if (nb_status e = predict(c, L, nbh::ctx_acc(c), true, L.net_ok)) return e;
+                    if (nb_status e = take_flag_count(L)) return e;
+                    if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
                 }
-                if (nb_status e = take_flag_count(L)) return e;
-                if (nb_status e = exact_subset(c, L, eps, L.exact_kform)) return e;
+                if (nb_status e = exact_subset_scatter(c, L)) return e;
                 L.st.predicted_evals++;
                 if (L.pred_net) L.st.net_evals++;
                 L.st.subset_rows += (uint64_t)L.nflag;
