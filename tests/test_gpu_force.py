@@ -273,3 +273,32 @@ def test_accel_block_four_rows_per_lane(solver, orc):
     assert rel_err(acc.T, ref.T) < FP64_TOL
     accm = solver.accel_block(pi, pj, "mixed", 0.005)
     assert rel_err(accm.T, ref.T) < MIXED_TOL
+
+
+def test_full_size_vs_oracle_sampled(solver, orc):
+    """The bench workload (Plummer N = 262144, eps = 0.005, symmetric kernel):
+    FP64 and mixed accelerations against the oracle on 1024 sampled rows (the
+    oracle sums each sampled row over all N particles)."""
+    n = 262144
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 5, [1.0, 20.0])
+    upload(solver, m, pos)
+    rows = np.random.default_rng(4).choice(n, 1024, replace=False)
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    ref = orc.accel_block(pj[rows], pj, 0.005).T
+    assert rel_err(solver.accel("fp64", 0.005)[:, rows], ref) < FP64_TOL
+    assert rel_err(solver.accel("mixed", 0.005)[:, rows], ref) < MIXED_TOL
+
+
+def test_ring_share_full_size_sampled(solver, orc):
+    """One rank's share of the ring at full size: the first 131072 of a
+    Plummer N = 1048576 against all of them in j-blocks of 131072, FP64 and
+    mixed, against the oracle on 512 sampled rows."""
+    ni, nj = 131072, 1048576
+    m, pos, _, _ = nbody_amd.init_system("plummer", nj, 5, [1.0, 20.0])
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    rows = np.random.default_rng(7).choice(ni, 512, replace=False)
+    ref = orc.accel_block(pj[rows], pj, 0.005).T
+    acc = solver.accel_block(pj[:ni], pj, "fp64", 0.005, j_block=131072)
+    assert rel_err(acc.T[:, rows], ref) < FP64_TOL
+    accm = solver.accel_block(pj[:ni], pj, "mixed", 0.005, j_block=131072)
+    assert rel_err(accm.T[:, rows], ref) < MIXED_TOL
