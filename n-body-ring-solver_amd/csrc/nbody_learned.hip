@@ -408,7 +408,7 @@ struct Learned {
     int* nflag_h = nullptr;          // pinned host copy of the flag count
     hipEvent_t nflag_ev = nullptr;   // recorded after that copy
     bool nflag_wait = false;         // a copy is in flight: take_flag_count() waits for it
-    hipStream_t side = nullptr;      // the exact substeps' near field runs here, beside the force kernel
+    hipStream_t side = nullptr;      // the near field runs here, beside the main stream's work
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<double> chk_h;
     nb_learned_stats st{};
@@ -603,7 +603,6 @@ nb_status take_flag_count(Learned& L) {
     return NB_OK;
 }
 
-// Exact sums for the flagged particles only, written into acc.
 // Exact sums of the flagged particles into L.sub_acc; exact_subset_scatter()
 // writes them into acc.
 nb_status exact_subset_compute(nb_ctx* c, Learned& L, double eps, int kform) {
