@@ -42,10 +42,11 @@
 // FP32 sums over each 64-particle batch, flushed to FP64 -- the precision of
 // the mixed force form.
 //
-// Bound: VALU.  About 30 operations per examined pair; the 5 x 5 x
-// (5 + chunk height) cells around a chunk hold at least 15.6 rc^3 against the
-// 4.19 rc^3 of the cut-off sphere, so at most 27% of the examined pairs
-// contribute.
+// Bound: VALU.  About 26 operations per examined pair (FP32 variant).  With
+// the bounding-box cull about 27 % of the examined pairs lie within rc (a CPU
+// model of this chunking on a Plummer sphere of 262144 at rc = 0.25): 64
+// particles at that density fill a box whose rc-neighbourhood is ~3.4 times
+// the cut-off sphere, whatever its shape.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
