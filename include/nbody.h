@@ -18,7 +18,10 @@
  *
  * Threading: a context is used by one host thread at a time.  Work is ordered
  * on the context's own HIP stream; entry points that return host data
- * synchronise that stream before returning.
+ * synchronise that stream before returning.  nb_step with NB_FORM_LEARNED
+ * also runs the near field on a second stream of the context, forked from
+ * and joined back to the first by events inside every substep, so the
+ * first stream orders everything a caller sees.
  */
 #ifndef NBODY_H
 #define NBODY_H
