@@ -240,6 +240,10 @@ def main():
                     help="initial conditions: Plummer sphere (a = 1, r < 20) or uniform unit sphere at rest")
     ap.add_argument("--forms", default="fp64,mixed,learned,learned-cubic,learned-p8")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--near-radii", default="", metavar="RC,..",
+                    help="only evaluate the learned form's near field of the workload's initial state at these "
+                         "radii (5 times each, mixed then FP64 arithmetic) for a kernel trace, print one JSON "
+                         "line and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (a known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -250,6 +254,18 @@ def main():
     args = ap.parse_args()
     if args.ring_share:
         ring_share(args.eps, args.seed)
+        return
+    if args.near_radii:
+        m, pos, vel, _ = nbody_amd.init_system("plummer", args.n, args.seed, [1.0, 20.0])
+        radii = [float(r) for r in args.near_radii.split(",")]
+        with nbody_amd.Solver(0) as s:
+            s.set_state(m, pos, vel)
+            for rc in radii:
+                for form in ("mixed", "fp64"):
+                    for _ in range(5):
+                        s.near_accel(rc, args.eps, form=form)
+        print(json.dumps({"near_radii": radii, "n": args.n, "eps": args.eps, "calls_per_radius_and_form": 5}),
+              flush=True)
         return
     if args.copy_probe:
         import torch
