@@ -19,17 +19,29 @@ A step is one leapfrog step of the whole system: one force evaluation (N^2
 ordered pairs) plus the kick/drift.  The state is resident in HBM before the
 timed region starts.
 
+Per form, K steps (--steps) are timed in the steady state, after W warm-up
+steps (--warmup; a learned form warms up for at least 6 periods and keeps
+what the warm-up built: a full history of exact samples and a network that
+passed its check).  Then each form runs its production run from the initial
+state, cold (--energy-steps, default 1000, the length the energy bound is
+stated for): its |dE/E| decides whether the form meets the bound, and its
+wall-clock rate is the from-cold figure reported beside the steady one.
+
 Prints ONE JSON line.  `value` is the effective throughput in GPairs/s
-(N^2 pairs per simulated step x steps / wall time) of the fastest of the
-three forms fp64, mixed and learned whose energy drift |dE/E| over the run
-stays <= 1e-6; `forms` holds every form's
-GPairs/s, steps/s, |dE/E|, speed-up over FP64 and force error (max relative
-against FP64 for fp64/mixed; RMS and max relative at the last check for the
-learned forms).  `roofline` is the dominant kernel of the headline form,
-timed with HIP events around every launch on the solver's stream;
-`rooflines` adds the others.  `cpu_baseline` times the reference's own
-AVX-512 force kernel (compiled from its sources) on this host's cores on a
-bounded sample.
+(N^2 pairs per simulated step x steps / wall time, whether the pairs were
+evaluated or, on a learned form's extrapolated substeps, not) of the fastest
+of the three forms fp64, mixed and learned whose production run keeps
+|dE/E| <= 1e-6; steps/s and the speed-up over FP64 are the plain figures.
+`forms` holds every form's GPairs/s, steps/s, |dE/E|, speed-up over FP64 and
+force error (max relative against FP64 for fp64/mixed; RMS and max relative
+at the last check for the learned forms), the learned forms' counters
+(exact and extrapolated substeps, network uses, exactly summed rows) and the
+pairs they evaluated directly.  `network_gain` compares the learned form
+with its cubic-only ablation.  `roofline` is the dominant kernel of the
+headline form as the library reports it, timed with HIP events around every
+launch on the solver's stream; `rooflines` adds the others.  `cpu_baseline`
+times the reference's own AVX-512 force kernel (compiled from its sources)
+on this host's cores on a bounded sample.
 """
 from __future__ import annotations
 
@@ -85,12 +97,21 @@ def learned_config(form):
 
 
 def configure_learned(s, form):
+    """Configure the learned form (this also restarts its history and zeroes its counters)."""
     cfg = learned_config(form)
     s.learned_configure(period=cfg["period"], threshold=cfg["threshold"], exact_form=cfg["exact_form"])
     s.learned_set_near_radius(cfg["rc"])
     s.learned_set_flag_ratio(cfg["rho"])
     s.learned_set_hold_ratio(cfg["kappa"])
     s.learned_set_mode(0 if form == "learned" else 1)
+
+
+def learned_warmup(form, warmup):
+    """Warm-up steps of a learned form (order 2: one substep per step): the
+    history holds four exact samples one period apart from substep 4 period on
+    and the network is first applied after it passed the check at substep
+    5 period, so the timed steps start in the steady state from 6 periods on."""
+    return max(warmup, 6 * learned_config(form)["period"])
 
 
 def train_network(s, dt, eps):
@@ -107,20 +128,24 @@ def train_network(s, dt, eps):
     return {"loss_first": loss0, "loss_final": loss, "seconds": time.perf_counter() - t0, **TRAIN}
 
 
-def run_form(s, torch, state, form, steps, warmup, dt, eps):
-    m, pos, vel = state
-    n = m.shape[0]
-    kform = "learned" if form.startswith("learned") else form
-    if form.startswith("learned"):
-        configure_learned(s, form)
-    if warmup:
-        s.set_state(m, pos, vel)
-        s.step(warmup, dt, 2, kform, eps)
-    s.set_state(m, pos, vel)
+COUNTERS = ("exact_evals", "predicted_evals", "fallbacks", "subset_rows", "net_evals", "checks", "net_checks",
+            "net_cubic_err_sum", "net_err_sum")
+
+
+def stats_delta(after, before):
+    """Learned-form counters accumulated between two reads of the stats."""
+    d = dict(after)
+    for k in COUNTERS:
+        d[k] = after[k] - before[k]
+    return d
+
+
+def timed_steps(s, torch, n, steps, dt, kform, eps, learned):
+    """`steps` steps of the resident state, wall-clock timed between device
+    synchronisations, with HIP events around every force launch; energies
+    before and after (outside the timed region)."""
     T0, U0 = s.energy(eps)
-    if form.startswith("learned"):   # counters start at the timed run
-        configure_learned(s, form)
-        s.set_state(m, pos, vel)
+    st0 = s.learned_stats() if learned else None
     s.timing(True)
     s.timing_reset()
     s.synchronize()
@@ -130,11 +155,11 @@ def run_form(s, torch, state, form, steps, warmup, dt, eps):
     s.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    force_ms, evals, step_ms = s.timing_read()
+    force_ms, evals, _ = s.timing_read()
     s.timing(False)
     T1, U1 = s.energy(eps)
     E0, E1 = T0 + U0, T1 + U1
-    return {
+    r = {
         "steps": steps,
         "wall_s": wall,
         "ms_per_step": 1e3 * wall / steps,
@@ -143,7 +168,46 @@ def run_form(s, torch, state, form, steps, warmup, dt, eps):
         "dE_E": abs((E0 - E1) / E0),
         "exact_force_evals": int(evals),
         "force_kernel_ms_avg": force_ms / max(evals, 1),
+        "force_kernel": s.kernel_info() if evals else None,
     }
+    if learned:
+        st = stats_delta(s.learned_stats(), st0)
+        r["learned_stats"] = st
+        # pairs the run evaluated directly: every exact substep's N^2 and the
+        # flagged particles' rows (the near field's pairs are not counted)
+        r["direct_gpairs_per_s"] = (float(n) * n * st["exact_evals"] + float(n) * st["subset_rows"]) / wall / 1e9
+        r["gpairs_counted_as"] = "effective: N^2 ordered pairs per simulated step, evaluated or extrapolated"
+    return r
+
+
+def run_form(s, torch, state, form, steps, warmup, dt, eps, energy_steps):
+    """One form: `steps` timed steps after the warm-up (steady state: a learned
+    form keeps the history and the validated network the warm-up built), then
+    the production run of `energy_steps` steps from the initial state, cold,
+    over which the energy bound is judged."""
+    m, pos, vel = state
+    n = m.shape[0]
+    learned = form.startswith("learned")
+    kform = "learned" if learned else form
+    s.set_state(m, pos, vel)
+    if learned:
+        configure_learned(s, form)
+    w = learned_warmup(form, warmup) if learned else warmup
+    if w:
+        s.step(w, dt, 2, kform, eps)
+    r = timed_steps(s, torch, n, steps, dt, kform, eps, learned)
+    r["warmup_steps"] = w
+    if energy_steps:
+        s.set_state(m, pos, vel)
+        if learned:
+            configure_learned(s, form)
+        cold = timed_steps(s, torch, n, energy_steps, dt, kform, eps, learned)
+        cold.pop("force_kernel", None)
+        r["production_run"] = cold
+        r["energy_bound_met"] = cold["dE_E"] <= DE_BOUND
+    else:
+        r["energy_bound_met"] = r["dE_E"] <= DE_BOUND
+    return r
 
 
 def cpu_baseline(state, ic_text):
@@ -178,22 +242,13 @@ def cpu_baseline(state, ic_text):
             "sample": f"oracle C port, {ni} i-particles x {n} j-particles"}
 
 
-def force_kernel_name(form, n):
-    """The force kernel the library picks for a resident system of n particles."""
-    kf = "FP64" if form == "fp64" else "MIXED_NR"
-    if n < 16384:
-        return f"k_accel<{kf}, IPT=1> (tile)"
-    k = (8 if form == "fp64" else 6) if n >= 131072 else 4
-    return f"k_accel_sym<{kf}, K={k}> (symmetric)"
-
-
-def roofline_entry(name, kernel_ms, n, peak, traffic=None, pairs="N^2 ordered pairs (20 flops each)"):
+def roofline_entry(kinfo, note, kernel_ms, n, peak, traffic=None, pairs="N^2 ordered pairs (20 flops each)"):
     t = kernel_ms * 1e-3
     achieved = FLOPS_PER_PAIR * float(n) * n / t / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": traffic, "kernel": name, "launch_avg_ms": kernel_ms,
+            "traffic": traffic, "kernel": f"{kinfo['name']} ({kinfo['kernel']}; {note})", "launch_avg_ms": kernel_ms,
             "algorithmic_flops_per_launch": FLOPS_PER_PAIR * float(n) * n, "counted_as": pairs,
-            "algorithmic_bytes_per_launch": 64 * n}
+            "algorithmic_bytes_per_launch": 64 * n, "scratch_bytes": kinfo["scratch_bytes"]}
 
 
 def ring_share(eps, seed, ni=131072, nj=1048576, j_block=131072, reps=3):
@@ -232,6 +287,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--energy-steps", type=int, default=1000,
+                    help="length of each form's production run from the initial state, over which |dE/E| is "
+                         "judged (0: judge it over the timed steps)")
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--dt", type=float, default=1.0 / 256)
@@ -244,14 +302,15 @@ def main():
                     help="only evaluate the learned form's near field of the workload's initial state at these "
                          "radii (5 times each, mixed then FP64 arithmetic) for a kernel trace, print one JSON "
                          "line and exit")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a plain device-to-device copy of MIB MiB three times (a known-size "
-                         "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--ring-share", action="store_true",
                     help="only time one rank's share of the reference's ring (131072 i-particles against "
                          "1048576 j-particles in blocks of 131072) for fp64 and mixed, print one JSON line "
                          "and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the ring's sharding over "
+              f"several GPUs is not implemented here", file=sys.stderr)
+        sys.exit(2)
     if args.ring_share:
         ring_share(args.eps, args.seed)
         return
@@ -267,19 +326,6 @@ def main():
         print(json.dumps({"near_radii": radii, "n": args.n, "eps": args.eps, "calls_per_radius_and_form": 5}),
               flush=True)
         return
-    if args.copy_probe:
-        import torch
-        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": src.numel() * 4}), flush=True)
-        return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the ring's sharding over "
-              f"several GPUs is not implemented here", file=sys.stderr)
-        sys.exit(2)
 
     import torch  # only for the device-wide synchronisation around the timed region
 
@@ -297,15 +343,14 @@ def main():
 
     forms, training = {}, None
     for form in args.forms.split(","):
-        log(f"[bench] form {form}: N={n} steps={args.steps} warmup={args.warmup}")
+        log(f"[bench] form {form}: N={n} steps={args.steps} warmup={args.warmup} production={args.energy_steps}")
         try:
             if form == "learned" and training is None:
                 training = train_network(s, args.dt, args.eps)
                 log(f"[bench] learned: trained {json.dumps(training)}")
-            r = run_form(s, torch, state, form, args.steps, args.warmup, args.dt, args.eps)
+            r = run_form(s, torch, state, form, args.steps, args.warmup, args.dt, args.eps, args.energy_steps)
             if form.startswith("learned"):
-                st = s.learned_stats()
-                r["learned_stats"] = st
+                st = r["learned_stats"]
                 r["force_err_rms_at_check"] = st["last_check_err"]
                 r["force_err_max_at_check"] = st["last_check_max"]
                 r["config"] = dict(learned_config(form), mode="cubic + network" if form == "learned" else "cubic only")
@@ -321,27 +366,53 @@ def main():
         for r in forms.values():
             if "steps_per_s" in r:
                 r["speedup_vs_fp64"] = r["steps_per_s"] / forms["fp64"]["steps_per_s"]
+                if "production_run" in r and "production_run" in forms["fp64"]:
+                    r["production_run"]["speedup_vs_fp64"] = (r["production_run"]["steps_per_s"] /
+                                                              forms["fp64"]["production_run"]["steps_per_s"])
 
-    ok = {k: v for k, v in forms.items() if "gpairs_per_s" in v and v["dE_E"] <= DE_BOUND and k in HEADLINE_FORMS}
+    # what the network gains over the cubic alone, same schedule and system
+    network_gain = None
+    if "steps_per_s" in forms.get("learned", {}) and "steps_per_s" in forms.get("learned-cubic", {}):
+        ln, lc = forms["learned"], forms["learned-cubic"]
+        sn, sc = ln["learned_stats"], lc["learned_stats"]
+        network_gain = {
+            "steps_per_s_ratio": ln["steps_per_s"] / lc["steps_per_s"],
+            "check_err_ratio_cubic_over_network": (sn["net_cubic_err_sum"] / sn["net_err_sum"]
+                                                   if sn["net_err_sum"] > 0 else None),
+            "net_checks": sn["net_checks"], "net_evals": sn["net_evals"], "predicted_evals": sn["predicted_evals"],
+            "subset_rows_network": sn["subset_rows"], "subset_rows_cubic": sc["subset_rows"],
+            "fallbacks_network": sn["fallbacks"], "fallbacks_cubic": sc["fallbacks"],
+        }
+        if "production_run" in ln and "production_run" in lc:
+            pn, pc = ln["production_run"], lc["production_run"]
+            network_gain["production"] = {
+                "steps_per_s_ratio": pn["steps_per_s"] / pc["steps_per_s"],
+                "dE_E_network": pn["dE_E"], "dE_E_cubic": pc["dE_E"],
+                "check_err_ratio_cubic_over_network": (pn["learned_stats"]["net_cubic_err_sum"] /
+                                                       pn["learned_stats"]["net_err_sum"]
+                                                       if pn["learned_stats"]["net_err_sum"] > 0 else None),
+                "subset_rows_network": pn["learned_stats"]["subset_rows"],
+                "subset_rows_cubic": pc["learned_stats"]["subset_rows"],
+                "net_evals": pn["learned_stats"]["net_evals"],
+                "predicted_evals": pn["learned_stats"]["predicted_evals"]}
+
+    ok = {k: v for k, v in forms.items() if "gpairs_per_s" in v and v["energy_bound_met"] and k in HEADLINE_FORMS}
     pool = ok if ok else {k: v for k, v in forms.items() if "gpairs_per_s" in v and k in HEADLINE_FORMS}
     head = max(pool, key=lambda k: pool[k]["gpairs_per_s"])
     hr = forms[head]
 
-    rooflines = {}
-    # HBM traffic was counted for the default workload only
+    # HBM traffic was counted for the default workload's kernels only
     profiled = args.ic == "plummer" and n == 262144
-    if forms.get("fp64", {}).get("exact_force_evals"):
-        rooflines["fp64"] = roofline_entry(force_kernel_name("fp64", n) + " (FP64 VALU)",
-                                           forms["fp64"]["force_kernel_ms_avg"], n, FP64_PEAK_TFLOPS,
-                                           profile_traffic("k_accel_sym_plummer_n262144_pmc.md") if profiled
-                                           else None)
-    if forms.get("mixed", {}).get("exact_force_evals"):
-        rooflines["mixed"] = roofline_entry(force_kernel_name("mixed", n) + " (FP32 pair math on FP64 "
-                                            "differences; against the FP32 peak, which counts packed FP32)",
-                                            forms["mixed"]["force_kernel_ms_avg"], n, FP32_PEAK_TFLOPS,
-                                            profile_traffic("k_accel_sym_plummer_n262144_pmc.md",
-                                                            "traffic_bytes_per_launch_mixed") if profiled
-                                            else None)
+    rooflines = {}
+    for form, peak, note in (("fp64", FP64_PEAK_TFLOPS, "FP64 VALU"),
+                             ("mixed", FP32_PEAK_TFLOPS, "FP32 pair math on FP64 differences; against the FP32 "
+                                                         "peak, which counts packed FP32")):
+        r = forms.get(form, {})
+        if r.get("exact_force_evals") and r.get("force_kernel"):
+            k = r["force_kernel"]
+            traffic = profile_traffic("k_accel_sym_plummer_n262144_pmc.md",
+                                      f"traffic_bytes_per_launch {k['name']}") if profiled else None
+            rooflines[form] = roofline_entry(k, note, r["force_kernel_ms_avg"], n, peak, traffic)
     dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
     roofline = rooflines.get(dominant) or rooflines.get("fp64")
 
@@ -374,11 +445,17 @@ def main():
         "dtype": "f64" if head == "fp64" else "f64+f32",
         "data": "synthetic (seeded {})".format("Plummer sphere" if args.ic == "plummer" else "uniform sphere"),
         "config": {"workload": f"{ic_text}, eps={args.eps}, dt={args.dt:g}, 2nd-order leapfrog, "
-                               f"{args.steps} steps", "n": n, "eps": args.eps, "dt": args.dt,
-                   "headline_form": head, "energy_bound_met": head in ok},
+                               f"{args.steps} timed steps in the steady state; |dE/E| over a "
+                               f"{args.energy_steps or args.steps}-step run from the initial state",
+                   "n": n, "eps": args.eps, "dt": args.dt, "headline_form": head,
+                   "energy_bound_met": bool(hr["energy_bound_met"]),
+                   "value_counted_as": "effective GPairs/s: N^2 ordered pairs per simulated step x steps / wall "
+                                       "time (a learned form evaluates the direct sum on 1 substep in its period)"},
         "steps_per_s": hr["steps_per_s"],
-        "dE_E": hr["dE_E"],
+        "speedup_vs_fp64": hr.get("speedup_vs_fp64"),
+        "dE_E": hr.get("production_run", hr)["dE_E"],
         "forms": forms,
+        "network_gain": network_gain,
         "training": training,
         "host_io": host_io,
         "roofline": roofline,

@@ -192,6 +192,22 @@ typedef enum nb_kernel {
 
 NB_API nb_status nb_set_kernel(nb_ctx* ctx, int kernel);
 
+/* What the last force evaluation of the resident system ran (nb_accel,
+ * nb_compute_forces, nb_accel_device, nb_step, nb_step_euler, nb_step_rk and
+ * the learned form's exact substeps): for callers that label timings or
+ * profiles with the kernel the library chose.  No reference counterpart. */
+typedef struct nb_kernel_info {
+    int kernel;              /* NB_KERNEL_TILE or NB_KERNEL_SYMMETRIC; NB_KERNEL_AUTO before any evaluation */
+    int form;                /* NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW */
+    int per_lane;            /* particles per lane: symmetric K, tile i-particles */
+    int block;               /* symmetric: particles per block; tile: j-splits */
+    uint64_t items;          /* symmetric: work items; tile: workgroups */
+    uint64_t scratch_bytes;  /* partial-sum bytes the evaluation wrote */
+    char name[64];           /* kernel and its parameters, e.g. "k_accel_sym<FP64, K=8>" */
+} nb_kernel_info;
+
+NB_API nb_status nb_kernel_info_read(nb_ctx* ctx, nb_kernel_info* out);
+
 /* Snapshots in the reference's binary layout (NBody_system::save_particles,
  * CPU/nbody_system.cpp:133-152): frames of doubles [t, m[N], x[N], y[N],
  * z[N], vx[N], vy[N], vz[N], fx[N], fy[N], fz[N]] one after another, host
@@ -268,6 +284,10 @@ typedef struct nb_learned_stats {
     uint64_t net_evals;        /* extrapolated substeps that applied the network */
     double last_cubic_err;     /* the cubic alone at the last check */
     double last_check_max;     /* largest per-particle relative error at the last check */
+    uint64_t checks;           /* exact substeps that checked an extrapolation made for them */
+    uint64_t net_checks;       /* of those, checks that also evaluated the network's extrapolation */
+    double net_cubic_err_sum;  /* sum over net_checks of the cubic's RMS relative error */
+    double net_err_sum;        /* sum over net_checks of the network's RMS relative error */
 } nb_learned_stats;
 
 /* period >= 2 (exact every period-th substep), threshold > 0 (0 = always

@@ -87,6 +87,7 @@ struct nb_ctx {
         DevBuf items;
     } sym[2];
     int sym_next = 0;
+    nb_kernel_info kinfo{};   // the last force evaluation of the resident system
 };
 
 namespace nbh {
@@ -228,6 +229,33 @@ static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     return fits && n >= 16384;   // measured crossover: N = 8192 tile 57 us / symmetric 74 us, 16384 196 / 164 us
 }
 
+static const char* form_label(int kform) {
+    switch (kform) {
+    case nbk::FORM_FP64: return "FP64";
+    case nbk::FORM_MIXED_NR: return "MIXED_NR";
+    default: return "MIXED";
+    }
+}
+
+static int public_form(int kform) {
+    return kform == nbk::FORM_FP64 ? NB_FORM_FP64 : (kform == nbk::FORM_MIXED_NR ? NB_FORM_MIXED : NB_FORM_MIXED_RAW);
+}
+
+// Record what a force evaluation of the resident system ran (nb_kernel_info_read).
+static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int block, uint64_t items, uint64_t scratch) {
+    nb_kernel_info& k = c->kinfo;
+    k.kernel = kernel;
+    k.form = public_form(kform);
+    k.per_lane = per_lane;
+    k.block = block;
+    k.items = items;
+    k.scratch_bytes = scratch;
+    if (kernel == NB_KERNEL_SYMMETRIC)
+        std::snprintf(k.name, sizeof k.name, "k_accel_sym<%s, K=%d>", form_label(kform), per_lane);
+    else
+        std::snprintf(k.name, sizeof k.name, "k_accel<%s, IPT=%d>", form_label(kform), per_lane);
+}
+
 // One force evaluation of the resident state; `out` says where the partial
 // sums are and how to add them.
 hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
@@ -262,6 +290,7 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         out.nB = P.nB;
         out.S = 64 * K;
         out.L = P.L;
+        note_kernel(c, NB_KERNEL_SYMMETRIC, kform, K, int(S), uint64_t(P.nitems), R.act_bytes + R.react_bytes);
         const long ev = timing_begin(c);
         e = nbk::launch_accel_sym(L, c->stream);
         timing_end(c, ev);
@@ -298,6 +327,8 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     out.sym = false;
     out.part = part;
     out.S = S;
+    note_kernel(c, NB_KERNEL_TILE, kform, L.ipt, S, uint64_t((n + 256L * L.ipt - 1) / (256L * L.ipt)) * uint64_t(S),
+                S > 1 ? uint64_t(S) * uint64_t(n) * sizeof(P4) : 0);
     const long ev = timing_begin(c);
     hipError_t e = nbk::launch_accel(L, c->stream);
     timing_end(c, ev);
@@ -476,6 +507,7 @@ nb_status nb_set_state(nb_ctx* c, size_t n, const double* mass, const double* co
     }
     c->mass_h.assign(mass, mass + n);
     c->n = n;
+    c->kinfo = nb_kernel_info{};
     nbl::on_new_state(c->learned);
     return NB_OK;
 }
@@ -854,6 +886,12 @@ nb_status nb_set_kernel(nb_ctx* c, int kernel) {
     if (kernel != NB_KERNEL_AUTO && kernel != NB_KERNEL_TILE && kernel != NB_KERNEL_SYMMETRIC)
         return fail(NB_ERR_ARG, "nb_set_kernel: unknown kernel");
     c->kernel = kernel;
+    return NB_OK;
+}
+
+nb_status nb_kernel_info_read(nb_ctx* c, nb_kernel_info* out) {
+    if (!c || !out) return fail(NB_ERR_ARG, "nb_kernel_info_read: null argument");
+    *out = c->kinfo;
     return NB_OK;
 }
 

@@ -703,6 +703,12 @@ nb_status check(nb_ctx* c, Learned& L) {
         mxn = r[1].mx;
     }
     L.net_ok = L.pred_net && (err_n <= L.threshold);   // NaN fails
+    L.st.checks++;
+    if (L.pred_net) {
+        L.st.net_checks++;
+        L.st.net_cubic_err_sum += err_c;
+        L.st.net_err_sum += err_n;
+    }
     const double err = L.net_ok ? err_n : err_c;
     L.st.last_check_err = err;
     L.st.last_check_max = L.net_ok ? mxn : mxc;

@@ -41,7 +41,7 @@ EXPORTS = (
     "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
     "nb_near_accel", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
     "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
-    "nb_ring_predict",
+    "nb_ring_predict", "nb_kernel_info_read",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -61,7 +61,14 @@ class NBodyError(RuntimeError):
 class LearnedStats(C.Structure):
     _fields_ = [("exact_evals", C.c_uint64), ("predicted_evals", C.c_uint64), ("fallbacks", C.c_uint64),
                 ("subset_rows", C.c_uint64), ("last_check_err", C.c_double), ("max_check_err", C.c_double),
-                ("net_evals", C.c_uint64), ("last_cubic_err", C.c_double), ("last_check_max", C.c_double)]
+                ("net_evals", C.c_uint64), ("last_cubic_err", C.c_double), ("last_check_max", C.c_double),
+                ("checks", C.c_uint64), ("net_checks", C.c_uint64), ("net_cubic_err_sum", C.c_double),
+                ("net_err_sum", C.c_double)]
+
+
+class KernelInfo(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("form", C.c_int), ("per_lane", C.c_int), ("block", C.c_int),
+                ("items", C.c_uint64), ("scratch_bytes", C.c_uint64), ("name", C.c_char * 64)]
 
 
 _lib = None
@@ -112,6 +119,7 @@ def load_library(path: str = LIB_PATH):
     L.nb_deviation.argtypes = [_SZ, _P3, _P3, _D]
     L.nb_inertia.argtypes = [_SZ, _D, _P3, _D]
     L.nb_ring_predict.argtypes = [_SZ, C.c_double, C.c_double, _SZ, _SZ, _P3, _P3]
+    L.nb_kernel_info_read.argtypes = [_VP, C.POINTER(KernelInfo)]
     L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
     L.nb_learned_train.argtypes = [_VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]
     L.nb_learned_stats_read.argtypes = [_VP, C.POINTER(LearnedStats)]
@@ -364,6 +372,16 @@ class Solver:
 
     def synchronize(self):
         _check(self.L.nb_synchronize(self.h), "nb_synchronize")
+
+    def kernel_info(self) -> dict:
+        """What the last force evaluation of the resident system ran (kernel, form, name, scratch bytes)."""
+        k = KernelInfo()
+        _check(self.L.nb_kernel_info_read(self.h, C.byref(k)), "nb_kernel_info_read")
+        d = {f: getattr(k, f) for f, _ in KernelInfo._fields_}
+        d["name"] = k.name.decode()
+        d["kernel"] = {v: n for n, v in KERNELS.items()}.get(k.kernel, k.kernel)
+        d["form"] = {v: n for n, v in FORMS.items()}.get(k.form, k.form)
+        return d
 
     # --- learned extrapolator -----------------------------------------------
     def learned_configure(self, period: int = 4, threshold: float = 1e-3, exact_form="mixed"):

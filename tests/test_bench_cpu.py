@@ -1,5 +1,5 @@
-"""bench.py's host-side contract: one GPU only, and the kernel labels it
-attaches to its roofline entries follow the library's kernel choice."""
+"""bench.py's host-side contract: one GPU only, a learned form's warm-up that
+reaches the steady state, and counters taken over the timed steps only."""
 import os
 import subprocess
 import sys
@@ -18,14 +18,22 @@ def test_more_than_one_gpu_exits_with_message():
     assert r.stdout == ""
 
 
-@pytest.mark.parametrize("form,n,name", [
-    ("fp64", 2048, "k_accel<FP64, IPT=1> (tile)"),
-    ("mixed", 16383, "k_accel<MIXED_NR, IPT=1> (tile)"),
-    ("fp64", 16384, "k_accel_sym<FP64, K=4> (symmetric)"),
-    ("mixed", 131071, "k_accel_sym<MIXED_NR, K=4> (symmetric)"),
-    ("fp64", 262144, "k_accel_sym<FP64, K=8> (symmetric)"),
-    ("mixed", 262144, "k_accel_sym<MIXED_NR, K=6> (symmetric)"),
-])
-def test_force_kernel_names(form, n, name):
+def test_learned_warmup_reaches_steady_state():
+    """A learned form's warm-up covers the substeps after which the network
+    has passed its first check (order 2: one substep per step)."""
     import bench
-    assert bench.force_kernel_name(form, n) == name
+    for form in ("learned", "learned-cubic", "learned-p8"):
+        P = bench.learned_config(form)["period"]
+        assert bench.learned_warmup(form, 5) >= 6 * P
+        assert bench.learned_warmup(form, 1000) == 1000
+
+
+def test_stats_delta_subtracts_counters_only():
+    import bench
+    before = {k: 1 for k in bench.COUNTERS}
+    before.update(last_check_err=0.5)
+    after = {k: 4 for k in bench.COUNTERS}
+    after.update(last_check_err=0.25)
+    d = bench.stats_delta(after, before)
+    assert all(d[k] == 3 for k in bench.COUNTERS)
+    assert d["last_check_err"] == 0.25

@@ -80,6 +80,7 @@ def test_argument_errors():
     assert L.nb_step_euler(None, 1, 1e-3, 0, 0.0) == 1
     assert L.nb_step_rk(None, 1, 1e-3, None, None, 4, 0, 0.0) == 1
     assert L.nb_runge_kutta(4, None, None, None) == 1
+    assert L.nb_kernel_info_read(None, None) == 1
     assert L.nb_destroy(None) == 0
 
 

@@ -302,3 +302,30 @@ def test_ring_share_full_size_sampled(solver, orc):
     assert rel_err(acc.T[:, rows], ref) < FP64_TOL
     accm = solver.accel_block(pj[:ni], pj, "mixed", 0.005, j_block=131072)
     assert rel_err(accm.T[:, rows], ref) < MIXED_TOL
+
+
+@pytest.mark.parametrize("form,n,kernel,name", [
+    ("fp64", 2048, "tile", "k_accel<FP64, IPT=1>"),
+    ("mixed", 16383, "tile", "k_accel<MIXED_NR, IPT=1>"),
+    ("fp64", 16384, "symmetric", "k_accel_sym<FP64, K=4>"),
+    ("mixed-raw", 40000, "symmetric", "k_accel_sym<MIXED, K=4>"),
+    ("fp64", 262144, "symmetric", "k_accel_sym<FP64, K=8>"),
+    ("mixed", 262144, "symmetric", "k_accel_sym<MIXED_NR, K=6>"),
+])
+def test_kernel_info_names_the_kernel_that_ran(solver, form, n, kernel, name):
+    """nb_kernel_info_read reports the kernel the library chose for the last
+    force evaluation (the bench labels its rooflines with it)."""
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 3, [1.0, 20.0])
+    upload(solver, m, pos)
+    assert solver.kernel_info()["kernel"] == "auto"   # nothing evaluated since the state was set
+    solver.accel(form, 0.01)
+    k = solver.kernel_info()
+    assert (k["kernel"], k["name"], k["form"]) == (kernel, name, form)
+    if kernel == "symmetric":
+        assert k["block"] == 64 * k["per_lane"] and k["scratch_bytes"] > 0
+    solver.set_kernel("tile")
+    try:
+        solver.accel(form, 0.01)
+        assert solver.kernel_info()["kernel"] == "tile"
+    finally:
+        solver.set_kernel("auto")
