@@ -356,6 +356,13 @@ NB_API nb_status nb_mlp_forward(nb_ctx* ctx, size_t batch, const float* x, float
 NB_API nb_status nb_mlp_grad(nb_ctx* ctx, size_t batch, const float* x, const float* target,
                       float* grads, double* loss);
 
+/* Test hook for the optimiser kernel: one Adam step (beta1 0.9, beta2 0.999,
+ * epsilon 1e-8, bias-corrected) of the network's FP32 parameters with the
+ * host gradient `grads` (packed as above) and learning rate lr; the moments
+ * and the step count persist in the context until nb_mlp_set_params,
+ * nb_learned_reset or training start them again. */
+NB_API nb_status nb_mlp_adam_step(nb_ctx* ctx, const float* grads, double lr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1228,4 +1228,17 @@ nb_status nb_mlp_grad(nb_ctx* c, size_t batch, const float* x, const float* targ
     return NB_OK;
 }
 
+nb_status nb_mlp_adam_step(nb_ctx* c, const float* grads, double lr) {
+    if (!c || !grads || !(lr > 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_mlp_adam_step: bad arguments");
+    Learned* L = nbh::ctx_learned(c);
+    if (!L || !L->initialised) return nbh::set_error(NB_ERR_STATE, "nb_mlp_adam_step: set or train the network first");
+    hipStream_t st = nbh::ctx_stream(c);
+    LH(hipMemcpyAsync(L->grad.p, grads, mlp::NPARAM * sizeof(float), hipMemcpyHostToDevice, st), "upload gradient");
+    LH(mlp::adam((float*)L->params.p, (const float*)L->grad.p, (float*)L->am.p, (float*)L->av.p, (int)mlp::NPARAM,
+                 (float)lr, ++L->adam_t, st), "Adam kernel");
+    LH(mlp::publish((const float*)L->params.p, L->net, st), "publish kernel");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return NB_OK;
+}
+
 }  // extern "C"

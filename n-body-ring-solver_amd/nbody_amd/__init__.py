@@ -41,7 +41,7 @@ EXPORTS = (
     "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
     "nb_near_accel", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
     "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
-    "nb_ring_predict", "nb_kernel_info_read",
+    "nb_ring_predict", "nb_kernel_info_read", "nb_mlp_adam_step",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -128,6 +128,7 @@ def load_library(path: str = LIB_PATH):
     L.nb_mlp_set_params.argtypes = [_VP, _F]
     L.nb_mlp_forward.argtypes = [_VP, _SZ, _F, _F]
     L.nb_mlp_grad.argtypes = [_VP, _SZ, _F, _F, _F, _D]
+    L.nb_mlp_adam_step.argtypes = [_VP, _F, C.c_double]
     for name in EXPORTS:
         if name not in ("nb_last_error", "nb_version"):
             getattr(L, name).restype = C.c_int
@@ -447,3 +448,9 @@ class Solver:
         _check(self.L.nb_mlp_grad(self.h, x.shape[0], _p(x, np.float32), _p(target, np.float32), _p(g, np.float32),
                                   C.byref(loss)), "nb_mlp_grad")
         return g, loss.value
+
+    def mlp_adam_step(self, grads: np.ndarray, lr: float):
+        """One Adam step of the network's parameters with a host gradient (test hook)."""
+        grads = np.ascontiguousarray(grads, dtype=np.float32)
+        assert grads.shape == (mlp_param_count(),)
+        _check(self.L.nb_mlp_adam_step(self.h, _p(grads, np.float32), lr), "nb_mlp_adam_step")

@@ -59,6 +59,70 @@ def test_adam_closed_form(orc):
     np.testing.assert_allclose(p, expect, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("b1,b2", [(0.9, 0.999), (0.5, 0.75)])
+def test_adam_two_steps_closed_form(orc, b1, b2):
+    """Two steps with different gradients fix beta1, beta2 and the
+    t-dependence of the bias correction: after g1 then g2 the corrected
+    moments are (b1 g1 + g2) / (1 + b1) and (b2 g1^2 + g2^2) / (1 + b2)."""
+    lr, eps = 1e-2, 1e-8
+    p0 = np.linspace(-1, 1, 6)
+    g1 = np.array([0.5, -0.25, 1e-3, 0.0, 2.0, -1.0])
+    g2 = np.array([-0.5, 0.75, 3e-3, 1.0, 2.0, 0.25])
+    p, m, v = p0.copy(), np.zeros(6), np.zeros(6)
+    orc.adam(p, g1, m, v, lr, 1, b1, b2, eps)
+    orc.adam(p, g2, m, v, lr, 2, b1, b2, eps)
+    p1 = p0 - lr * g1 / (np.abs(g1) + eps)
+    mh = (b1 * g1 + g2) / (1 + b1)
+    vh = (b2 * g1 ** 2 + g2 ** 2) / (1 + b2)
+    np.testing.assert_allclose(p, p1 - lr * mh / (np.sqrt(vh) + eps), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(m, (1 - b1) * (b1 * g1 + g2), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(v, (1 - b2) * (b2 * g1 ** 2 + g2 ** 2), rtol=0, atol=1e-15)
+    # without the bias correction of step 2 the update would differ
+    wrong = p1 - lr * ((1 - b1) * (b1 * g1 + g2) / (1 - b1)) / (np.sqrt((1 - b2) * (b2 * g1 ** 2 + g2 ** 2) / (1 - b2)) + eps)
+    assert np.max(np.abs(p - wrong)) > 1e-4
+
+
+def single_path_params(orc):
+    """Weights with one active path x[2] -> h1[0] -> h2[5] -> h3[7] -> y[1]
+    (W_k row-major [out][in]), biases on that path and on y."""
+    P = np.zeros(orc.mlp_param_count())
+    W1, b1, W2, b2, W3, b3, W4, b4 = 0, 1920, 2048, 18432, 18560, 34944, 35072, 35456
+    P[W1 + 0 * 15 + 2] = 1.5
+    P[b1 + 0] = -0.25
+    P[W2 + 5 * 128 + 0] = -2.0
+    P[b2 + 5] = 0.5
+    P[W3 + 7 * 128 + 5] = 3.0
+    P[b3 + 7] = -1.0
+    P[W4 + 1 * 128 + 7] = 0.75
+    P[b4:b4 + 3] = [0.125, -0.0625, 0.25]
+    return P
+
+
+def single_path_expect(x):
+    silu = lambda z: z / (1.0 + np.exp(-z))
+    h1 = silu(1.5 * x[:, 2] - 0.25)
+    h2 = silu(-2.0 * h1 + 0.5)
+    h3 = silu(3.0 * h2 - 1.0)
+    y = np.zeros((x.shape[0], 3))
+    y[:, 0] = 0.125
+    y[:, 1] = 0.75 * h3 - 0.0625
+    y[:, 2] = 0.25
+    return y
+
+
+def test_mlp_forward_single_path_closed_form(orc):
+    """Layout [out][in], bias placement and SiLU against a closed form: only
+    one unit per layer is active; the other inputs must not leak in."""
+    P = single_path_params(orc)
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-2, 2, (7, 15))
+    np.testing.assert_allclose(orc.mlp_forward(P, x, bf16=False), single_path_expect(x), rtol=0, atol=1e-14)
+    # a transposed first layer ([in][out]) would read input 0 for unit 2 instead
+    Pt = P.copy()
+    Pt[2], Pt[2 * 128 + 0] = 0.0, 1.5
+    assert np.max(np.abs(orc.mlp_forward(Pt, x, bf16=False) - single_path_expect(x))) > 1e-3
+
+
 def test_cubic_extrapolation_exact_on_cubics(orc):
     rng = np.random.default_rng(1)
     coef = rng.standard_normal((4, 30))
