@@ -132,6 +132,37 @@ NB_API nb_status nb_accel_block(nb_ctx* ctx, int form, double eps, size_t ni, co
 NB_API nb_status nb_accel_block_device(nb_ctx* ctx, int form, double eps, size_t ni, const double* d_pi,
                                 size_t nj, const double* d_pj, size_t j_block, double* d_acc4);
 
+/* One ring round in the reference's reduced form: replaces one call of
+ * compute_forces_avx2_blocked / compute_forces_avx512_rsqrt_4i_blocked as
+ * hybrid_ring_solver makes it per received block (CPU/force_calculation.h:7-13,
+ * CPU/force_calculation.cpp:477-690, called at CPU/hybrid_solver.cpp:84-95),
+ * with the same arguments minus the per-thread scratch the context replaces:
+ * local_pos are the rank's local_n positions (nb_sys.pos), local_force its
+ * forces (nb_sys.force), recv_pos / recv_force the received block's,
+ * mass the global masses (comm_sz * local_n, nb_sys.mass).  With partner =
+ * (rank + recv_round) % comm_sz, every pair of local particle k (mass
+ * mass[rank * local_n + k]) and received particle q (mass
+ * mass[partner * local_n + q]) with q >= k if rank < partner, else q > k
+ * (CPU/force_calculation.cpp:530-531), is evaluated once, and its force
+ * f = -G m_k m_q (x_k - x_q) / (|x_k - x_q|^2 + eps^2)^(3/2) is ADDED to
+ * local_force[.][k] and SUBTRACTED from recv_force[.][q]: both arrays
+ * accumulate, as the reference's do.  eps = 0 is the reference exactly
+ * (coincident particles contribute nothing instead of NaN).  Host planes;
+ * synchronises. */
+NB_API nb_status nb_forces_ring_round(nb_ctx* ctx, int form, double eps, size_t local_n, const double* mass,
+                                      const double* const local_pos[3], double* const local_force[3],
+                                      const double* const recv_pos[3], double* const recv_force[3], size_t rank,
+                                      size_t comm_sz, size_t recv_round);
+
+/* As nb_forces_ring_round with DEVICE records: d_local and d_recv hold
+ * local_n (x, y, z, m) records each (the masses the caller indexes as the
+ * reference does), d_local_force4 and d_recv_force4 local_n (fx, fy, fz, 0)
+ * records that accumulate.  Queued on the context's stream, no
+ * synchronisation; the partial sums live in the context's scratch. */
+NB_API nb_status nb_forces_ring_round_device(nb_ctx* ctx, int form, double eps, size_t local_n,
+                                             const double* d_local, double* d_local_force4, const double* d_recv,
+                                             double* d_recv_force4, size_t rank, size_t comm_sz, size_t recv_round);
+
 /* Advance the resident state by time_steps DKD leapfrog steps of size dt with
  * coefficients c[lf_steps], d[lf_steps-1] (reduced_solver_omp_lf,
  * CPU/omp_solver.cpp:257-421): per step, for each substep s < lf_steps-1,

@@ -67,7 +67,7 @@ struct nb_ctx {
     size_t n = 0;
     std::vector<double> mass_h;
     DevBuf pm, vel, acc, part, rows, eblk;
-    DevBuf bi, bj, bout;
+    DevBuf bi, bj, bout, bout2;
     DevBuf rk;   // Runge-Kutta stage records: x0, kv[s], ka[s]
     // timing
     bool timing = false;
@@ -153,9 +153,9 @@ static long sym_item_tiles(const nb_ctx* c, long nB) {
     return std::max(2L, tiles / target);
 }
 
-static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
+static hipError_t sym_plan(nb_ctx* c, size_t n, int K, const nb_ctx::SymPlan*& plan) {
     for (const auto& Q : c->sym)
-        if (Q.n == c->n && Q.K == K) {
+        if (Q.n == n && Q.K == K) {
             plan = &Q;
             return hipSuccess;
         }
@@ -163,7 +163,7 @@ static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
     c->sym_next ^= 1;
     P.n = 0;   // invalid until built
     const long S = 64L * K;
-    const long nB = (long(c->n) + S - 1) / S;
+    const long nB = (long(n) + S - 1) / S;
     const long L = sym_item_tiles(c, nB);
     const long W = (nB - 1) / L + 1;
     std::vector<int4> items;
@@ -178,7 +178,7 @@ static hipError_t sym_plan(nb_ctx* c, int K, const nb_ctx::SymPlan*& plan) {
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return e;
-    P.n = c->n;
+    P.n = n;
     P.K = K;
     P.nB = int(nB);
     P.L = int(L);
@@ -263,7 +263,7 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     int K;
     if (use_symmetric(c, kform, K)) {
         const nb_ctx::SymPlan* plan = nullptr;
-        hipError_t e = sym_plan(c, K, plan);
+        hipError_t e = sym_plan(c, c->n, K, plan);
         if (e != hipSuccess) return e;
         const auto& P = *plan;
         const long S = 64L * K;
@@ -431,7 +431,8 @@ nb_status nb_destroy(nb_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     nbl::destroy(c->learned);
     c->learned = nullptr;
-    for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->rk,
+    for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->bout2,
+                      &c->rk,
                       &c->sym[0].items, &c->sym[1].items})
         b->release();
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
@@ -681,6 +682,117 @@ nb_status nb_accel_block_device(nb_ctx* c, int form, double eps, size_t ni, cons
     }
     const long jb = (j_block == 0 || j_block > nj) ? long(nj) : long(j_block);
     return block_eval(c, kf, eps, long(ni), (const P4*)d_pi, long(nj), (const P4*)d_pj, jb, (P4*)d_acc4);
+}
+
+// One ring round on the device: the local set d_l against the received set
+// d_r (n records each, masses in the records) with the symmetric kernel's
+// two-set form; m a accumulated into the force records d_fl, d_fr.
+static nb_status ring_round_eval(nb_ctx* c, int kf, double eps, long n, const P4* d_l, const P4* d_r, bool incl,
+                                 P4* d_fl, P4* d_fr) {
+    const int K = (n >= 131072) ? (kf == nbk::FORM_FP64 ? 8 : 6) : 4;
+    const nb_ctx::SymPlan* plan = nullptr;
+    NB_HIP(sym_plan(c, size_t(n), K, plan), "symmetric plan");
+    const long S = 64L * K;
+    SymRegions R = sym_regions(plan->nB, S, plan->L, kf);
+    R.react_bytes += size_t(plan->nB) * size_t(S) * (kf == nbk::FORM_FP64 ? sizeof(P4) : sizeof(float4));
+    NB_HIP(c->part.ensure(R.act_bytes + R.react_bytes), "allocate partial sums");
+    char* const react = (char*)c->part.p + R.act_bytes;
+    nbk::SymLaunch L;
+    L.form = kf;
+    L.K = K;
+    L.pm = d_l;
+    L.pj = d_r;
+    L.incl = incl;
+    L.n = int(n);
+    L.items = (const int4*)plan->items.p;
+    L.nitems = plan->nitems;
+    L.nB = plan->nB;
+    L.eps2 = eps * eps;
+    L.part = (P4*)c->part.p;
+    if (kf == nbk::FORM_FP64) L.react64 = (P4*)react;
+    else L.react32 = (float4*)react;
+    const long ev = timing_begin(c);
+    hipError_t e = nbk::launch_accel_sym(L, c->stream);
+    timing_end(c, ev);
+    NB_HIP(e, "force kernel");
+    nbk::SymSums P;
+    P.part = L.part;
+    P.react64 = L.react64;
+    P.react32 = L.react32;
+    P.nB = plan->nB;
+    P.S = int(S);
+    P.L = plan->L;
+    NB_HIP(nbk::launch_ring_finish(P, int(n), d_l, d_r, d_fl, d_fr, c->stream), "ring finish kernel");
+    return NB_OK;
+}
+
+static nb_status ring_args(nb_ctx* c, int form, double eps, size_t n, size_t rank, size_t comm_sz, const char* fn,
+                           int& kf) {
+    if (!c) return fail(NB_ERR_ARG, std::string(fn) + ": null context");
+    if (comm_sz == 0 || rank >= comm_sz) return fail(NB_ERR_ARG, std::string(fn) + ": need 0 <= rank < comm_sz");
+    if (n > kMaxN || comm_sz > kMaxN || n * comm_sz > kMaxN)
+        return fail(NB_ERR_ARG, std::string(fn) + ": comm_sz * local_n must stay below 1e8");
+    if (nb_status s = check_eps(eps, fn)) return s;
+    kf = kernel_form(form);
+    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
+    return NB_OK;
+}
+
+nb_status nb_forces_ring_round(nb_ctx* c, int form, double eps, size_t local_n, const double* mass,
+                               const double* const local_pos[3], double* const local_force[3],
+                               const double* const recv_pos[3], double* const recv_force[3], size_t rank,
+                               size_t comm_sz, size_t recv_round) {
+    int kf;
+    if (nb_status s = ring_args(c, form, eps, local_n, rank, comm_sz, "nb_forces_ring_round", kf)) return s;
+    if (local_n == 0) return NB_OK;
+    auto planes = [](const void* const* v) { return v && v[0] && v[1] && v[2]; };
+    if (!mass || !planes((const void* const*)local_pos) || !planes((const void* const*)local_force) ||
+        !planes((const void* const*)recv_pos) || !planes((const void* const*)recv_force))
+        return fail(NB_ERR_ARG, "nb_forces_ring_round: null array");
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    const size_t n = local_n;
+    const size_t partner = (rank + recv_round) % comm_sz;
+    const double* ml = mass + rank * n;
+    const double* mr = mass + partner * n;
+    std::vector<P4> h(4 * n);
+    for (size_t i = 0; i < n; i++) {
+        h[i] = P4{local_pos[0][i], local_pos[1][i], local_pos[2][i], ml[i]};
+        h[n + i] = P4{recv_pos[0][i], recv_pos[1][i], recv_pos[2][i], mr[i]};
+        h[2 * n + i] = P4{local_force[0][i], local_force[1][i], local_force[2][i], 0.0};
+        h[3 * n + i] = P4{recv_force[0][i], recv_force[1][i], recv_force[2][i], 0.0};
+    }
+    NB_HIP(c->bi.ensure(2 * n * sizeof(P4)), "allocate blocks");
+    NB_HIP(c->bout2.ensure(2 * n * sizeof(P4)), "allocate forces");
+    P4* d_sets = (P4*)c->bi.p;
+    P4* d_forces = (P4*)c->bout2.p;
+    NB_HIP(hipMemcpyAsync(d_sets, h.data(), 2 * n * sizeof(P4), hipMemcpyHostToDevice, c->stream), "upload blocks");
+    NB_HIP(hipMemcpyAsync(d_forces, h.data() + 2 * n, 2 * n * sizeof(P4), hipMemcpyHostToDevice, c->stream),
+           "upload forces");
+    if (nb_status s = ring_round_eval(c, kf, eps, long(n), d_sets, d_sets + n, rank < partner, d_forces, d_forces + n))
+        return s;
+    NB_HIP(hipMemcpyAsync(h.data(), d_forces, 2 * n * sizeof(P4), hipMemcpyDeviceToHost, c->stream),
+           "download forces");
+    NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            local_force[d][i] = (&h[i].x)[d];
+            recv_force[d][i] = (&h[n + i].x)[d];
+        }
+    return NB_OK;
+}
+
+nb_status nb_forces_ring_round_device(nb_ctx* c, int form, double eps, size_t local_n, const double* d_local,
+                                      double* d_local_force4, const double* d_recv, double* d_recv_force4,
+                                      size_t rank, size_t comm_sz, size_t recv_round) {
+    int kf;
+    if (nb_status s = ring_args(c, form, eps, local_n, rank, comm_sz, "nb_forces_ring_round_device", kf)) return s;
+    if (local_n == 0) return NB_OK;
+    if (!d_local || !d_local_force4 || !d_recv || !d_recv_force4)
+        return fail(NB_ERR_ARG, "nb_forces_ring_round_device: null buffer");
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    const size_t partner = (rank + recv_round) % comm_sz;
+    return ring_round_eval(c, kf, eps, long(local_n), (const P4*)d_local, (const P4*)d_recv, rank < partner,
+                           (P4*)d_local_force4, (P4*)d_recv_force4);
 }
 
 nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, const double* dd, size_t lf_steps,

@@ -66,6 +66,10 @@ struct SymLaunch {
     P4* part = nullptr;          // action slots: nB*W slots of S records, W = (nB - 1)/L + 1
     P4* react64 = nullptr;       // FP64 form: reaction slots, nB(nB-1)/2 slots of S records
     float4* react32 = nullptr;   // mixed forms: FP32 reaction slots, nB(nB-1)/2 slots of S float4
+    // two-set form (one ring round): j-chunks from pj, n records like pm;
+    // pairs q > k (q >= k with incl); nB more reaction slots for the diagonal
+    const P4* pj = nullptr;
+    bool incl = false;
 };
 
 hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st);
@@ -81,6 +85,9 @@ struct SymSums {
 hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st);
 hipError_t launch_sym_kick_drift(const SymSums& P, int n, P4* pm, P4* vel, P4* acc, double kdt, double c1dt,
                                  double c2dt, hipStream_t st);
+// Two-set finish: fl[i] += m_l a_l (actions), fr[i] += m_r a_r (reactions
+// incl. the diagonal slots), masses from the records pl / pr.
+hipError_t launch_ring_finish(const SymSums& P, int n, const P4* pl, const P4* pr, P4* fl, P4* fr, hipStream_t st);
 hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st);
 hipError_t launch_kick_drift(const P4* part, int S, long stride, int n, P4* pm, P4* vel, P4* acc,
                              double kdt, double c1dt, double c2dt, hipStream_t st);

@@ -24,6 +24,16 @@
 // carries that index); diagonal reactions fold into the actions.  Each slot
 // holds S records, one per particle of the block.
 // k_sym_kick_drift / k_sym_reduce add a block's slots in a fixed order.
+//
+// Two-set form (TWO): one round of the reference's ring (hybrid_ring_solver,
+// CPU/hybrid_solver.cpp:81-95 over compute_forces_avx2_blocked,
+// CPU/force_calculation.cpp:477-690): i-blocks come from the local set pm,
+// j-chunks from the received set pj (same count n), every pair of the local
+// particle k and the received particle q with q > k -- q >= k when `incl` --
+// evaluated once.  The diagonal tile's reactions belong to the received set's
+// block A and go to their own slot nB(nB-1)/2 + A instead of folding into the
+// actions; k_ring_finish adds the local actions and the received reactions
+// and accumulates m a into the two force arrays.
 #include <hip/hip_runtime.h>
 
 #include "nbody_kernels.h"
@@ -122,11 +132,12 @@ __device__ __forceinline__ P4 load_or_zero(const P4* __restrict__ p, int j, int 
 
 }  // namespace
 
-template <int FORM, bool EPS0, int K>
-__global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __restrict__ pm, int n,
+template <int FORM, bool EPS0, int K, bool TWO>
+__global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __restrict__ pm,
+                                                                   const P4* __restrict__ pj, int n,
                                                                    const int4* __restrict__ items, int nitems,
                                                                    int nB, double eps2, double cshift,
-                                                                   double coff, P4* __restrict__ part,
+                                                                   double coff, int incl, P4* __restrict__ part,
                                                                    P4* __restrict__ react64,
                                                                    float4* __restrict__ react32) {
     constexpr bool MIXED = (FORM != FORM_FP64);
@@ -159,7 +170,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
     for (int B = b0; B < b1; B++) {
         const bool diag = (B == A);
         for (int c = 0; c < K; c++) {
-            P4 q = load_or_zero(pm, B * S + c * 64 + lane, n);
+            P4 q = load_or_zero(TWO ? pj : pm, B * S + c * 64 + lane, n);
             q.m *= MSCALE;
             lds[lane] = q;
             lds[lane + 64] = q;
@@ -180,7 +191,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                 } else {
                     for (int t = 0; t < 64; t++) {
                         const P4 J = lds[lane + t];
-                        const bool same_ok = (t >= 1) && (lane + t < 64);   // i = c*64+lane < j = c*64+lane+t
+                        // i = c*64+lane < j = c*64+lane+t (<= in the two-set form with incl)
+                        const bool same_ok = (t >= (TWO ? 1 - incl : 1)) && (lane + t < 64);
 #pragma unroll
                         for (int k = 0; k < K; k++)
                             sym64<EPS0, true>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
@@ -208,7 +220,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                     for (int t = 0; t < 64; t++) {
                         const P4 J = lds[lane + t];
                         const float mj = (float)J.m;
-                        const bool same_ok = (t >= 1) && (lane + t < 64);
+                        const bool same_ok = (t >= (TWO ? 1 - incl : 1)) && (lane + t < 64);
 #pragma unroll
                         for (int k = 0; k < K; k++)
                             sym32<EPS0, true, NR>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f,
@@ -225,7 +237,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                 }
                 bx = (double)fbx; by = (double)fby; bz = (double)fbz;
             }
-            if (diag) {
+            if (diag && !TWO) {
                 // the chunk is this wave's own particles k == c: fold the reactions in
 #pragma unroll
                 for (int k = 0; k < K; k++)
@@ -234,8 +246,10 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
                     }
             } else {
                 // reactions of the mixed forms are FP32 sums: stored as such
-                // (16 B), the FP64 form's as records (32 B)
-                const long slot = ((long)B * (B - 1) / 2 + A) * S + c * 64 + lane;
+                // (16 B), the FP64 form's as records (32 B); the two-set
+                // form's diagonal reactions after the triangle
+                const long tri = diag ? (long)nB * (nB - 1) / 2 + A : (long)B * (B - 1) / 2 + A;
+                const long slot = tri * S + c * 64 + lane;
                 if constexpr (MIXED) react32[slot] = make_float4((float)bx, (float)by, (float)bz, 0.0f);
                 else react64[slot] = P4{bx, by, bz, 0.0};
             }
@@ -308,6 +322,55 @@ __device__ __forceinline__ void sym_sum(const P4* __restrict__ part, const P4* _
     }
 }
 
+// Two-set finish: the local particle i's actions (row X) and the received
+// particle i's reactions (triangular slots Y < X, then the diagonal slot),
+// each in slot order, times the particle's mass, added to the force records.
+template <bool F32R>
+__device__ __forceinline__ void ring_react_sum(const P4* __restrict__ react64, const float4* __restrict__ react32,
+                                               int nB, int S, int i, double& bx, double& by, double& bz) {
+    const int X = i / S, r = i - X * S;
+    bx = by = bz = 0.0;
+    const long rrow = (long)X * (X - 1) / 2 * S + r;
+    const long dslot = ((long)nB * (nB - 1) / 2 + X) * S + r;
+    for (int Y = 0; Y <= X; Y++) {
+        const long at = (Y < X) ? rrow + (long)Y * S : dslot;
+        if constexpr (F32R) {
+            const float4 q = react32[at];
+            bx += (double)q.x; by += (double)q.y; bz += (double)q.z;
+        } else {
+            const P4 q = react64[at];
+            bx += q.x; by += q.y; bz += q.z;
+        }
+    }
+}
+
+template <bool F32R>
+__global__ __launch_bounds__(BLOCK) void k_ring_finish(const P4* __restrict__ part, const P4* __restrict__ react64,
+                                                       const float4* __restrict__ react32, int nB, int S, int L,
+                                                       int n, const P4* __restrict__ pl, const P4* __restrict__ pr,
+                                                       P4* __restrict__ fl, P4* __restrict__ fr) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int X = i / S, r = i - X * S;
+    const int W = (nB - 1) / L + 1;
+    const int cnt = (nB - 1 - X) / L + 1;   // items of row X
+    const long arow = (long)X * W * S + r;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int j = 0; j < cnt; j++) {
+        const P4 p = part[arow + (long)j * S];
+        ax += p.x; ay += p.y; az += p.z;
+    }
+    double bx, by, bz;
+    ring_react_sum<F32R>(react64, react32, nB, S, i, bx, by, bz);
+    const double ml = pl[i].m, mr = pr[i].m;
+    P4 f = fl[i];
+    f.x += ml * ax; f.y += ml * ay; f.z += ml * az;
+    fl[i] = f;
+    P4 g = fr[i];
+    g.x += mr * bx; g.y += mr * by; g.z += mr * bz;
+    fr[i] = g;
+}
+
 template <bool F32R>
 __global__ __launch_bounds__(BLOCK) void k_sym_reduce(const P4* __restrict__ part, const P4* __restrict__ react64,
                                                       const float4* __restrict__ react32, int nB, int S, int L,
@@ -349,8 +412,14 @@ __global__ __launch_bounds__(BLOCK) void k_sym_kick_drift(const P4* __restrict__
 template <int FORM, bool EPS0, int K>
 static hipError_t launch_sym_t(const SymLaunch& L, hipStream_t st) {
     const int groups = (L.nitems + kWavesPerGroup - 1) / kWavesPerGroup;
-    hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm, L.n,
-                       L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, L.part, L.react64, L.react32);
+    if (L.pj)
+        hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K, true>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm,
+                           L.pj, L.n, L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, L.incl ? 1 : 0,
+                           L.part, L.react64, L.react32);
+    else
+        hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K, false>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm,
+                           L.pm, L.n, L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, 0, L.part,
+                           L.react64, L.react32);
     return hipGetLastError();
 }
 
@@ -391,6 +460,19 @@ hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st) {
     else
         hipLaunchKernelGGL(k_sym_reduce<false>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB, P.S,
                            P.L, n, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_ring_finish(const SymSums& P, int n, const P4* pl, const P4* pr, P4* fl, P4* fr, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!P.react64 == !P.react32) return hipErrorInvalidValue;
+    const dim3 grid((n + BLOCK - 1) / BLOCK);
+    if (P.react32)
+        hipLaunchKernelGGL(k_ring_finish<true>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB, P.S,
+                           P.L, n, pl, pr, fl, fr);
+    else
+        hipLaunchKernelGGL(k_ring_finish<false>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB, P.S,
+                           P.L, n, pl, pr, fl, fr);
     return hipGetLastError();
 }
 

@@ -41,7 +41,8 @@ EXPORTS = (
     "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
     "nb_near_accel", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
     "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
-    "nb_ring_predict", "nb_kernel_info_read", "nb_mlp_adam_step",
+    "nb_ring_predict", "nb_kernel_info_read", "nb_mlp_adam_step", "nb_forces_ring_round",
+    "nb_forces_ring_round_device",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -120,6 +121,8 @@ def load_library(path: str = LIB_PATH):
     L.nb_inertia.argtypes = [_SZ, _D, _P3, _D]
     L.nb_ring_predict.argtypes = [_SZ, C.c_double, C.c_double, _SZ, _SZ, _P3, _P3]
     L.nb_kernel_info_read.argtypes = [_VP, C.POINTER(KernelInfo)]
+    L.nb_forces_ring_round.argtypes = [_VP, C.c_int, C.c_double, _SZ, _D, _P3, _P3, _P3, _P3, _SZ, _SZ, _SZ]
+    L.nb_forces_ring_round_device.argtypes = [_VP, C.c_int, C.c_double, _SZ, _VP, _VP, _VP, _VP, _SZ, _SZ, _SZ]
     L.nb_learned_collect.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
     L.nb_learned_train.argtypes = [_VP, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]
     L.nb_learned_stats_read.argtypes = [_VP, C.POINTER(LearnedStats)]
@@ -332,6 +335,26 @@ class Solver:
         """Device pointers (ints, e.g. torch tensor.data_ptr()); queued, not synchronised."""
         _check(self.L.nb_accel_block_device(self.h, _form(form), eps, ni, d_pi, nj, d_pj, j_block, d_out),
                "nb_accel_block_device")
+
+    def forces_ring_round(self, mass, rank: int, comm_sz: int, recv_round: int, local_pos, recv_pos, force,
+                          recv_force, form="fp64", eps: float = 0.0):
+        """One ring round in the reference's reduced form; force and recv_force
+        ((3, local_n) float64, C-contiguous) accumulate in place."""
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        lp = np.ascontiguousarray(local_pos, dtype=np.float64)
+        rp = np.ascontiguousarray(recv_pos, dtype=np.float64)
+        n = lp.shape[1]
+        assert rp.shape == lp.shape == force.shape == recv_force.shape and mass.shape == (comm_sz * n,)
+        _check(self.L.nb_forces_ring_round(self.h, _form(form), eps, n, _p(mass), _planes(lp), _planes(force),
+                                           _planes(rp), _planes(recv_force), rank, comm_sz, recv_round),
+               "nb_forces_ring_round")
+
+    def forces_ring_round_device(self, form, eps, n, d_local, d_local_force4, d_recv, d_recv_force4, rank, comm_sz,
+                                 recv_round):
+        """Device pointers (ints); queued, not synchronised."""
+        _check(self.L.nb_forces_ring_round_device(self.h, _form(form), eps, n, d_local, d_local_force4, d_recv,
+                                                  d_recv_force4, rank, comm_sz, recv_round),
+               "nb_forces_ring_round_device")
 
     # --- integration ---------------------------------------------------------
     def step(self, steps: int, dt: float, order: int = 2, form="fp64", eps: float = 0.0, coeffs=None):

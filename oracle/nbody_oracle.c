@@ -374,6 +374,49 @@ ORC_API void orc_forces_reduced(size_t n, const double *m, const double *x, cons
     }
 }
 
+/* One ring round of the reference's reduced scheme, as hybrid_ring_solver
+ * runs it per received block (CPU/hybrid_solver.cpp:81-95 calling
+ * compute_forces_avx2_blocked, CPU/force_calculation.cpp:477-690): the local
+ * block of n particles (positions lx..lz, masses mass[rank*n + k]) against
+ * the received block (rx..rz, masses mass[partner*n + q], partner =
+ * (rank + recv_round) % comm_sz, :531).  Pairs q >= k when rank < partner,
+ * else q > k (:530); the pair force f = -G m_k m_q (x_k - x_q) /
+ * (r^2+eps^2)^(3/2) is ADDED to the local force F_k and SUBTRACTED from the
+ * received block's force R_q (:617-630, :681-688: both accumulate into the
+ * caller's arrays).  Neumaier-compensated per component, starting from the
+ * caller's values; pairs at exactly zero separation contribute nothing. */
+ORC_API void orc_ring_round(size_t n, const double *mass, size_t rank, size_t comm_sz, size_t recv_round,
+                            const double *lx, const double *ly, const double *lz, const double *rx,
+                            const double *ry, const double *rz, double eps, double G, double *fx, double *fy,
+                            double *fz, double *rfx, double *rfy, double *rfz)
+{
+    const double e2 = eps * eps;
+    const size_t partner = (rank + recv_round) % comm_sz;
+    const double *ml = mass + rank * n, *mr = mass + partner * n;
+    double *out[6] = {fx, fy, fz, rfx, rfy, rfz};
+    orc_ksum *acc = (orc_ksum *)malloc(6 * (n ? n : 1) * sizeof(orc_ksum));
+    if (!acc) return;
+    for (int c = 0; c < 6; c++)
+        for (size_t i = 0; i < n; i++) acc[c * n + i] = (orc_ksum){out[c][i], 0.0};
+    for (size_t k = 0; k < n; k++) {
+        const double pre = -G * ml[k];
+        for (size_t q = (rank < partner) ? k : k + 1; q < n; q++) {
+            double d[3] = {lx[k] - rx[q], ly[k] - ry[q], lz[k] - rz[q]};
+            double s2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + e2;
+            if (!(s2 > 0.0)) continue;
+            double dist = sqrt(s2);
+            double sc = pre * mr[q] / (dist * dist * dist);
+            for (int c = 0; c < 3; c++) {
+                orc_kadd(&acc[c * n + k], sc * d[c]);
+                orc_kadd(&acc[(3 + c) * n + q], -(sc * d[c]));
+            }
+        }
+    }
+    for (int c = 0; c < 6; c++)
+        for (size_t i = 0; i < n; i++) out[c][i] = orc_kval(&acc[c * n + i]);
+    free(acc);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Integrator                                                                */
 /* ------------------------------------------------------------------------- */

@@ -50,6 +50,7 @@ class Oracle:
         for name in ("orc_accel", "orc_accel_ld", "orc_forces_reduced"):
             getattr(L, name).argtypes = [_SZ, _D, _D, _D, _D, C.c_double, C.c_double, _D, _D, _D]
         L.orc_accel_block.argtypes = [_SZ, _D, _SZ, _D, C.c_double, C.c_double, _D]
+        L.orc_ring_round.argtypes = [_SZ, _D, _SZ, _SZ, _SZ] + [_D] * 6 + [C.c_double, C.c_double] + [_D] * 6
         L.orc_yoshida.argtypes = [C.c_int, _D, _D]
         L.orc_yoshida.restype = _SZ
         L.orc_leapfrog.argtypes = [_SZ, _D, _D, _D, _D, _D, _D, _D, _SZ, C.c_double, _D, _D, _SZ,
@@ -131,6 +132,18 @@ class Oracle:
         self.lib.orc_forces_reduced(n, _p(m), *[_p_row(pos, k) for k in range(3)], eps, G,
                                     *[_p_row(f, k) for k in range(3)])
         return f
+
+    def ring_round(self, mass, rank: int, comm_sz: int, recv_round: int, local_pos, recv_pos, force,
+                   recv_force, eps: float = 0.0, G: float = 1.0):
+        """One ring round of the reduced scheme; force and recv_force ((3, n),
+        float64, C-contiguous) accumulate in place."""
+        n = local_pos.shape[1]
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        lp, rp = np.ascontiguousarray(local_pos), np.ascontiguousarray(recv_pos)
+        assert mass.shape[0] == comm_sz * n and force.flags.c_contiguous and recv_force.flags.c_contiguous
+        self.lib.orc_ring_round(n, _p(mass), rank, comm_sz, recv_round, *[_p_row(lp, k) for k in range(3)],
+                                *[_p_row(rp, k) for k in range(3)], eps, G, *[_p_row(force, k) for k in range(3)],
+                                *[_p_row(recv_force, k) for k in range(3)])
 
     def accel_block(self, pi: np.ndarray, pj: np.ndarray, eps: float, G: float = 1.0) -> np.ndarray:
         """pi (ni, 4), pj (nj, 4) rows of (x, y, z, m) -> (ni, 3) accelerations."""
@@ -299,6 +312,7 @@ class Reference:
         L.ref_inertia.argtypes = [_SZ, _D, P3]
         L.ref_inertia.restype = C.c_double
         L.ref_ring_predict.argtypes = [_SZ, C.c_double, C.c_double, _SZ, _SZ, P3, P3]
+        L.ref_ring_round.argtypes = [C.c_int, _SZ, _SZ, _SZ, _SZ, _D, P3, P3, P3, P3]
 
     def forces(self, variant: int, m, pos) -> np.ndarray:
         n = m.shape[0]
@@ -307,6 +321,17 @@ class Reference:
         rc = self.lib.ref_forces(variant, n, _p(m), _ptr3(pos), _ptr3(f))
         assert rc == 0
         return f
+
+    def ring_round(self, variant: int, mass, rank: int, comm_sz: int, recv_round: int, local_pos, recv_pos,
+                   force, recv_force):
+        """The reference kernel `variant` for one ring round; force and
+        recv_force ((3, n), C-contiguous) accumulate in place."""
+        lp, rp = np.ascontiguousarray(local_pos), np.ascontiguousarray(recv_pos)
+        assert force.flags.c_contiguous and recv_force.flags.c_contiguous
+        rc = self.lib.ref_ring_round(variant, lp.shape[1], comm_sz, rank, recv_round,
+                                     _p(np.ascontiguousarray(mass, dtype=np.float64)), _ptr3(lp), _ptr3(rp),
+                                     _ptr3(force), _ptr3(recv_force))
+        assert rc == 0
 
     def time_forces(self, variant: int, m, pos, reps: int) -> float:
         pos = np.ascontiguousarray(pos)

@@ -138,6 +138,51 @@ __attribute__((visibility("default"))) int ref_forces(int variant, size_t n, con
     return 0;
 }
 
+// One ring round of the reference's reduced scheme with variant 0
+// (compute_forces_avx2_blocked) or 1 (compute_forces_avx512_rsqrt_4i_blocked),
+// as hybrid_ring_solver calls it per round (CPU/hybrid_solver.cpp:81-95) but
+// in one process: the system holds comm_sz * n masses and the local block's
+// n positions, the received block is recv_pos; force and recv_force come in
+// with the caller's values and the kernel accumulates into them.
+__attribute__((visibility("default"))) int ref_ring_round(int variant, size_t n, size_t comm_sz, size_t rank,
+                                                          size_t recv_round, const double* mass,
+                                                          const double* const* local_pos,
+                                                          const double* const* recv_pos, double* const* force,
+                                                          double* const* recv_force) {
+    if (comm_sz == 0 || rank >= comm_sz || n == 0) return 1;
+    NBody_system s;
+    s.alloc_particles(comm_sz * n, n);
+    for (size_t i = 0; i < comm_sz * n; i++) {
+        s.mass[i] = mass[i];
+        s.mass_sp[i] = float(mass[i]);
+    }
+    for (int d = 0; d < DIM; d++)
+        for (size_t i = 0; i < comm_sz * n; i++) s.pos[d][i] = i < n ? local_pos[d][i] : 0.0;
+    Scratch sc(n);
+    for (int d = 0; d < DIM; d++)
+        for (size_t i = 0; i < n; i++) {
+            s.force[d][i] = force[d][i];
+            sc.recv_pos[d][i] = recv_pos[d][i];
+            sc.recv_force[d][i] = recv_force[d][i];
+        }
+    switch (variant) {
+    case 0:
+        compute_forces_avx2_blocked(s, sc.recv_pos, sc.recv_force, sc.lf, sc.lrf, n, rank, comm_sz, recv_round, sc.T);
+        break;
+    case 1:
+        compute_forces_avx512_rsqrt_4i_blocked(s, sc.recv_pos, sc.recv_force, sc.lf, sc.lrf, n, rank, comm_sz,
+                                               recv_round, sc.T);
+        break;
+    default: return 1;
+    }
+    for (int d = 0; d < DIM; d++)
+        for (size_t i = 0; i < n; i++) {
+            force[d][i] = s.force[d][i];
+            recv_force[d][i] = sc.recv_force[d][i];
+        }
+    return 0;
+}
+
 // Repeated force evaluations for timing (the cpu_baseline leg): `reps` calls of
 // variant's kernel on the same positions; returns the best wall time in seconds.
 __attribute__((visibility("default"))) double ref_time_forces(int variant, size_t n, const double* m,

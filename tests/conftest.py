@@ -60,3 +60,37 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     d = np.linalg.norm(a - b, axis=0)
     mask = nb > 0
     return float(np.max(d[mask] / nb[mask])) if mask.any() else 0.0
+
+
+class DeviceArray:
+    """A float64 array in device memory through the HIP runtime the solver
+    library itself loaded (hipMalloc / hipMemcpy / hipFree), for the tests of
+    the entry points that take device pointers."""
+
+    def __init__(self, host: np.ndarray):
+        import ctypes as C
+        import nbody_amd
+        nbody_amd.load_library()
+        self._hip = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)   # the runtime the library links
+        self._C = C
+        self.host = np.ascontiguousarray(host, dtype=np.float64)
+        self.ptr = C.c_void_p()
+        assert self._hip.hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.host.nbytes, 8))) == 0
+        assert self._hip.hipMemcpy(self.ptr, self.host.ctypes.data_as(C.c_void_p), C.c_size_t(self.host.nbytes),
+                                   1) == 0   # hipMemcpyHostToDevice
+
+    @property
+    def addr(self) -> int:
+        return self.ptr.value
+
+    def get(self) -> np.ndarray:
+        out = np.empty_like(self.host)
+        assert self._hip.hipDeviceSynchronize() == 0
+        assert self._hip.hipMemcpy(out.ctypes.data_as(self._C.c_void_p), self.ptr, self._C.c_size_t(out.nbytes),
+                                   2) == 0   # hipMemcpyDeviceToHost
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = self._C.c_void_p()

@@ -34,9 +34,18 @@ Fixtures
                     the ring (deviation), compute_inertia (inertia), and the
                     rigid-rotation prediction after 37 of 1000 steps per
                     revolution (pos_pred, vel_pred).
+  ringround_<n>_p<P>.npz
+                    one ring round of the reduced scheme per (rank,
+                    recv_round) of a P-rank ring over the jittered system of
+                    P*n particles in contiguous blocks of n (mass, pos): the
+                    force arrays the caller passes in (force0, recv0, seeded
+                    non-zero so the accumulation shows) and what
+                    compute_forces_avx2_blocked (force_avx2, recv_avx2) and
+                    compute_forces_avx512_rsqrt_4i_blocked (force_avx512,
+                    recv_avx512) leave in them; indexed [rank, recv_round].
 
 Usage: python tests/golden/make_golden.py [group ...] with groups ring,
-jitter, drivers, accuracy (default: all).
+jitter, drivers, accuracy, ringround (default: all).
 """
 from __future__ import annotations
 
@@ -89,9 +98,32 @@ def write_accuracy(ref):
                         pos_pred=pp, vel_pred=vp)
 
 
-def main(groups=("ring", "jitter", "drivers", "accuracy")):
+def write_ringround(ref):
+    for n, P in ((64, 2), (300, 3)):
+        m, pos, _ = jittered(ref, P * n)
+        blocks = [np.ascontiguousarray(pos[:, r * n:(r + 1) * n]) for r in range(P)]
+        rng = np.random.default_rng(4321 + n)
+        out = {"mass": m, "pos": pos}
+        shape = (P, P, 3, n)
+        out["force0"] = rng.standard_normal(shape) * 1e-2
+        out["recv0"] = rng.standard_normal(shape) * 1e-2
+        for name, variant in (("avx2", Reference.AVX2_BLOCKED), ("avx512", Reference.AVX512_RSQRT_4I)):
+            fo, ro = np.empty(shape), np.empty(shape)
+            for rank in range(P):
+                for rr in range(P):
+                    f = out["force0"][rank, rr].copy()   # the round accumulates into its arrays
+                    r = out["recv0"][rank, rr].copy()
+                    ref.ring_round(variant, m, rank, P, rr, blocks[rank], blocks[(rank + rr) % P], f, r)
+                    fo[rank, rr], ro[rank, rr] = f, r
+            out[f"force_{name}"], out[f"recv_{name}"] = fo, ro
+        np.savez_compressed(os.path.join(HERE, f"ringround_{n}_p{P}.npz"), **out)
+
+
+def main(groups=("ring", "jitter", "drivers", "accuracy", "ringround")):
     ref = Reference()
     orc = Oracle()
+    if "ringround" in groups:
+        write_ringround(ref)
     if "drivers" in groups:
         write_drivers(ref)
     if "accuracy" in groups:
@@ -121,4 +153,4 @@ def main(groups=("ring", "jitter", "drivers", "accuracy")):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("ring", "jitter", "drivers", "accuracy"))
+    main(tuple(sys.argv[1:]) or ("ring", "jitter", "drivers", "accuracy", "ringround"))

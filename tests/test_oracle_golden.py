@@ -100,3 +100,62 @@ def test_live_leapfrog_sphere(orc, ref):
     p2, v2 = orc.leapfrog(m, pos, vel, 5, 1e-3, 4, 0.0)
     assert np.max(np.abs(p1 - p2)) < 1e-13
     assert np.max(np.abs(v1 - v2)) < 1e-11 * np.max(np.abs(v1))
+
+
+@pytest.mark.parametrize("name", ["ringround_64_p2.npz", "ringround_300_p3.npz"])
+def test_oracle_ring_round_vs_reference_fixtures(orc, name):
+    """The oracle's ring round against compute_forces_avx2_blocked's and
+    compute_forces_avx512_rsqrt_4i_blocked's outputs for every (rank,
+    recv_round): pair selection, mass indexing, signs and accumulation."""
+    g = golden(name)
+    P, n = g["force0"].shape[0], g["force0"].shape[3]
+    blocks = [np.ascontiguousarray(g["pos"][:, r * n:(r + 1) * n]) for r in range(P)]
+    for rank in range(P):
+        for rr in range(P):
+            f = np.ascontiguousarray(g["force0"][rank, rr].copy())
+            r = np.ascontiguousarray(g["recv0"][rank, rr].copy())
+            orc.ring_round(g["mass"], rank, P, rr, blocks[rank], blocks[(rank + rr) % P], f, r)
+            for variant in ("avx2", "avx512"):
+                for out, key in ((f, "force"), (r, "recv")):
+                    ref = g[f"{key}_{variant}"][rank, rr]
+                    scale = np.max(np.abs(ref - g[f"{key}0"][rank, rr]))
+                    assert np.max(np.abs(out - ref)) <= 1e-12 * scale, (rank, rr, variant, key)
+
+
+def ring_sum(orc, m, pos, P, n, round_fn):
+    blocks = [np.ascontiguousarray(pos[:, r * n:(r + 1) * n]) for r in range(P)]
+    F = np.zeros((3, P * n))
+    for rank in range(P):
+        f = np.zeros((3, n))
+        for rr in range(P):
+            partner = (rank + rr) % P
+            rf = np.zeros((3, n))
+            round_fn(m, rank, P, rr, blocks[rank], blocks[partner], f, rf)
+            F[:, partner * n:(partner + 1) * n] += rf
+        F[:, rank * n:(rank + 1) * n] += f
+    return F
+
+
+@pytest.mark.parametrize("P,n", [(2, 5), (3, 7), (4, 16)])
+def test_oracle_ring_covers_every_pair_once(orc, P, n):
+    """Over a whole ring (every rank, every round, received forces returned
+    to their owners) each pair is evaluated exactly once: the sum equals the
+    whole system's reduced force.  A pair rule of q > k in every round misses
+    the pairs with q = k across blocks and fails."""
+    rng = np.random.default_rng(P * 100 + n)
+    m = rng.uniform(0.5, 1.5, P * n)
+    pos = rng.standard_normal((3, P * n))
+    full = orc.forces_reduced(m, pos, 0.0)
+    F = ring_sum(orc, m, pos, P, n, orc.ring_round)
+    assert np.max(np.abs(F - full)) <= 1e-13 * np.max(np.abs(full))
+
+    def strict(mm, rank, PP, rr, lp, rp, f, rf):   # planted mistake: always q > k
+        nn = lp.shape[1]
+        partner = (rank + rr) % PP
+        for k in range(nn):
+            for q in range(k + 1, nn):
+                d = lp[:, k] - rp[:, q]
+                fk = -mm[rank * nn + k] * mm[partner * nn + q] * d / np.linalg.norm(d) ** 3
+                f[:, k] += fk
+                rf[:, q] -= fk
+    assert np.max(np.abs(ring_sum(orc, m, pos, P, n, strict) - full)) > 1e-3 * np.max(np.abs(full))
