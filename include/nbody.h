@@ -132,6 +132,21 @@ NB_API nb_status nb_accel_block(nb_ctx* ctx, int form, double eps, size_t ni, co
 NB_API nb_status nb_accel_block_device(nb_ctx* ctx, int form, double eps, size_t ni, const double* d_pi,
                                 size_t nj, const double* d_pj, size_t j_block, double* d_acc4);
 
+/* As nb_accel_block_device, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels and summation order, so the results equal
+ * nb_accel_block_device's bit for bit, but the j-split partial sums go to a
+ * static 256 MiB device array that the library holds on every device it runs
+ * on until the process ends: calls whose partial sums need more (more than
+ * the 131072 x 1048576 ring share in j-blocks of 131072) fail with NB_ERR_ARG
+ * before anything is queued.  Every call on one device shares that array, so
+ * calls that overlap on one device (two contexts, or two host threads) race;
+ * calls queued on one context's stream are ordered.  Not timed by
+ * nb_timing_*; nj = 0 writes zeros with a fill kernel. */
+NB_API nb_status nb_accel_block_device_noalloc(nb_ctx* ctx, int form, double eps, size_t ni, const double* d_pi,
+                                               size_t nj, const double* d_pj, size_t j_block, double* d_acc4);
+
 /* One ring round in the reference's reduced form: replaces one call of
  * compute_forces_avx2_blocked / compute_forces_avx512_rsqrt_4i_blocked as
  * hybrid_ring_solver makes it per received block (CPU/force_calculation.h:7-13,

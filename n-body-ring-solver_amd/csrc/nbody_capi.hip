@@ -37,6 +37,20 @@ nb_status hip_fail(hipError_t e, const char* what) {
 
 constexpr size_t kMaxN = 100000000;   // the reference's bound (CPU/main.cpp:271)
 
+// Partial sums of nb_accel_block_device_noalloc: a fixed array in device
+// memory that every GPU the library runs on holds until the process ends
+// (256 MiB: the 131072 x 1048576 ring share in j-blocks of 131072 needs
+// exactly that).  Host code never passes the array's name to a kernel or a
+// copy: it uses the address hipGetSymbolAddress returns for the device
+// (nb_ctx::scratch_ptr, taken in nb_create with the device current).
+constexpr size_t kStaticScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) P4 scratch[kStaticScratchBytes / sizeof(P4)];
+
+__global__ void k_zero_records(P4* __restrict__ p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = P4{0.0, 0.0, 0.0, 0.0};
+}
+
 }  // namespace
 
 namespace nbh {
@@ -88,6 +102,8 @@ struct nb_ctx {
     } sym[2];
     int sym_next = 0;
     nb_kernel_info kinfo{};   // the last force evaluation of the resident system
+    P4* scratch_ptr = nullptr;          // this device's address of the static scratch array
+    hipError_t scratch_status = hipSuccess;
 };
 
 namespace nbh {
@@ -242,14 +258,15 @@ static int public_form(int kform) {
 }
 
 // Record what a force evaluation of the resident system ran (nb_kernel_info_read).
-static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int block, uint64_t items, uint64_t scratch) {
+static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int block, uint64_t items,
+                        uint64_t scratch_bytes) {
     nb_kernel_info& k = c->kinfo;
     k.kernel = kernel;
     k.form = public_form(kform);
     k.per_lane = per_lane;
     k.block = block;
     k.items = items;
-    k.scratch_bytes = scratch;
+    k.scratch_bytes = scratch_bytes;
     if (kernel == NB_KERNEL_SYMMETRIC)
         std::snprintf(k.name, sizeof k.name, "k_accel_sym<%s, K=%d>", form_label(kform), per_lane);
     else
@@ -420,6 +437,9 @@ nb_status nb_create(nb_ctx** out, int device) {
         return hip_fail(e, "hipStreamCreate");
     }
     for (auto& ev : c->step_ev) (void)hipEventCreate(&ev);
+    // the static scratch's address on this device (each device has its own copy)
+    c->scratch_status = hipGetSymbolAddress((void**)&c->scratch_ptr, HIP_SYMBOL(scratch));
+    if (c->scratch_status != hipSuccess) c->scratch_ptr = nullptr;
     *out = c;
     nbh::clear_error();
     return NB_OK;
@@ -592,7 +612,7 @@ nb_status nb_accel_device(nb_ctx* c, int form, double eps) {
 // Shared body of the ring-share entry points: d_pi (ni records) against d_pj
 // (nj records) in j-blocks of jb, partials then one ordered reduction.
 static nb_status block_eval(nb_ctx* c, int kf, double eps, long ni, const P4* d_pi, long nj,
-                            const P4* d_pj, long jb, P4* d_out) {
+                            const P4* d_pj, long jb, P4* d_out, P4* fixed = nullptr, size_t fixed_bytes = 0) {
     const int ipt = choose_ipt(c, ni);
     const long nb = (nj + jb - 1) / jb;
     int S = choose_splits(c, ni, jb, ipt);
@@ -602,8 +622,17 @@ static nb_status block_eval(nb_ctx* c, int kf, double eps, long ni, const P4* d_
     const long parts = nb * S;
     P4* part = d_out;
     if (parts > 1) {
-        NB_HIP(c->part.ensure(size_t(parts) * ni * sizeof(P4)), "allocate partial sums");
-        part = (P4*)c->part.p;
+        const size_t need = size_t(parts) * size_t(ni) * sizeof(P4);
+        if (fixed) {   // partial sums in caller-provided memory: nothing allocated, nothing timed
+            if (need > fixed_bytes)
+                return fail(NB_ERR_ARG, "nb_accel_block_device_noalloc: " + std::to_string(parts) + " partial sums of " +
+                                            std::to_string(ni) + " records need " + std::to_string(need) +
+                                            " bytes; the static scratch holds " + std::to_string(fixed_bytes));
+            part = fixed;
+        } else {
+            NB_HIP(c->part.ensure(need), "allocate partial sums");
+            part = (P4*)c->part.p;
+        }
     }
     for (long b = 0; b < nb; b++) {
         nbk::AccelLaunch L;
@@ -621,7 +650,7 @@ static nb_status block_eval(nb_ctx* c, int kf, double eps, long ni, const P4* d_
         L.out = part;
         L.out_stride = ni;
         L.split_base = int(b * S);
-        const long ev = timing_begin(c);
+        const long ev = fixed ? -1 : timing_begin(c);
         hipError_t e = nbk::launch_accel(L, c->stream);
         timing_end(c, ev);
         NB_HIP(e, "force kernel");
@@ -736,6 +765,27 @@ static nb_status ring_args(nb_ctx* c, int form, double eps, size_t n, size_t ran
     kf = kernel_form(form);
     if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
     return NB_OK;
+}
+
+nb_status nb_accel_block_device_noalloc(nb_ctx* c, int form, double eps, size_t ni, const double* d_pi, size_t nj,
+                                        const double* d_pj, size_t j_block, double* d_acc4) {
+    int kf;
+    if (nb_status s = block_args(c, form, eps, ni, d_pi, nj, d_pj, d_acc4, "nb_accel_block_device_noalloc", kf))
+        return s;
+    if (ni == 0) return NB_OK;
+    if (!c->scratch_ptr)
+        return fail(NB_ERR_DEVICE, std::string("nb_accel_block_device_noalloc: no address of the static scratch on "
+                                               "this device: ") + hipGetErrorString(c->scratch_status));
+    NB_HIP(hipSetDevice(c->device), "hipSetDevice");
+    if (nj == 0) {   // a fill kernel, not a memset node, so the call can be captured
+        hipLaunchKernelGGL(k_zero_records, dim3(unsigned((ni + 255) / 256)), dim3(256), 0, c->stream, (P4*)d_acc4,
+                           long(ni));
+        NB_HIP(hipGetLastError(), "zero kernel");
+        return NB_OK;
+    }
+    const long jb = (j_block == 0 || j_block > nj) ? long(nj) : long(j_block);
+    return block_eval(c, kf, eps, long(ni), (const P4*)d_pi, long(nj), (const P4*)d_pj, jb, (P4*)d_acc4,
+                      c->scratch_ptr, kStaticScratchBytes);
 }
 
 nb_status nb_forces_ring_round(nb_ctx* c, int form, double eps, size_t local_n, const double* mass,

@@ -42,7 +42,7 @@ EXPORTS = (
     "nb_near_accel", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
     "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
     "nb_ring_predict", "nb_kernel_info_read", "nb_mlp_adam_step", "nb_forces_ring_round",
-    "nb_forces_ring_round_device",
+    "nb_forces_ring_round_device", "nb_accel_block_device_noalloc",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH):
     L.nb_accel_device.argtypes = [_VP, C.c_int, C.c_double]
     L.nb_accel_block.argtypes = [_VP, C.c_int, C.c_double, _SZ, _D, _SZ, _D, _SZ, _D]
     L.nb_accel_block_device.argtypes = [_VP, C.c_int, C.c_double, _SZ, _VP, _SZ, _VP, _SZ, _VP]
+    L.nb_accel_block_device_noalloc.argtypes = [_VP, C.c_int, C.c_double, _SZ, _VP, _SZ, _VP, _SZ, _VP]
     L.nb_step.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
     L.nb_energy.argtypes = [_VP, C.c_double, _D, _D]
     L.nb_timing_enable.argtypes = [_VP, C.c_int]
@@ -355,6 +356,12 @@ class Solver:
         _check(self.L.nb_forces_ring_round_device(self.h, _form(form), eps, n, d_local, d_local_force4, d_recv,
                                                   d_recv_force4, rank, comm_sz, recv_round),
                "nb_forces_ring_round_device")
+
+    def accel_block_device_noalloc(self, form, eps, ni, d_pi, nj, d_pj, j_block, d_out):
+        """As accel_block_device with the partial sums in the library's static
+        device scratch (nothing allocated); queued, not synchronised."""
+        _check(self.L.nb_accel_block_device_noalloc(self.h, _form(form), eps, ni, d_pi, nj, d_pj, j_block, d_out),
+               "nb_accel_block_device_noalloc")
 
     # --- integration ---------------------------------------------------------
     def step(self, steps: int, dt: float, order: int = 2, form="fp64", eps: float = 0.0, coeffs=None):
