@@ -58,8 +58,13 @@ typedef enum nb_form {
                              pair arithmetic, FP32 per-tile sums flushed to FP64 */
     NB_FORM_LEARNED = 2,  /* nb_step only: exact mixed sum every `period`-th
                              substep, learned extrapolation in between */
-    NB_FORM_MIXED_RAW = 3 /* as NB_FORM_MIXED without the Newton step (v_rsq_f32
+    NB_FORM_MIXED_RAW = 3, /* as NB_FORM_MIXED without the Newton step (v_rsq_f32
                              is already accurate to about 1 ulp) */
+    NB_FORM_FP32 = 4      /* every step in FP32: positions rounded to FP32, FP32
+                             differences, rsqrt + Newton step, FP32 sums -- the
+                             reference's -p sp kernels (compute_forces_avx2_blocked_sp,
+                             CPU/force_calculation.cpp:692-891); always on the
+                             tile kernel; not for the ring round */
 } nb_form;
 
 /* Initial conditions (system set-up).  NB_IC_RING is the reference's ring of

@@ -225,6 +225,7 @@ static SymRegions sym_regions(long nB, long S, long L, int kform) {
 // kernel otherwise.
 static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     const long n = long(c->n);
+    if (kform == nbk::FORM_FP32) return false;   // the FP32 form is a tile-kernel form
     // More particles per lane at large N (fewer LDS reads and rotations per
     // pair): 8 for FP64, 6 for the mixed forms, whose t-loop is left unrolled
     // once to fit the VGPRs (measured at N = 262144, mixed: K = 5 / 6 / 7 / 8
@@ -249,12 +250,18 @@ static const char* form_label(int kform) {
     switch (kform) {
     case nbk::FORM_FP64: return "FP64";
     case nbk::FORM_MIXED_NR: return "MIXED_NR";
+    case nbk::FORM_FP32: return "FP32";
     default: return "MIXED";
     }
 }
 
 static int public_form(int kform) {
-    return kform == nbk::FORM_FP64 ? NB_FORM_FP64 : (kform == nbk::FORM_MIXED_NR ? NB_FORM_MIXED : NB_FORM_MIXED_RAW);
+    switch (kform) {
+    case nbk::FORM_FP64: return NB_FORM_FP64;
+    case nbk::FORM_MIXED_NR: return NB_FORM_MIXED;
+    case nbk::FORM_FP32: return NB_FORM_FP32;
+    default: return NB_FORM_MIXED_RAW;
+    }
 }
 
 // Record what a force evaluation of the resident system ran (nb_kernel_info_read).
@@ -387,6 +394,7 @@ int kernel_form(int form) {
     case NB_FORM_FP64: return nbk::FORM_FP64;
     case NB_FORM_MIXED: return nbk::FORM_MIXED_NR;
     case NB_FORM_MIXED_RAW: return nbk::FORM_MIXED;
+    case NB_FORM_FP32: return nbk::FORM_FP32;
     default: return -1;
     }
 }
@@ -583,7 +591,7 @@ static nb_status accel_common(nb_ctx* c, int form, double eps, double* const out
     if (!out || !out[0] || !out[1] || !out[2]) return fail(NB_ERR_ARG, std::string(fn) + ": null output plane");
     if (nb_status s = check_eps(eps, fn)) return s;
     const int kf = kernel_form(form);
-    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
+    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED, NB_FORM_MIXED_RAW or NB_FORM_FP32");
     if (c->n == 0) return NB_OK;
     NB_HIP(hipSetDevice(c->device), "hipSetDevice");
     NB_HIP(accel_to_acc(c, kf, eps), "force kernel");
@@ -763,7 +771,8 @@ static nb_status ring_args(nb_ctx* c, int form, double eps, size_t n, size_t ran
         return fail(NB_ERR_ARG, std::string(fn) + ": comm_sz * local_n must stay below 1e8");
     if (nb_status s = check_eps(eps, fn)) return s;
     kf = kernel_form(form);
-    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
+    if (kf < 0 || kf == nbk::FORM_FP32)
+        return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
     return NB_OK;
 }
 
@@ -892,7 +901,7 @@ static nb_status driver_args(nb_ctx* c, double dt, int form, double eps, const c
     if (!std::isfinite(dt)) return fail(NB_ERR_ARG, std::string(fn) + ": dt must be finite");
     if (nb_status s = check_eps(eps, fn)) return s;
     kf = kernel_form(form);
-    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED or NB_FORM_MIXED_RAW");
+    if (kf < 0) return fail(NB_ERR_ARG, std::string(fn) + ": form must be NB_FORM_FP64, NB_FORM_MIXED, NB_FORM_MIXED_RAW or NB_FORM_FP32");
     return NB_OK;
 }
 

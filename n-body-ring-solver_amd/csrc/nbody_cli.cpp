@@ -15,12 +15,13 @@
 // hybrid runs) are not provided and end with a message.  -V is accepted for
 // compatibility (every vectorisation variant maps to the gfx950 kernels),
 // -S full selects the kernel that evaluates every ordered pair, and -p sp
-// selects the mixed-precision form.  The experiments' permutations come from
+// selects the all-FP32 form (the reference's single-precision kernels,
+// CPU/force_calculation.cpp:692-891) unless -form names another.  The experiments' permutations come from
 // a seeded generator (-seed) instead of the reference's std::random_device,
 // so their runs repeat.  Added options:
 //   -eps <e>       softening length (default 0: the reference's force)
 //   -ic ring|sphere|plummer, -seed <k>, -a <scale>, -rmax <r>
-//   -form fp64|mixed|mixed-raw|learned
+//   -form fp64|mixed|mixed-raw|fp32|learned
 //   -device <d>
 //   -period <k> -threshold <x> -rc <r> -train-steps <n> -train-iters <n>  (learned form)
 //   -snapshot <file>   initial and final state (with FP64 forces) as frames in
@@ -179,7 +180,7 @@ void parse(int argc, char** argv, Args& a) {
         printf("eps has to be >= 0!\n Resetting to 0\n");
         a.eps = 0.0;
     }
-    if (a.form.empty()) a.form = (a.prec == "sp") ? "mixed" : "fp64";
+    if (a.form.empty()) a.form = (a.prec == "sp") ? "fp32" : "fp64";
     if (a.out.empty()) a.out = "output.csv";   // the reference's default output file
 }
 
@@ -187,6 +188,7 @@ int form_id(const std::string& f) {
     if (f == "fp64") return NB_FORM_FP64;
     if (f == "mixed") return NB_FORM_MIXED;
     if (f == "mixed-raw") return NB_FORM_MIXED_RAW;
+    if (f == "fp32") return NB_FORM_FP32;
     if (f == "learned") return NB_FORM_LEARNED;
     printf("Unknown form: %s\n Defaulting to fp64\n", f.c_str());
     return NB_FORM_FP64;
@@ -426,7 +428,7 @@ FILE* open_table(const std::string& path, const char* header) {
 bool experiment_form(const Args& a, int& form) {
     form = form_id(a.form);
     if (form == NB_FORM_LEARNED) {
-        fprintf(stderr, "the accuracy experiments take fp64, mixed or mixed-raw forms\n");
+        fprintf(stderr, "the accuracy experiments take fp64, mixed, mixed-raw or fp32 forms\n");
         return false;
     }
     return true;

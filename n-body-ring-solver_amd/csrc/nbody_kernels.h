@@ -13,7 +13,10 @@ struct __attribute__((aligned(32))) P4 {
     double x, y, z, m;
 };
 
-enum Form { FORM_FP64 = 0, FORM_MIXED = 1, FORM_MIXED_NR = 2 };
+// FORM_FP32: every step in FP32 (positions rounded to FP32, FP32 differences,
+// rsqrt + Newton step, FP32 sums): the reference's -p sp kernels
+// (CPU/force_calculation.cpp:692-891); tile kernel only.
+enum Form { FORM_FP64 = 0, FORM_MIXED = 1, FORM_MIXED_NR = 2, FORM_FP32 = 3 };
 
 // Correction of the hardware reciprocal square root, shared by the pair
 // functions.  With y ~ r2^(-1/2) and s = r2 y^2 = 1 - e, the exact

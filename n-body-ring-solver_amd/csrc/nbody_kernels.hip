@@ -84,6 +84,26 @@ __device__ __forceinline__ void pair_mixed(double xi, double yi, double zi, doub
     az = __builtin_fmaf(s, dz, az);
 }
 
+// FP32 throughout (FORM_FP32): FP32 positions and differences, the mixed
+// form's rsqrt and Newton step, FP32 sums.
+__device__ __forceinline__ void pair_fp32(float xi, float yi, float zi, float xj, float yj, float zj, float mj,
+                                          float eps2, bool eps0, float& ax, float& ay, float& az) {
+    const float dx = xj - xi, dy = yj - yi, dz = zj - zi;
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float y2 = y * y;
+    float s = (mj * y) * y2 * __builtin_fmaf(-r2, y2, kNewtonOffset);
+    if (eps0) s = (r2 > 0.0f) ? s : 0.0f;
+    ax = __builtin_fmaf(s, dx, ax);
+    ay = __builtin_fmaf(s, dy, ay);
+    az = __builtin_fmaf(s, dz, az);
+}
+
+// LDS record of the FP32 form.
+struct __attribute__((aligned(16))) P4f {
+    float x, y, z, m;
+};
+
 // ---------------------------------------------------------------------------
 // Force kernel (asymmetric: all ordered pairs of i-set x j-range)
 // ---------------------------------------------------------------------------
@@ -93,11 +113,12 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
                                                  int jend_all, int jchunk, double eps2, double G,
                                                  double cshift, double coff, P4* __restrict__ out,
                                                  long out_stride, int split_base) {
+    constexpr bool F32 = (FORM == FORM_FP32);
     constexpr bool MIXED = (FORM != FORM_FP64);
-    constexpr bool NR = (FORM == FORM_MIXED_NR);
+    constexpr bool NR = (FORM == FORM_MIXED_NR) || F32;
     // the factor of the correction polynomial carried by the staged masses
     constexpr double MSCALE = MIXED ? (NR ? kNewtonMass : 1.0) : kCubicMass;
-    using Rec = typename std::conditional<MIXED, P4m, P4>::type;
+    using Rec = typename std::conditional<F32, P4f, typename std::conditional<MIXED, P4m, P4>::type>::type;
     __shared__ Rec tile[2][BLOCK];
 
     const int t = threadIdx.x;
@@ -119,10 +140,19 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
     const int ntiles = (jend > jbeg) ? (jend - jbeg + BLOCK - 1) / BLOCK : 0;
     P4 nxt = load_p4(pj, jbeg + t, jend);
     const float eps2f = (float)eps2;
+    // the FP32 form: positions rounded once, sums over the whole j-range in FP32
+    float xs[IPT], ys[IPT], zs[IPT], gx[IPT], gy[IPT], gz[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) {
+        xs[k] = (float)xi[k]; ys[k] = (float)yi[k]; zs[k] = (float)zi[k];
+        gx[k] = gy[k] = gz[k] = 0.0f;
+    }
 
     for (int tt = 0; tt < ntiles; tt++) {
         const int buf = tt & 1;
-        if constexpr (MIXED) {
+        if constexpr (F32) {
+            tile[buf][t] = P4f{(float)nxt.x, (float)nxt.y, (float)nxt.z, (float)(nxt.m * MSCALE)};
+        } else if constexpr (MIXED) {
             tile[buf][t] = P4m{nxt.x, nxt.y, nxt.z, (float)(nxt.m * MSCALE), 0.0f};
         } else {
             tile[buf][t] = P4{nxt.x, nxt.y, nxt.z, nxt.m * MSCALE};
@@ -133,7 +163,17 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
         const int cnt = min(BLOCK, jend - j0);
         const int cnt8 = (cnt + 7) & ~7;   // entries past jend are zero-mass records
         const Rec* tl = tile[buf];
-        if constexpr (MIXED) {
+        if constexpr (F32) {
+            for (int jj = 0; jj < cnt8; jj += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const P4f q = tl[jj + u];
+#pragma unroll
+                    for (int k = 0; k < IPT; k++)
+                        pair_fp32(xs[k], ys[k], zs[k], q.x, q.y, q.z, q.m, eps2f, EPS0, gx[k], gy[k], gz[k]);
+                }
+            }
+        } else if constexpr (MIXED) {
             float fx[IPT], fy[IPT], fz[IPT];
 #pragma unroll
             for (int k = 0; k < IPT; k++) fx[k] = fy[k] = fz[k] = 0.0f;
@@ -167,6 +207,12 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
         }
     }
 
+    if constexpr (F32) {
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            ax[k] = (double)gx[k]; ay[k] = (double)gy[k]; az[k] = (double)gz[k];
+        }
+    }
     P4* o = out + (long)(split_base + split) * out_stride;
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
@@ -508,6 +554,7 @@ hipError_t launch_accel(const AccelLaunch& L, hipStream_t st) {
     case FORM_FP64: return launch_accel_eps<FORM_FP64>(L, st);
     case FORM_MIXED: return launch_accel_eps<FORM_MIXED>(L, st);
     case FORM_MIXED_NR: return launch_accel_eps<FORM_MIXED_NR>(L, st);
+    case FORM_FP32: return launch_accel_eps<FORM_FP32>(L, st);
     default: return hipErrorInvalidValue;
     }
 }

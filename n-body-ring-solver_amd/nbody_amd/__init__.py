@@ -25,7 +25,7 @@ __all__ = [
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libnbody_hip.so")
 
-FORMS = {"fp64": 0, "mixed": 1, "learned": 2, "mixed-raw": 3}
+FORMS = {"fp64": 0, "mixed": 1, "learned": 2, "mixed-raw": 3, "fp32": 4}
 IC_KINDS = {"ring": 0, "sphere": 1, "plummer": 2}
 STATUS = {0: "NB_OK", 1: "NB_ERR_ARG", 2: "NB_ERR_DEVICE", 3: "NB_ERR_ALLOC", 4: "NB_ERR_STATE",
           5: "NB_ERR_LAUNCH"}

@@ -176,7 +176,7 @@ def test_cli_accuracy_experiment(tmp_path):
         if r[1] == "dp":
             assert length < 1e-10 and angle < 1e-10        # percent; FP64 sums
         else:
-            assert 1e-10 < length < 1e-2 and angle < 1e-2   # the mixed form
+            assert 1e-10 < length < 1e-2 and angle < 1e-2   # the all-FP32 form
 
 
 def test_cli_ratio_experiment_vs_oracle(orc, tmp_path):
@@ -213,7 +213,7 @@ def test_cli_csv_row_and_verbose(tmp_path):
     assert row[0] == "16" and row[1] == "3" and row[4] == "lf" and row[5] == "2" and row[9] == "fp64"
     cli("-t", "omp", "-N", "16", "-s", "3", "-T", "0.003", "-i", "rk", "-order", "4", "-p", "sp", "-o", csv)
     row = [x.strip() for x in open(csv).read().strip().splitlines()[2].split(",")]
-    assert row[4] == "rk" and row[5] == "4" and row[9] == "mixed"
+    assert row[4] == "rk" and row[5] == "4" and row[9] == "fp32"   # -p sp: the all-FP32 form
 
 
 def test_cli_learned_form():

@@ -47,6 +47,29 @@ def test_jitter_fixture_forces(solver, n):
     assert err_mixed < 3 * err_ref_sp + 1e-6   # no worse than the reference's FP32 form
 
 
+@pytest.mark.parametrize("name", ["jitter_64.npz", "jitter_300.npz", "ring_257.npz", "ring_2049.npz"])
+def test_fp32_form_at_the_reference_sp_error_level(solver, name, capsys):
+    """NB_FORM_FP32 (the CLI's -p sp) against the reference's own FP32 kernel
+    compute_forces_avx2_blocked_sp: both measured against the reference's
+    FP64 result, the GPU form's error is of the same order: no more than a
+    few times the reference's, and on the jittered systems above what the
+    mixed form reaches with its FP64 differences."""
+    g = golden(name)
+    upload(solver, g["m"], g["pos"])
+    # the ring's centre (the last particle) feels a force that is zero up to rounding
+    k = slice(0, -1) if name.startswith("ring") else slice(None)
+    ref64 = g["force_avx2"][:, k]
+    err32 = rel_err(solver.compute_forces("fp32", 0.0)[:, k], ref64)
+    err_ref = rel_err(g["force_sp"][:, k], ref64)
+    err_mixed = rel_err(solver.compute_forces("mixed", 0.0)[:, k], ref64)
+    assert solver.kernel_info()["kernel"] == "tile"
+    with capsys.disabled():
+        print(f"\n[fp32 {name}] fp32 {err32:.2e}, reference sp {err_ref:.2e}, mixed {err_mixed:.2e}")
+    assert err32 < 3 * err_ref
+    if name.startswith("jitter"):   # on the symmetric ring every form's error is the positions' rounding
+        assert err32 > err_mixed
+
+
 @pytest.mark.parametrize("kind,n,eps", [("sphere", 2048, 1e-3), ("plummer", 4096, 0.01), ("plummer", 1000, 0.0),
                                         ("sphere", 777, 0.05)])
 def test_forces_vs_oracle(solver, orc, kind, n, eps):
@@ -57,6 +80,7 @@ def test_forces_vs_oracle(solver, orc, kind, n, eps):
     assert rel_err(a, ref) < FP64_TOL
     for form in ("mixed", "mixed-raw"):
         assert rel_err(solver.accel(form, eps), ref) < MIXED_TOL, form
+    assert rel_err(solver.accel("fp32", eps), ref) < 1e-3
 
 
 @pytest.mark.parametrize("eps", [0.0, 1e-3, 0.05])
