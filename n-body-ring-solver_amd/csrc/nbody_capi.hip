@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -101,6 +102,15 @@ struct nb_ctx {
         DevBuf items;
     } sym[2];
     int sym_next = 0;
+    // grouped symmetric plans: per-workgroup chunk ranges and the action
+    // segments of each group row
+    struct SymgPlan {
+        size_t n = 0;
+        int K = 0, nB = 0, nG = 0, nwg = 0, nseg = 0;
+        long nslots = 0;
+        DevBuf ranges, rowseg;
+    } symg[2];
+    int symg_next = 0;
     nb_kernel_info kinfo{};   // the last force evaluation of the resident system
     P4* scratch_ptr = nullptr;          // this device's address of the static scratch array
     hipError_t scratch_status = hipSuccess;
@@ -203,14 +213,95 @@ static hipError_t sym_plan(nb_ctx* c, size_t n, int K, const nb_ctx::SymPlan*& p
     return hipSuccess;
 }
 
+struct SymRegions {
+    size_t act_bytes, react_bytes;
+};
+
+// Grouped plan: the chunk sequence (rows g of W blocks, j-blocks B >= g W,
+// chunks c < K) cut into equal ranges, one per resident workgroup (12 waves
+// per CU at 3 per SIMD); each workgroup's action slots (one per row it
+// touches) follow the previous workgroup's, so a row's segments are
+// consecutive slots.
+// 24 ranges per CU: filling the chip exactly once (3 workgroups per CU) left
+// it under-occupied (N = 262144: mixed 18.6 against 16.0 ms), many short
+// ranges let the dispatcher balance the load at the price of more action
+// segments (N = 262144, FP64 / mixed kernel ms at 3 / 12 / 24 / 48 / 96
+// ranges per CU: 24.2 / 23.6 / 23.0 / 23.5 / 23.8 and 18.6 / 16.6 / 16.0 /
+// 15.9 / 15.8, with the partial sums growing 0.60 / 0.75 / 0.95 / 1.35 /
+// 2.16 GB FP64).
+static int symg_workgroups(const nb_ctx* c, long units) {
+    return int(std::max(1L, std::min(long(c->num_cus) * 24, units)));
+}
+
+static long symg_slots(long nB, long W) {
+    const long nG = (nB + W - 1) / W;
+    return nG * nB - W * nG * (nG - 1) / 2;
+}
+
+static hipError_t symg_plan(nb_ctx* c, size_t n, int K, const nb_ctx::SymgPlan*& plan) {
+    for (const auto& Q : c->symg)
+        if (Q.n == n && Q.K == K) {
+            plan = &Q;
+            return hipSuccess;
+        }
+    auto& P = c->symg[c->symg_next];
+    c->symg_next ^= 1;
+    P.n = 0;
+    const long W = nbk::kGroupWaves, S = 64L * K;
+    const long nB = (long(n) + S - 1) / S, nG = (nB + W - 1) / W;
+    std::vector<long> rowstart(nG + 1, 0);
+    for (long g = 0; g < nG; g++) rowstart[g + 1] = rowstart[g] + (nB - g * W) * K;
+    const long U = rowstart[nG];
+    const int nwg = symg_workgroups(c, U);
+    std::vector<int4> ranges(nwg);
+    std::vector<int2> rowseg(nG, make_int2(0, 0));
+    long slot = 0, g = 0;
+    for (int wg = 0; wg < nwg; wg++) {
+        const long u0 = U * wg / nwg, u1 = U * (wg + 1) / nwg;
+        while (rowstart[g + 1] <= u0) g++;
+        long gl = g;
+        while (rowstart[gl + 1] < u1) gl++;   // row of the last chunk u1 - 1
+        const long off = u0 - rowstart[g];
+        ranges[wg] = make_int4(int(g), int(g * W * K + off), int(u1 - u0), int(slot));
+        for (long r = g; r <= gl; r++) {
+            if (rowseg[r].y == 0) rowseg[r].x = int(slot + (r - g));
+            rowseg[r].y++;
+        }
+        slot += gl - g + 1;
+    }
+    hipError_t e = P.ranges.ensure(ranges.size() * sizeof(int4));
+    if (e == hipSuccess) e = P.rowseg.ensure(rowseg.size() * sizeof(int2));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(P.ranges.p, ranges.data(), ranges.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(P.rowseg.p, rowseg.data(), rowseg.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    P.n = n;
+    P.K = K;
+    P.nB = int(nB);
+    P.nG = int(nG);
+    P.nwg = nwg;
+    P.nseg = int(slot);
+    P.nslots = symg_slots(nB, W);
+    plan = &P;
+    return hipSuccess;
+}
+
+// Partial sums of the grouped kernel: action slots (nseg of W blocks of S
+// records, 32 B) then reaction slots (nslots of S records: 32 B FP64, 16 B
+// FP32 for the mixed forms).  `nseg` at most workgroups + rows.
+static SymRegions symg_regions(long nB, long S, long nseg, int kform) {
+    const long W = nbk::kGroupWaves;
+    const size_t rec = (kform == nbk::FORM_FP64) ? sizeof(P4) : sizeof(float4);
+    return {size_t(nseg) * size_t(W) * size_t(S) * sizeof(P4), size_t(symg_slots(nB, W)) * size_t(S) * rec};
+}
+
 // Partial sums of the symmetric kernel: the action slots (nB*W slots of S
 // records of 32 B, W = (nB - 1)/L + 1) and after them the reaction slots
 // (nB(nB-1)/2 slots of S records: 32 B for FP64, 16 B FP32 for the mixed
 // forms).  N^2/32 + 32 N W bytes for FP64 at K = 8 (34 GB at N = 1048576),
 // N^2/48 + 32 N W for the mixed forms at K = 6.
-struct SymRegions {
-    size_t act_bytes, react_bytes;
-};
 static SymRegions sym_regions(long nB, long S, long L, int kform) {
     const long W = (nB - 1) / L + 1;
     const size_t rec = (kform == nbk::FORM_FP64) ? sizeof(P4) : sizeof(float4);
@@ -223,6 +314,12 @@ static SymRegions sym_regions(long nB, long S, long L, int kform) {
 // of the free device memory (N = 1M needs 35 GB FP64 / 24 GB mixed of the
 // 288 GB of an MI355X, so one device runs it to N ~ 2.6M FP64); the tile
 // kernel otherwise.
+// The grouped symmetric kernel from 131072 particles on (where the blocks hold
+// 8 / 6 particles per lane), the item kernel below it: with 4 per lane and
+// fewer chunks per range the item kernel measured faster (N = 32768 mixed
+// 283 against 294 us, N = 65536 1.00 against 1.04 ms per evaluation).
+static bool grouped_size(long n) { return n >= 131072; }
+
 static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     const long n = long(c->n);
     if (kform == nbk::FORM_FP32) return false;   // the FP32 form is a tile-kernel form
@@ -233,7 +330,13 @@ static bool use_symmetric(const nb_ctx* c, int kform, int& K) {
     K = (n >= 131072) ? (kform == nbk::FORM_FP64 ? 8 : 6) : 4;
     const long S = 64L * K;
     const long nB = (n + S - 1) / S;
-    const SymRegions R = sym_regions(nB, S, sym_item_tiles(c, nB), kform);
+    SymRegions R;
+    if (grouped_size(n)) {
+        const long W = nbk::kGroupWaves, nG = (nB + W - 1) / W;
+        R = symg_regions(nB, S, symg_workgroups(c, symg_slots(nB, W) * K) + nG, kform);
+    } else {
+        R = sym_regions(nB, S, sym_item_tiles(c, nB), kform);
+    }
     const double need = double(R.act_bytes) + double(R.react_bytes);
     bool fits = need <= 16.0 * (1L << 30) || need <= double(c->part.bytes);
     if (!fits) {
@@ -266,7 +369,7 @@ static int public_form(int kform) {
 
 // Record what a force evaluation of the resident system ran (nb_kernel_info_read).
 static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int block, uint64_t items,
-                        uint64_t scratch_bytes) {
+                        uint64_t scratch_bytes, bool grouped = false) {
     nb_kernel_info& k = c->kinfo;
     k.kernel = kernel;
     k.form = public_form(kform);
@@ -274,7 +377,10 @@ static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int bloc
     k.block = block;
     k.items = items;
     k.scratch_bytes = scratch_bytes;
-    if (kernel == NB_KERNEL_SYMMETRIC)
+    if (kernel == NB_KERNEL_SYMMETRIC && grouped)
+        std::snprintf(k.name, sizeof k.name, "k_accel_symg<%s, K=%d, W=%d>", form_label(kform), per_lane,
+                      nbk::kGroupWaves);
+    else if (kernel == NB_KERNEL_SYMMETRIC)
         std::snprintf(k.name, sizeof k.name, "k_accel_sym<%s, K=%d>", form_label(kform), per_lane);
     else
         std::snprintf(k.name, sizeof k.name, "k_accel<%s, IPT=%d>", form_label(kform), per_lane);
@@ -285,6 +391,40 @@ static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int bloc
 hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     const long n = long(c->n);
     int K;
+    if (use_symmetric(c, kform, K) && grouped_size(n)) {
+        const nb_ctx::SymgPlan* plan = nullptr;
+        hipError_t e = symg_plan(c, c->n, K, plan);
+        if (e != hipSuccess) return e;
+        const long S = 64L * K;
+        const SymRegions R = symg_regions(plan->nB, S, plan->nseg, kform);
+        e = c->part.ensure(R.act_bytes + R.react_bytes);
+        if (e != hipSuccess) return e;
+        char* const react = (char*)c->part.p + R.act_bytes;
+        nbk::SymgLaunch L;
+        L.form = kform;
+        L.K = K;
+        L.pm = (const P4*)c->pm.p;
+        L.n = int(n);
+        L.ranges = (const int4*)plan->ranges.p;
+        L.nwg = plan->nwg;
+        L.nB = plan->nB;
+        L.eps2 = eps * eps;
+        L.act = (P4*)c->part.p;
+        if (kform == nbk::FORM_FP64) L.react64 = (P4*)react;
+        else L.react32 = (float4*)react;
+        out.sym = true;
+        out.part = L.act;
+        out.react64 = L.react64;
+        out.react32 = L.react32;
+        out.nB = plan->nB;
+        out.S = int(S);
+        out.rowseg = (const int2*)plan->rowseg.p;
+        note_kernel(c, NB_KERNEL_SYMMETRIC, kform, K, int(S), uint64_t(plan->nwg), R.act_bytes + R.react_bytes, true);
+        const long ev = timing_begin(c);
+        e = nbk::launch_accel_symg(L, c->stream);
+        timing_end(c, ev);
+        return e;
+    }
     if (use_symmetric(c, kform, K)) {
         const nb_ctx::SymPlan* plan = nullptr;
         hipError_t e = sym_plan(c, c->n, K, plan);
@@ -359,6 +499,17 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     return e;
 }
 
+static nbk::SymgSums symg_sums(const ForceOut& out) {
+    nbk::SymgSums P;
+    P.act = out.part;
+    P.rowseg = out.rowseg;
+    P.react64 = out.react64;
+    P.react32 = out.react32;
+    P.nB = out.nB;
+    P.S = out.S;
+    return P;
+}
+
 static nbk::SymSums sym_sums(const ForceOut& out) {
     nbk::SymSums P;
     P.part = out.part;
@@ -373,6 +524,9 @@ static nbk::SymSums sym_sums(const ForceOut& out) {
 // acc = sum of the partials of `out`.
 hipError_t finish_accel(nb_ctx* c, const ForceOut& out) {
     const int n = int(c->n);
+    if (out.sym && out.rowseg)
+        return nbk::launch_symg_finish(symg_sums(out), n, nullptr, nullptr, (P4*)c->acc.p, false, 0.0, 0.0, 0.0,
+                                       c->stream);
     if (out.sym) return nbk::launch_sym_reduce(sym_sums(out), n, (P4*)c->acc.p, c->stream);
     if (out.S == 1) return hipSuccess;
     return nbk::launch_reduce(out.part, out.S, long(n), n, (P4*)c->acc.p, c->stream);
@@ -384,6 +538,8 @@ hipError_t finish_kick_drift(nb_ctx* c, const ForceOut& out, double kdt, double 
     P4* pm = (P4*)c->pm.p;
     P4* vel = (P4*)c->vel.p;
     P4* acc = (P4*)c->acc.p;
+    if (out.sym && out.rowseg)
+        return nbk::launch_symg_finish(symg_sums(out), n, pm, vel, acc, true, kdt, c1dt, c2dt, c->stream);
     if (out.sym)
         return nbk::launch_sym_kick_drift(sym_sums(out), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
     return nbk::launch_kick_drift(out.part, out.S, long(n), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
@@ -460,7 +616,7 @@ nb_status nb_destroy(nb_ctx* c) {
     nbl::destroy(c->learned);
     c->learned = nullptr;
     for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->bout2,
-                      &c->rk,
+                      &c->rk, &c->symg[0].ranges, &c->symg[0].rowseg, &c->symg[1].ranges, &c->symg[1].rowseg,
                       &c->sym[0].items, &c->sym[1].items})
         b->release();
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
@@ -727,6 +883,41 @@ nb_status nb_accel_block_device(nb_ctx* c, int form, double eps, size_t ni, cons
 static nb_status ring_round_eval(nb_ctx* c, int kf, double eps, long n, const P4* d_l, const P4* d_r, bool incl,
                                  P4* d_fl, P4* d_fr) {
     const int K = (n >= 131072) ? (kf == nbk::FORM_FP64 ? 8 : 6) : 4;
+    if (grouped_size(n)) {
+        const nb_ctx::SymgPlan* plan = nullptr;
+        NB_HIP(symg_plan(c, size_t(n), K, plan), "symmetric plan");
+        const long S = 64L * K;
+        const SymRegions R = symg_regions(plan->nB, S, plan->nseg, kf);
+        NB_HIP(c->part.ensure(R.act_bytes + R.react_bytes), "allocate partial sums");
+        char* const react = (char*)c->part.p + R.act_bytes;
+        nbk::SymgLaunch L;
+        L.form = kf;
+        L.K = K;
+        L.pm = d_l;
+        L.pj = d_r;
+        L.incl = incl;
+        L.n = int(n);
+        L.ranges = (const int4*)plan->ranges.p;
+        L.nwg = plan->nwg;
+        L.nB = plan->nB;
+        L.eps2 = eps * eps;
+        L.act = (P4*)c->part.p;
+        if (kf == nbk::FORM_FP64) L.react64 = (P4*)react;
+        else L.react32 = (float4*)react;
+        const long ev = timing_begin(c);
+        hipError_t e = nbk::launch_accel_symg(L, c->stream);
+        timing_end(c, ev);
+        NB_HIP(e, "force kernel");
+        nbk::SymgSums P;
+        P.act = L.act;
+        P.rowseg = (const int2*)plan->rowseg.p;
+        P.react64 = L.react64;
+        P.react32 = L.react32;
+        P.nB = plan->nB;
+        P.S = int(S);
+        NB_HIP(nbk::launch_symg_ring_finish(P, int(n), d_l, d_r, d_fl, d_fr, c->stream), "ring finish kernel");
+        return NB_OK;
+    }
     const nb_ctx::SymPlan* plan = nullptr;
     NB_HIP(sym_plan(c, size_t(n), K, plan), "symmetric plan");
     const long S = 64L * K;

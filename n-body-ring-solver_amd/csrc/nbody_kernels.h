@@ -88,6 +88,46 @@ struct SymSums {
 hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st);
 hipError_t launch_sym_kick_drift(const SymSums& P, int n, P4* pm, P4* vel, P4* acc, double kdt, double c1dt,
                                  double c2dt, hipStream_t st);
+// Grouped symmetric kernel (nbody_sym.hip): workgroups of kGroupWaves waves
+// over W consecutive blocks, each walking a contiguous range of the chunk
+// sequence; ranges[wg] = (g, B * K + c, chunks, first action slot); action
+// slots of kGroupWaves * S records; reaction slots (g, B), B >= g W, of S
+// records (FP32 for the mixed forms).
+// four waves: three workgroups a CU at three waves per SIMD; groups of two
+// measured slower (N = 262144 FP64 33.4 against 24.4 ms)
+constexpr int kGroupWaves = 4;
+struct SymgLaunch {
+    int form = FORM_FP64;
+    int K = 8;
+    const P4* pm = nullptr;
+    const P4* pj = nullptr;   // two-set form: the received set
+    bool incl = false;
+    int n = 0;
+    const int4* ranges = nullptr;
+    int nwg = 0;
+    int nB = 0;
+    double eps2 = 0.0;
+    P4* act = nullptr;
+    P4* react64 = nullptr;
+    float4* react32 = nullptr;
+};
+hipError_t launch_accel_symg(const SymgLaunch& L, hipStream_t st);
+
+// The grouped kernel's partial sums: rowseg[g] = (first action slot of row g,
+// number of its segments).
+struct SymgSums {
+    const P4* act = nullptr;
+    const int2* rowseg = nullptr;
+    const P4* react64 = nullptr;
+    const float4* react32 = nullptr;
+    int nB = 0, S = 0;
+};
+// acc = the sums (kick: then v += kdt a, x += c1dt v + c2dt v as the kick/drift kernels)
+hipError_t launch_symg_finish(const SymgSums& P, int n, P4* pm, P4* vel, P4* acc, bool kick, double kdt, double c1dt,
+                              double c2dt, hipStream_t st);
+hipError_t launch_symg_ring_finish(const SymgSums& P, int n, const P4* pl, const P4* pr, P4* fl, P4* fr,
+                                   hipStream_t st);
+
 // Two-set finish: fl[i] += m_l a_l (actions), fr[i] += m_r a_r (reactions
 // incl. the diagonal slots), masses from the records pl / pr.
 hipError_t launch_ring_finish(const SymSums& P, int n, const P4* pl, const P4* pr, P4* fl, P4* fr, hipStream_t st);

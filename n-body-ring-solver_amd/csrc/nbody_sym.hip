@@ -260,6 +260,296 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
     for (int k = 0; k < K; k++) part[(long)act * S + k * 64 + lane] = P4{ax[k], ay[k], az[k], 0.0};
 }
 
+// ---------------------------------------------------------------------------
+// Grouped, persistent form.  A workgroup of W waves holds W consecutive
+// blocks (a group g: blocks gW .. gW+W-1, wave w block A = gW + w) and walks a
+// contiguous range of the chunk sequence (g, B, c) -- rows g in order, j-blocks
+// B = gW .. nB-1, chunks c = 0 .. K-1 -- that the host cut into equal parts,
+// one per resident workgroup.  Per chunk: the workgroup stages the 64 records
+// once (its 256 doubles spread over the threads, prefetched a chunk ahead), every wave sweeps
+// them against its own block (B < A: nothing; B == A: the diagonal mask), the
+// waves' reaction sums meet in LDS and wave 0 adds them in wave order (FP64)
+// into one record per chunk particle of reaction slot (g, B),
+// g nB - W g(g-1)/2 + (B - gW).  The actions stay in registers along the row
+// and are written once per row segment, to action slot seg (W blocks of S
+// records); the segments of a row are consecutive.  Sums are in a fixed
+// order: results are bitwise reproducible.
+// ---------------------------------------------------------------------------
+
+// One chunk (64 records staged twice in lds) against the lane's K particles;
+// the chunk's reaction sums end in their home lanes.  DIAG: the chunk is the
+// c-th of the wave's own block (pairs i < j only; i <= j with incl).
+template <int FORM, bool EPS0, int K, bool DIAG>
+__device__ __forceinline__ void sweep_chunk(const P4* __restrict__ lds, int lane, int c, int tmin,
+                                            const double (&xi)[K], const double (&yi)[K], const double (&zi)[K],
+                                            const double (&mi)[K], const float (&mif)[K], double eps2,
+                                            double cshift, double coff, double (&ax)[K], double (&ay)[K],
+                                            double (&az)[K], double& bx, double& by, double& bz) {
+    constexpr bool MIXED = (FORM != FORM_FP64);
+    constexpr bool NR = (FORM == FORM_MIXED_NR);
+    if constexpr (!MIXED) {
+        bx = by = bz = 0.0;
+#pragma unroll 2
+        for (int t = 0; t < 64; t++) {
+            const P4 J = lds[lane + t];
+            const bool same_ok = (t >= tmin) && (lane + t < 64);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                sym64<EPS0, DIAG>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
+                                  (k < c) || (k == c && same_ok), ax[k], ay[k], az[k], bx, by, bz);
+            bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
+        }
+    } else {
+        const float eps2f = (float)eps2;
+        float fbx = 0.f, fby = 0.f, fbz = 0.f;
+        float fax[K], fay[K], faz[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) fax[k] = fay[k] = faz[k] = 0.f;
+#pragma unroll 1
+        for (int t = 0; t < 64; t++) {
+            const P4 J = lds[lane + t];
+            const float mj = (float)J.m;
+            const bool same_ok = (t >= tmin) && (lane + t < 64);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                sym32<EPS0, DIAG, NR>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f,
+                                      (k < c) || (k == c && same_ok), fax[k], fay[k], faz[k], fbx, fby, fbz);
+            fbx = rotate_down(fbx); fby = rotate_down(fby); fbz = rotate_down(fbz);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            ax[k] += (double)fax[k];
+            ay[k] += (double)fay[k];
+            az[k] += (double)faz[k];
+        }
+        bx = (double)fbx; by = (double)fby; bz = (double)fbz;
+    }
+}
+
+template <int FORM, bool EPS0, int K, int W, bool TWO>
+__global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm, const P4* __restrict__ pj, int n,
+                                                       const int4* __restrict__ ranges, int nB, double eps2,
+                                                       double cshift, double coff, int incl, P4* __restrict__ act,
+                                                       P4* __restrict__ react64, float4* __restrict__ react32) {
+    constexpr bool MIXED = (FORM != FORM_FP64);
+    constexpr bool NR = (FORM == FORM_MIXED_NR);
+    constexpr int S = 64 * K;
+    constexpr double MSCALE = MIXED ? (NR ? kNewtonMass : 1.0) : kCubicMass;
+    __shared__ P4 lds[128];             // the chunk, stored twice so lane + t never wraps
+    __shared__ double red[W][3][64];    // the waves' reaction sums of the chunk
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x;
+    const int4 rg = ranges[blockIdx.x];   // (g, B * K + c, chunks, first action slot)
+    int g = rg.x, B = rg.y / K, c = rg.y - (rg.y / K) * K, slot = rg.w;
+    const int count = rg.z;
+    if (count <= 0) return;   // the whole workgroup leaves together
+    const P4* __restrict__ src = TWO ? pj : pm;
+    const int tmin = TWO ? 1 - incl : 1;
+
+    double xi[K], yi[K], zi[K], mi[K], ax[K], ay[K], az[K];
+    float mif[K];
+    auto load_block = [&](int A) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const P4 p = load_or_zero(pm, A * S + k * 64 + lane, n);
+            xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.m * MSCALE; mif[k] = (float)mi[k];
+            ax[k] = ay[k] = az[k] = 0.0;
+        }
+        // settle these loads here (once per row): otherwise the sweeps, which
+        // read them, wait at their first pair for every load in flight --
+        // the next chunk's prefetch included (s_waitcnt vmcnt(0): vmcnt 0,
+        // expcnt and lgkmcnt at their maxima)
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+    };
+    auto flush = [&]() {
+        P4* __restrict__ o = act + ((long)slot * W + wave) * S + lane;
+#pragma unroll
+        for (int k = 0; k < K; k++) o[k * 64] = P4{ax[k], ay[k], az[k], 0.0};
+    };
+    // the chunk's 256 doubles spread over the threads (NPT each; indexes past
+    // 255 stage nothing); the mass takes its factor when staged, so nothing
+    // waits for the load before the sweep it overlaps
+    constexpr int NPT = (256 + 64 * W - 1) / (64 * W);
+    auto fetch = [&](int b, int ch, double (&v)[NPT]) {
+#pragma unroll
+        for (int d = 0; d < NPT; d++) {
+            const int idx = tid + d * 64 * W;
+            const int j = b * S + ch * 64 + (idx >> 2);
+            v[d] = (idx < 256 && j < n) ? ((const double*)(src + j))[idx & 3] : 0.0;
+        }
+    };
+    // wave 0's summed reactions of the previous chunk: stored after the next
+    // barrier, so the store is in flight during a sweep instead of being
+    // waited for at a barrier
+    long pend_at = -1;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    auto store_pending = [&]() {
+        if (pend_at < 0) return;
+        if constexpr (MIXED) react32[pend_at] = make_float4((float)px, (float)py, (float)pz, 0.0f);
+        else react64[pend_at] = P4{px, py, pz, 0.0};
+    };
+    int A = g * W + wave;
+    load_block(A);
+    double nxt[NPT];
+    fetch(B, c, nxt);
+    for (int step = 0; step < count; step++) {
+#pragma unroll
+        for (int d = 0; d < NPT; d++) {
+            const int idx = tid + d * 64 * W;
+            if (idx < 256) {
+                double* l = (double*)lds;
+                const double v = (idx & 3) == 3 ? nxt[d] * MSCALE : nxt[d];
+                l[idx] = v;
+                l[idx + 256] = v;
+            }
+        }
+        __syncthreads();   // the chunk is staged; the last chunk's reaction sums have been read
+        // prefetch the next chunk (past the last row it lies beyond n: zeros,
+        // nothing loaded), then store the last chunk's reactions
+        int g2 = g, B2 = B, c2 = c + 1;
+        if (c2 == K) {
+            c2 = 0;
+            if (++B2 == nB) { ++g2; B2 = g2 * W; }
+        }
+        fetch(B2, c2, nxt);
+        if (wave == 0) store_pending();
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        if (B > A) sweep_chunk<FORM, EPS0, K, false>(lds, lane, c, tmin, xi, yi, zi, mi, mif, eps2, cshift, coff, ax,
+                                                     ay, az, bx, by, bz);
+        else if (B == A) sweep_chunk<FORM, EPS0, K, true>(lds, lane, c, tmin, xi, yi, zi, mi, mif, eps2, cshift, coff,
+                                                          ax, ay, az, bx, by, bz);
+        red[wave][0][lane] = bx;
+        red[wave][1][lane] = by;
+        red[wave][2][lane] = bz;
+        __syncthreads();   // every wave's reactions of this chunk are in LDS; the chunk is free
+        if (wave == 0) {
+            px = py = pz = 0.0;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                px += red[w][0][lane]; py += red[w][1][lane]; pz += red[w][2][lane];
+            }
+            const long rslot = (long)g * nB - (long)W * g * (g - 1) / 2 + (B - g * W);
+            pend_at = rslot * S + c * 64 + lane;
+        }
+        if (g2 != g) {   // the row ends: its actions go to their slot, the next row's block is loaded
+            flush();
+            ++slot;
+            A = g2 * W + wave;
+            load_block(A);
+        }
+        g = g2; B = B2; c = c2;
+    }
+    if (wave == 0) store_pending();
+    if (c != 0 || B != g * W) flush();   // the range ended inside a row
+}
+
+// Sums of the grouped form for particle i of block X (group gX = X / W):
+// reactions from slots (g, X), g = 0 .. gX; actions from the segments of row
+// gX (slots seg0 .. seg0 + nseg - 1, the wave X - gX W block of each).
+template <bool F32R, int W>
+__device__ __forceinline__ void symg_react(const P4* __restrict__ react64, const float4* __restrict__ react32,
+                                           int nB, int S, int i, double& bx, double& by, double& bz) {
+    constexpr int U = 8;   // slots loaded together, added in slot order
+    const int X = i / S, r = i - X * S, gX = X / W;
+    bx = by = bz = 0.0;
+    auto at = [&](int g) { return ((long)g * nB - (long)W * g * (g - 1) / 2 + (X - g * W)) * S + r; };
+    int g = 0;
+    for (; g + U <= gX + 1; g += U) {
+        if constexpr (F32R) {
+            float4 q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = react32[at(g + u)];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                bx += (double)q[u].x; by += (double)q[u].y; bz += (double)q[u].z;
+            }
+        } else {
+            P4 q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = react64[at(g + u)];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                bx += q[u].x; by += q[u].y; bz += q[u].z;
+            }
+        }
+    }
+    for (; g <= gX; g++) {
+        if constexpr (F32R) {
+            const float4 q = react32[at(g)];
+            bx += (double)q.x; by += (double)q.y; bz += (double)q.z;
+        } else {
+            const P4 q = react64[at(g)];
+            bx += q.x; by += q.y; bz += q.z;
+        }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void symg_act(const P4* __restrict__ act, const int2* __restrict__ rowseg, int S, int i,
+                                         double& ax, double& ay, double& az) {
+    const int X = i / S, r = i - X * S, gX = X / W, w = X - gX * W;
+    const int2 sg = rowseg[gX];
+    ax = ay = az = 0.0;
+    for (int q = 0; q < sg.y; q++) {
+        const P4 p = act[((long)(sg.x + q) * W + w) * S + r];
+        ax += p.x; ay += p.y; az += p.z;
+    }
+}
+
+template <bool F32R, int W, bool KICK>
+__global__ __launch_bounds__(BLOCK) void k_symg_finish(const P4* __restrict__ act, const int2* __restrict__ rowseg,
+                                                       const P4* __restrict__ react64,
+                                                       const float4* __restrict__ react32, int nB, int S, int n,
+                                                       P4* __restrict__ pm, P4* __restrict__ vel,
+                                                       P4* __restrict__ acc, double kdt, double c1dt, double c2dt) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double bx, by, bz, ax, ay, az;
+    symg_react<F32R, W>(react64, react32, nB, S, i, bx, by, bz);
+    symg_act<W>(act, rowseg, S, i, ax, ay, az);
+    ax += bx; ay += by; az += bz;
+    acc[i] = P4{ax, ay, az, 0.0};
+    if constexpr (KICK) {
+        P4 v = vel[i];
+        v.x = __builtin_fma(kdt, ax, v.x);
+        v.y = __builtin_fma(kdt, ay, v.y);
+        v.z = __builtin_fma(kdt, az, v.z);
+        vel[i] = v;
+        P4 p = pm[i];
+        p.x = __builtin_fma(c1dt, v.x, p.x);
+        p.y = __builtin_fma(c1dt, v.y, p.y);
+        p.z = __builtin_fma(c1dt, v.z, p.z);
+        p.x = __builtin_fma(c2dt, v.x, p.x);
+        p.y = __builtin_fma(c2dt, v.y, p.y);
+        p.z = __builtin_fma(c2dt, v.z, p.z);
+        pm[i] = p;
+    }
+}
+
+// Two-set finish of the grouped form: local forces from the actions, the
+// received set's from the reactions, times the particles' masses, added.
+template <bool F32R, int W>
+__global__ __launch_bounds__(BLOCK) void k_symg_ring_finish(const P4* __restrict__ act, const int2* __restrict__ rowseg,
+                                                            const P4* __restrict__ react64,
+                                                            const float4* __restrict__ react32, int nB, int S, int n,
+                                                            const P4* __restrict__ pl, const P4* __restrict__ pr,
+                                                            P4* __restrict__ fl, P4* __restrict__ fr) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double bx, by, bz, ax, ay, az;
+    symg_react<F32R, W>(react64, react32, nB, S, i, bx, by, bz);
+    symg_act<W>(act, rowseg, S, i, ax, ay, az);
+    const double ml = pl[i].m, mr = pr[i].m;
+    P4 f = fl[i];
+    f.x += ml * ax; f.y += ml * ay; f.z += ml * az;
+    fl[i] = f;
+    P4 h = fr[i];
+    h.x += mr * bx; h.y += mr * by; h.z += mr * bz;
+    fr[i] = h;
+}
+
 // a_i for i in block X = sum of the reactions from blocks Y = 0 .. X-1, then
 // of the actions of the items (X, b0 = X, X+L, ..) of row X, in that order.
 // F32R: the reactions are the FP32 records of react32 (the mixed forms), else
@@ -448,6 +738,80 @@ hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st) {
     case FORM_MIXED_NR: return launch_sym_eps<FORM_MIXED_NR>(L, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int FORM, bool EPS0, int K>
+static hipError_t launch_symg_k(const SymgLaunch& L, hipStream_t st) {
+    constexpr int W = kGroupWaves;
+    if (L.pj)
+        hipLaunchKernelGGL((k_accel_symg<FORM, EPS0, K, W, true>), dim3(L.nwg), dim3(64 * W), 0, st, L.pm, L.pj, L.n,
+                           L.ranges, L.nB, L.eps2, kCubicShift, kCubicOffset, L.incl ? 1 : 0, L.act, L.react64,
+                           L.react32);
+    else
+        hipLaunchKernelGGL((k_accel_symg<FORM, EPS0, K, W, false>), dim3(L.nwg), dim3(64 * W), 0, st, L.pm, L.pm,
+                           L.n, L.ranges, L.nB, L.eps2, kCubicShift, kCubicOffset, 0, L.act, L.react64, L.react32);
+    return hipGetLastError();
+}
+
+template <int FORM, bool EPS0>
+static hipError_t launch_symg_eps(const SymgLaunch& L, hipStream_t st) {
+    switch (L.K) {
+    case 8: return launch_symg_k<FORM, EPS0, 8>(L, st);
+    case 6: return launch_symg_k<FORM, EPS0, 6>(L, st);
+    case 4: return launch_symg_k<FORM, EPS0, 4>(L, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int FORM>
+static hipError_t launch_symg_form(const SymgLaunch& L, hipStream_t st) {
+    return (L.eps2 == 0.0) ? launch_symg_eps<FORM, true>(L, st) : launch_symg_eps<FORM, false>(L, st);
+}
+
+hipError_t launch_accel_symg(const SymgLaunch& L, hipStream_t st) {
+    if (L.nwg <= 0) return hipSuccess;
+    if (L.form == FORM_FP64 ? !L.react64 : !L.react32) return hipErrorInvalidValue;
+    switch (L.form) {
+    case FORM_FP64: return launch_symg_form<FORM_FP64>(L, st);
+    case FORM_MIXED: return launch_symg_form<FORM_MIXED>(L, st);
+    case FORM_MIXED_NR: return launch_symg_form<FORM_MIXED_NR>(L, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_symg_finish(const SymgSums& P, int n, P4* pm, P4* vel, P4* acc, bool kick, double kdt, double c1dt,
+                              double c2dt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!P.react64 == !P.react32) return hipErrorInvalidValue;
+    constexpr int W = kGroupWaves;
+    const dim3 grid((n + BLOCK - 1) / BLOCK);
+#define NB_SYMG_FINISH(F32R, KICK)                                                                                 \
+    hipLaunchKernelGGL((k_symg_finish<F32R, W, KICK>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,        \
+                       P.react32, P.nB, P.S, n, pm, vel, acc, kdt, c1dt, c2dt)
+    if (P.react32) {
+        if (kick) NB_SYMG_FINISH(true, true);
+        else NB_SYMG_FINISH(true, false);
+    } else {
+        if (kick) NB_SYMG_FINISH(false, true);
+        else NB_SYMG_FINISH(false, false);
+    }
+#undef NB_SYMG_FINISH
+    return hipGetLastError();
+}
+
+hipError_t launch_symg_ring_finish(const SymgSums& P, int n, const P4* pl, const P4* pr, P4* fl, P4* fr,
+                                   hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!P.react64 == !P.react32) return hipErrorInvalidValue;
+    constexpr int W = kGroupWaves;
+    const dim3 grid((n + BLOCK - 1) / BLOCK);
+    if (P.react32)
+        hipLaunchKernelGGL((k_symg_ring_finish<true, W>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,
+                           P.react32, P.nB, P.S, n, pl, pr, fl, fr);
+    else
+        hipLaunchKernelGGL((k_symg_ring_finish<false, W>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,
+                           P.react32, P.nB, P.S, n, pl, pr, fl, fr);
+    return hipGetLastError();
 }
 
 hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st) {

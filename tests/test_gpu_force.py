@@ -333,8 +333,8 @@ def test_ring_share_full_size_sampled(solver, orc):
     ("mixed", 16383, "tile", "k_accel<MIXED_NR, IPT=1>"),
     ("fp64", 16384, "symmetric", "k_accel_sym<FP64, K=4>"),
     ("mixed-raw", 40000, "symmetric", "k_accel_sym<MIXED, K=4>"),
-    ("fp64", 262144, "symmetric", "k_accel_sym<FP64, K=8>"),
-    ("mixed", 262144, "symmetric", "k_accel_sym<MIXED_NR, K=6>"),
+    ("fp64", 262144, "symmetric", "k_accel_symg<FP64, K=8, W=4>"),
+    ("mixed", 262144, "symmetric", "k_accel_symg<MIXED_NR, K=6, W=4>"),
 ])
 def test_kernel_info_names_the_kernel_that_ran(solver, form, n, kernel, name):
     """nb_kernel_info_read reports the kernel the library chose for the last
