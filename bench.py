@@ -415,7 +415,7 @@ def main():
         r = forms.get(form, {})
         if r.get("exact_force_evals") and r.get("force_kernel"):
             k = r["force_kernel"]
-            traffic = profile_traffic("k_accel_sym_plummer_n262144_pmc.md",
+            traffic = profile_traffic("k_accel_symg_plummer_n262144_pmc.md",
                                       f"traffic_bytes_per_launch {k['name']}") if profiled else None
             rooflines[form] = roofline_entry(k, note, r["force_kernel_ms_avg"], n, peak, traffic)
     dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
