@@ -368,6 +368,15 @@ NB_API nb_status nb_learned_set_near_radius(nb_ctx* ctx, double rc);
  * on FP64 differences (what the learned form uses with that exact form). */
 NB_API nb_status nb_near_accel(nb_ctx* ctx, int form, double rc, double eps, double* const acc[3]);
 
+/* The same sums on the cell grid of this context's last near-field build
+ * (nb_near_accel, or the learned form's exact substeps) when it was built
+ * for the same particle count and radius, else on a new grid: the current
+ * positions are gathered in the grid's order and the cells searched reach
+ * farther by the largest displacement since the build, measured on the
+ * device, so the same pairs are summed (in another order) however far the
+ * particles moved.  What the learned form's extrapolated substeps use. */
+NB_API nb_status nb_near_accel_kept(nb_ctx* ctx, int form, double rc, double eps, double* const acc[3]);
+
 /* 0: cubic extrapolation corrected by the network (default once trained);
  * 1: cubic extrapolation alone (the method without the network). */
 NB_API nb_status nb_learned_set_mode(nb_ctx* ctx, int mode);

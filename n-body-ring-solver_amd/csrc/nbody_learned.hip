@@ -751,13 +751,16 @@ nb_status near_field(nb_ctx* c, Learned& L, double eps) {
 // queued so far on the main stream; near_field_join() makes the main stream
 // wait for it.  On an exact substep it runs beside the all-pairs kernel and
 // fills the CUs that kernel's waves leave idle; on an extrapolated one beside
-// the extrapolation and the exact sums of the flagged particles.
-nb_status near_field_fork(nb_ctx* c, Learned& L, double eps) {
+// the extrapolation and the exact sums of the flagged particles.  rebuild:
+// sort the particles into a new grid (exact substeps); otherwise the grid of
+// the last exact substep is searched wider by the displacement since (the
+// same pairs, without the sort).
+nb_status near_field_fork(nb_ctx* c, Learned& L, double eps, bool rebuild) {
     if (!(L.rc > 0)) return NB_OK;
     LH(hipEventRecord(L.ev_fork, nbh::ctx_stream(c)), "record event");
     LH(hipStreamWaitEvent(L.side, L.ev_fork, 0), "stream wait");
     LH(L.near.near_accel(nbh::ctx_pm(c), (int)L.n, L.rc, eps, (P4*)L.nearacc.p, L.side,
-                         L.exact_kform != nbk::FORM_FP64),
+                         L.exact_kform != nbk::FORM_FP64, rebuild),
        "near-field kernels");
     LH(hipEventRecord(L.ev_join, L.side), "record event");
     return NB_OK;
@@ -881,7 +884,7 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 if (full) {
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, false, true)) return e;
                 }
-                if (nb_status e = near_field_fork(c, L, eps)) return e;
+                if (nb_status e = near_field_fork(c, L, eps, true)) return e;
                 if (nb_status e = exact_force(c, L.exact_kform, eps)) return e;
                 if (nb_status e = near_field_join(c, L)) return e;
                 L.st.exact_evals++;
@@ -897,7 +900,7 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 // stream extrapolates, reads the flag count back and sums the
                 // flagged particles exactly
                 if (L.rc > 0) {
-                    if (nb_status e = near_field_fork(c, L, eps)) return e;
+                    if (nb_status e = near_field_fork(c, L, eps, false)) return e;
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, true, L.net_ok)) return e;
                     if (nb_status e = take_flag_count(L)) return e;
                     if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
@@ -921,6 +924,36 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
             if (nb_status e = kick_drift(c, q)) return e;
             L.clock += (q.c1 + q.c2) / dt;
         }
+    }
+    return NB_OK;
+}
+
+// nb_near_accel and nb_near_accel_kept: the near field of the resident state
+// into host planes, on a new grid (rebuild) or the kept one.
+nb_status near_accel_host(nb_ctx* c, int form, double rc, double eps, double* const acc[3], bool rebuild,
+                          const char* name) {
+    if (!c || !acc || !acc[0] || !acc[1] || !acc[2]) return nbh::set_error(NB_ERR_ARG, std::string(name) + ": null argument");
+    const int kf = nbh::kernel_form(form);
+    if (kf < 0) return nbh::set_error(NB_ERR_ARG, std::string(name) + ": form must be FP64 or a mixed form");
+    if (!(rc > 0.0) || !std::isfinite(rc) || !(eps >= 0.0))
+        return nbh::set_error(NB_ERR_ARG, std::string(name) + ": need rc > 0, eps >= 0");
+    const size_t n = nbh::ctx_n(c);
+    if (n == 0) return NB_OK;
+    Learned* L = get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, std::string(name) + ": out of host memory");
+    hipStream_t st = nbh::ctx_stream(c);
+    nbh::DevBuf out;
+    LH(out.ensure(n * sizeof(P4)), "allocate output");
+    LH(L->near.near_accel(nbh::ctx_pm(c), (int)n, rc, eps, (P4*)out.p, st, kf != nbk::FORM_FP64, rebuild),
+       "near-field kernels");
+    std::vector<P4> h(n);
+    LH(hipMemcpyAsync(h.data(), out.p, n * sizeof(P4), hipMemcpyDeviceToHost, st), "download");
+    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    out.release();
+    for (size_t i = 0; i < n; i++) {
+        acc[0][i] = h[i].x;
+        acc[1][i] = h[i].y;
+        acc[2][i] = h[i].z;
     }
     return NB_OK;
 }
@@ -977,28 +1010,11 @@ nb_status nb_learned_set_near_radius(nb_ctx* c, double rc) {
 }
 
 nb_status nb_near_accel(nb_ctx* c, int form, double rc, double eps, double* const acc[3]) {
-    if (!c || !acc || !acc[0] || !acc[1] || !acc[2]) return nbh::set_error(NB_ERR_ARG, "nb_near_accel: null argument");
-    const int kf = nbh::kernel_form(form);
-    if (kf < 0) return nbh::set_error(NB_ERR_ARG, "nb_near_accel: form must be FP64 or a mixed form");
-    if (!(rc > 0.0) || !std::isfinite(rc) || !(eps >= 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_near_accel: need rc > 0, eps >= 0");
-    const size_t n = nbh::ctx_n(c);
-    if (n == 0) return NB_OK;
-    Learned* L = nbl::get(c);
-    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_near_accel: out of host memory");
-    hipStream_t st = nbh::ctx_stream(c);
-    nbh::DevBuf out;
-    LH(out.ensure(n * sizeof(P4)), "allocate output");
-    LH(L->near.near_accel(nbh::ctx_pm(c), (int)n, rc, eps, (P4*)out.p, st, kf != nbk::FORM_FP64), "near-field kernels");
-    std::vector<P4> h(n);
-    LH(hipMemcpyAsync(h.data(), out.p, n * sizeof(P4), hipMemcpyDeviceToHost, st), "download");
-    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
-    out.release();
-    for (size_t i = 0; i < n; i++) {
-        acc[0][i] = h[i].x;
-        acc[1][i] = h[i].y;
-        acc[2][i] = h[i].z;
-    }
-    return NB_OK;
+    return nbl::near_accel_host(c, form, rc, eps, acc, true, "nb_near_accel");
+}
+
+nb_status nb_near_accel_kept(nb_ctx* c, int form, double rc, double eps, double* const acc[3]) {
+    return nbl::near_accel_host(c, form, rc, eps, acc, false, "nb_near_accel_kept");
 }
 
 nb_status nb_learned_set_mode(nb_ctx* c, int mode) {

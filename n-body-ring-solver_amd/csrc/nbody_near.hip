@@ -27,16 +27,26 @@
 // (same x, y / kGroup equal), merged greedily up to 64, so sparse regions do
 // not cost one launch slot per column.  The blocks the device holds at once
 // take chunks from a work counter (no chunk count travels to the host); a
-// block of five waves
-// takes a chunk spanning columns (x0, ys..ye) and z slabs zmin..zmax: wave w
-// sums over the columns x0 - 2 + w, ys - 2..ye + 2, each restricted by binary
-// search to the z slabs zmin - 8..zmax + 8, streaming those runs through LDS;
-// the five partial sums are added in wave order (bitwise reproducible).  Of
+// block of five waves takes a chunk and the cells within rc of its bounding
+// box (after a build: five columns across x, the chunk's y-range +- 2
+// columns, its slabs +- 8): the columns are dealt to the waves in turn, each
+// restricted by binary search to the slab range, streaming those runs
+// through LDS; the five partial sums are added in wave order (bitwise
+// reproducible).  Of
 // each 64-particle batch of a run only the particles within rc of the chunk's
 // bounding box are kept (ballot + prefix count, order preserved), so far
 // corners of the cell box cost one distance test instead of a pair each.
 // Pairs beyond rc get S = 0, so the pair code is branch-free; the FP32
 // variant stages the masses already rounded to FP32.
+//
+// Reuse: between rebuilds (the learned form rebuilds on its exact substeps)
+// the grid is kept; each evaluation gathers the current positions in the
+// grid's order and measures the largest displacement d since the build, and
+// k_near searches the build's cells within rc + d of each chunk's current
+// bounding box: a partner within rc now was within rc + d of it at the
+// build, so the result stays exact however far the particles moved (a state
+// that moved too far for the cells to help is scanned whole).  The chunks'
+// bounding boxes and every pair use the current positions.
 //
 // FP64 throughout, or (fp32) FP64 differences with FP32 pair arithmetic and
 // FP32 sums over each 64-particle batch, flushed to FP64 -- the precision of
@@ -61,14 +71,12 @@ namespace nbn {
 namespace {
 
 constexpr int BLK = 256;
-constexpr int kWaves = 5;                 // waves per chunk: one per x-slice of neighbour columns
+constexpr int kWaves = 5;                 // waves per chunk
 constexpr int kColSide = 1 << 11;         // columns per side of the (x, y) table
 constexpr int kColBias = kColSide / 2;
 constexpr int kZBias = 1 << 20;
 constexpr int kZMax = (1 << 21) - 1;
-constexpr int kReach = 2;                 // neighbour columns each way (column edge rc/2)
 constexpr int kZFine = 4;                 // z slabs per rc/2: slab height rc/8
-constexpr int kZReach = kReach * kZFine;  // neighbour slabs each way
 constexpr int kDense = 32;                // particles from which a column gets chunks of its own
 constexpr int kGroup = 16;                // sparser columns are merged within groups of 16 along y
 
@@ -87,7 +95,6 @@ __device__ __forceinline__ unsigned long long pack(int ix, int iy, int iz) {
 }
 
 __device__ __forceinline__ int col_of(unsigned long long key) { return (int)(key >> 21); }
-__device__ __forceinline__ int z_of(unsigned long long key) { return (int)(key & kZMax); }
 
 __global__ __launch_bounds__(BLK) void k_keys(const nbk::P4* __restrict__ pm, int n, double inv_h,
                                               unsigned long long* __restrict__ keys, int* __restrict__ idx) {
@@ -145,28 +152,53 @@ __global__ __launch_bounds__(BLK) void k_chunk_flags(const int2* __restrict__ co
 }
 
 // The chunk list, its length and the work counter k_near takes chunks from
-// (reset here, so no fill is needed).
+// (reset here, so no fill is needed; the displacement since the build is 0).
 __global__ __launch_bounds__(BLK) void k_chunk_list(const int* __restrict__ flag, const int* __restrict__ pos, int n,
                                                     int* __restrict__ chunks, int* __restrict__ nchunks,
-                                                    int* __restrict__ next) {
+                                                    int* __restrict__ next, unsigned int* __restrict__ dmax) {
     const int p = blockIdx.x * BLK + threadIdx.x;
     if (p >= n) return;
     if (flag[p]) chunks[pos[p]] = p;
     if (p == n - 1) {
         *nchunks = pos[p] + flag[p];
         *next = 0;
+        *dmax = 0u;
         chunks[pos[p] + flag[p]] = n;   // sentinel: the end of the last chunk
     }
 }
 
-__device__ __forceinline__ int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
+// Before an evaluation on a kept grid: the work counter and the displacement
+// maximum start at zero.
+__global__ void k_reuse_reset(int* __restrict__ next, unsigned int* __restrict__ dmax) {
+    *next = 0;
+    *dmax = 0u;
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+// The current positions in the grid's order and the largest displacement
+// since the build (an FP32 upper bound, as bits: positive floats order like
+// their bit patterns).
+__global__ __launch_bounds__(BLK) void k_restage(const nbk::P4* __restrict__ pm, const int* __restrict__ order,
+                                                 const nbk::P4* __restrict__ built, int n, nbk::P4* __restrict__ cur,
+                                                 unsigned int* __restrict__ dmax) {
+    __shared__ float red[BLK / 64];
+    const int p = blockIdx.x * BLK + threadIdx.x;
+    float d = 0.0f;
+    if (p < n) {
+        const nbk::P4 q = pm[order[p]];
+        cur[p] = q;
+        const nbk::P4 b = built[p];
+        const double dx = q.x - b.x, dy = q.y - b.y, dz = q.z - b.z;
+        d = __double2float_ru(sqrt(dx * dx + dy * dy + dz * dz));
+        if (!(d <= 3.0e38f)) d = 3.0e38f;   // a non-finite position: the widest reach
+    }
+    for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+        for (int w = 1; w < BLK / 64; w++) m = fmaxf(m, red[w]);
+        atomicMax(dmax, __float_as_uint(m));
+    }
 }
 
 __device__ __forceinline__ double wave_min(double v) {
@@ -275,19 +307,32 @@ __device__ __forceinline__ void near_pair32(const nbk::P4& me, const nbk::P4& o,
 // counter `next` until the count `*nchunks` is reached (every block leaves
 // then), so the launch needs no count from the host; a chunk's sums do not
 // depend on which block takes it.
+//
+// src: positions in the grid's order (the build's, or current ones gathered
+// through it); keys: the build's cells.  A pair within rc now is within
+// rc + d of the chunk's current bounding box at the build (d: the largest
+// displacement since, *dmax), so the cells from cell(lo - reach) to
+// cell(hi + reach), reach = rc (1 + 2^-20) + d, hold every partner (the cell
+// maps are monotone; the margin covers the rounding of lo - reach).  The
+// columns of that (x, y) range are dealt to the waves in turn (column c to
+// wave c mod kWaves); a range wider than `brute` columns (a far-flung or
+// non-finite state) is scanned as the whole sorted array instead, split
+// evenly over the waves, so the work stays bounded.
 template <bool F32, bool EPS0>
-__global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict__ sorted,
+__global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict__ src,
                                                       const unsigned long long* __restrict__ keys,
                                                       const int* __restrict__ order, const int2* __restrict__ cols,
                                                       const int* __restrict__ chunks,
                                                       const int* __restrict__ nchunks, int* __restrict__ next,
-                                                      int n, double rc2, Taper tp, double eps2,
+                                                      const unsigned int* __restrict__ dmax, int n, double inv_h,
+                                                      double rc, double rc2, Taper tp, double eps2, int brute,
                                                       nbk::P4* __restrict__ out) {
     __shared__ nbk::P4 stage[kWaves][64];
     __shared__ nbk::P4 part[kWaves][64];
     __shared__ int taken;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nch = *nchunks;
+    const double reach = rc * (1.0 + 0x1p-20) + (double)__uint_as_float(*dmax);
     for (;;) {
         if (threadIdx.x == 0) taken = atomicAdd(next, 1);
         __syncthreads();
@@ -295,31 +340,37 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
         if (ch >= nch) break;
         const int begin = chunks[ch];
         const int cnt = chunks[ch + 1] - begin;   // 1..64
-        const int i = begin + (lane < cnt ? lane : 0);
-        const nbk::P4 me = sorted[i];
-        const unsigned long long ki = keys[i];
-        const int x0 = (int)(ki >> 32);   // one x for the whole chunk
-        const int ys = wave_min((int)((ki >> 21) & (kColSide - 1)));
-        const int ye = wave_max((int)((ki >> 21) & (kColSide - 1)));
-        const int zlo = max(wave_min(z_of(ki)) - kZReach, 0), zhi = min(wave_max(z_of(ki)) + kZReach, kZMax);
-        const int X = x0 - kReach + w;
-        const int ny = ye - ys + 1 + 2 * kReach;   // <= kGroup + 2 kReach <= 64
-        // lanes 0..ny-1 locate the z-runs of columns (X, ys - 2 .. ye + 2)
-        int a = 0, b = 0;
-        if (lane < ny && X >= 0 && X < kColSide) {
-            const int Y = ys - kReach + lane;
-            if (Y >= 0 && Y < kColSide) {
-                const int2 run = cols[X * kColSide + Y];
-                a = lower_bound(keys, run.x, run.y, pack(X, Y, zlo));
-                b = lower_bound(keys, a, run.y, pack(X, Y, zhi) + 1);
-            }
-        }
+        const nbk::P4 me = src[begin + (lane < cnt ? lane : 0)];
         double ax = 0.0, ay = 0.0, az = 0.0;
         // bounding box of the chunk's particles (lanes past the chunk repeat its first)
         const double3 lo = make_double3(wave_min(me.x), wave_min(me.y), wave_min(me.z));
         const double3 hi = make_double3(wave_max(me.x), wave_max(me.y), wave_max(me.z));
         nbk::P4* L = stage[w];
-        for (int k = 0; k < ny; k++) {
+        const int xa = cell_xy(lo.x - reach, inv_h), xb = cell_xy(hi.x + reach, inv_h);
+        const int ya = cell_xy(lo.y - reach, inv_h), yb = cell_xy(hi.y + reach, inv_h);
+        const int za = cell_z(lo.z - reach, inv_h), zb = cell_z(hi.z + reach, inv_h);
+        const int ny = yb - ya + 1, ncol = (xb - xa + 1) * ny;
+        const bool all = ncol > brute;   // wave-uniform
+        // column rounds: 64 columns of this wave's share per round; the whole
+        // array: one round whose single "column" is this wave's fifth of it
+        const int rounds = all ? 1 : (ncol - w + 64 * kWaves - 1) / (64 * kWaves);
+        for (int r = 0; r < rounds; r++) {
+            // lanes locate the z-runs of their columns c = w + kWaves (64 r + lane)
+            int a = 0, b = 0;
+            if (all) {
+                a = (int)((long long)n * w / kWaves);
+                b = (int)((long long)n * (w + 1) / kWaves);
+            } else {
+                const int c = w + kWaves * (64 * r + lane);
+                if (c < ncol) {
+                    const int X = xa + c / ny, Y = ya + c % ny;
+                    const int2 run = cols[X * kColSide + Y];
+                    a = lower_bound(keys, run.x, run.y, pack(X, Y, za));
+                    b = lower_bound(keys, a, run.y, pack(X, Y, zb) + 1);
+                }
+            }
+            const int kmax = all ? 1 : min(64, (ncol - w + kWaves - 1) / kWaves - 64 * r);
+            for (int k = 0; k < kmax; k++) {
             const int ak = __shfl(a, k, 64), bk = __shfl(b, k, 64);
             for (int j0 = ak; j0 < bk; j0 += 64) {
                 // keep only the particles within rc of the chunk's bounding box,
@@ -328,7 +379,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
                 nbk::P4 J = nbk::P4{0.0, 0.0, 0.0, 0.0};
                 bool use = false;
                 if (jj < bk) {
-                    J = sorted[jj];
+                    J = src[jj];
                     const double ex = fmax(fmax(lo.x - J.x, J.x - hi.x), 0.0);
                     const double ey = fmax(fmax(lo.y - J.y, J.y - hi.y), 0.0);
                     const double ez = fmax(fmax(lo.z - J.z, J.z - hi.z), 0.0);
@@ -367,6 +418,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
                 }
                 wave_sync();
             }
+            }
         }
         part[w][lane] = nbk::P4{ax, ay, az, 0.0};
         __syncthreads();
@@ -387,11 +439,58 @@ __global__ __launch_bounds__(64 * kWaves) void k_near(const nbk::P4* __restrict_
 }  // namespace
 
 hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk::P4* out, hipStream_t st,
-                            bool fp32) {
+                            bool fp32, bool rebuild) {
     if (n <= 0) return hipSuccess;
     const double inv_h = 2.0 / rc;
     const int ncols = kColSide * kColSide;
+    const dim3 grid((n + BLK - 1) / BLK);
     hipError_t e;
+    const bool keep = !rebuild && built_n == n && built_rc == rc;
+    if (!keep) built_n = 0;   // invalid until this build is queued
+    // chunk buffer: n flags, then n + 1 scan results, the list (n + 1), the
+    // count, the work counter, the displacement maximum
+    auto slots = [&](int*& flag, int*& pos, int*& list, int*& nchunks, int*& next, unsigned int*& dmax) {
+        flag = (int*)chunks.p;
+        pos = flag + n;
+        list = pos + n + 1;
+        nchunks = list + n + 1;
+        next = nchunks + 1;
+        dmax = (unsigned int*)(next + 1);
+    };
+    auto launch_near = [&](const nbk::P4* src) -> hipError_t {
+        int *flag, *pos, *list, *nchunks, *next;
+        unsigned int* dmax;
+        slots(flag, pos, list, nchunks, next, dmax);
+        const int variant = (fp32 ? 2 : 0) + (eps > 0.0 ? 0 : 1);
+        auto kernel = fp32 ? (eps > 0.0 ? k_near<true, false> : k_near<true, true>)
+                           : (eps > 0.0 ? k_near<false, false> : k_near<false, true>);
+        if (resident[variant] <= 0) {   // blocks the device holds at once: the persistent grid
+            int dev = 0, ncu = 0, per_cu = 0;
+            hipError_t q;
+            if ((q = hipGetDevice(&dev)) != hipSuccess) return q;
+            if ((q = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return q;
+            if ((q = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kWaves, 0)) != hipSuccess)
+                return q;
+            resident[variant] = std::max(1, ncu * std::max(per_cu, 1));
+        }
+        const int blocks = std::min(resident[variant], n);   // never more chunks than particles
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kWaves), 0, st, src, (const unsigned long long*)keys_out.p,
+                           (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, (const int*)nchunks, next,
+                           (const unsigned int*)dmax, n, inv_h, rc, rc * rc, make_taper(rc, eps), eps * eps,
+                           std::max(n / 32, 256), out);
+        return hipGetLastError();
+    };
+    if (keep) {
+        if ((e = cur.ensure((size_t)n * sizeof(nbk::P4))) != hipSuccess) return e;
+        int *flag, *pos, *list, *nchunks, *next;
+        unsigned int* dmax;
+        slots(flag, pos, list, nchunks, next, dmax);
+        hipLaunchKernelGGL(k_reuse_reset, dim3(1), dim3(1), 0, st, next, dmax);
+        hipLaunchKernelGGL(k_restage, grid, dim3(BLK), 0, st, pm, (const int*)idx_out.p, (const nbk::P4*)sorted.p, n,
+                           (nbk::P4*)cur.p, dmax);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        return launch_near((const nbk::P4*)cur.p);
+    }
     if ((e = keys_in.ensure((size_t)n * 8)) != hipSuccess) return e;
     if ((e = keys_out.ensure((size_t)n * 8)) != hipSuccess) return e;
     if ((e = idx_in.ensure((size_t)n * 4)) != hipSuccess) return e;
@@ -399,7 +498,6 @@ hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk
     if ((e = sorted.ensure((size_t)n * sizeof(nbk::P4))) != hipSuccess) return e;
     if ((e = cols.ensure((size_t)ncols * sizeof(int2))) != hipSuccess) return e;
     if ((e = chunks.ensure((size_t)(4 * n + 3) * 4 + 64)) != hipSuccess) return e;
-    const dim3 grid((n + BLK - 1) / BLK);
     hipLaunchKernelGGL(k_keys, grid, dim3(BLK), 0, st, pm, n, inv_h, (unsigned long long*)keys_in.p, (int*)idx_in.p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     size_t bytes = 0;
@@ -411,11 +509,9 @@ hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk
                                   (const int*)idx_in.p, (int*)idx_out.p, (unsigned int)n, 0, 43, st);
     if (e != hipSuccess) return e;
     const unsigned long long* keys = (const unsigned long long*)keys_out.p;
-    int* flag = (int*)chunks.p;   // n flags, then n + 1 scan results, the list (n + 1), the count, the counter
-    int* pos = flag + n;
-    int* list = pos + n + 1;
-    int* nchunks = list + n + 1;
-    int* next = nchunks + 1;
+    int *flag, *pos, *list, *nchunks, *next;
+    unsigned int* dmax;
+    slots(flag, pos, list, nchunks, next, dmax);
     hipLaunchKernelGGL(k_gather_sorted, grid, dim3(BLK), 0, st, pm, (const int*)idx_out.p, n, (nbk::P4*)sorted.p);
     hipLaunchKernelGGL(k_clear_cols, dim3((ncols + BLK - 1) / BLK), dim3(BLK), 0, st, (int2*)cols.p, ncols);
     hipLaunchKernelGGL(k_col_runs, grid, dim3(BLK), 0, st, keys, n, (int2*)cols.p);
@@ -431,29 +527,17 @@ hipError_t Near::near_accel(const nbk::P4* pm, int n, double rc, double eps, nbk
     e = rocprim::exclusive_scan(scan_temp.p, sbytes, (const int*)flag, pos, 0, (size_t)n, rocprim::plus<int>(), st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_chunk_list, grid, dim3(BLK), 0, st, (const int*)flag, (const int*)pos, n, list, nchunks,
-                       next);
+                       next, dmax);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int variant = (fp32 ? 2 : 0) + (eps > 0.0 ? 0 : 1);
-    auto kernel = fp32 ? (eps > 0.0 ? k_near<true, false> : k_near<true, true>)
-                       : (eps > 0.0 ? k_near<false, false> : k_near<false, true>);
-    if (resident[variant] <= 0) {   // blocks the device holds at once: the persistent grid
-        int dev = 0, ncu = 0, per_cu = 0;
-        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-        if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kWaves, 0)) != hipSuccess)
-            return e;
-        resident[variant] = std::max(1, ncu * std::max(per_cu, 1));
-    }
-    const int blocks = std::min(resident[variant], n);   // never more chunks than particles
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * kWaves), 0, st, (const nbk::P4*)sorted.p, keys,
-                       (const int*)idx_out.p, (const int2*)cols.p, (const int*)list, (const int*)nchunks, next, n,
-                       rc * rc, make_taper(rc, eps), eps * eps, out);
-    return hipGetLastError();
+    built_n = n;
+    built_rc = rc;
+    return launch_near((const nbk::P4*)sorted.p);
 }
 
 void Near::release() {
-    for (nbh::DevBuf* b : {&keys_in, &keys_out, &idx_in, &idx_out, &sorted, &temp, &scan_temp, &cols, &chunks})
+    for (nbh::DevBuf* b : {&keys_in, &keys_out, &idx_in, &idx_out, &sorted, &temp, &scan_temp, &cols, &chunks, &cur})
         b->release();
+    built_n = 0;
 }
 
 }  // namespace nbn

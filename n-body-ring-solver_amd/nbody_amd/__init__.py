@@ -39,7 +39,7 @@ EXPORTS = (
     "nb_learned_set_mode", "nb_learned_collect", "nb_learned_train", "nb_learned_stats_read",
     "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
     "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
-    "nb_near_accel", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
+    "nb_near_accel", "nb_near_accel_kept", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
     "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
     "nb_ring_predict", "nb_kernel_info_read", "nb_mlp_adam_step", "nb_forces_ring_round",
     "nb_forces_ring_round_device", "nb_accel_block_device_noalloc",
@@ -110,6 +110,7 @@ def load_library(path: str = LIB_PATH):
     L.nb_learned_set_flag_ratio.argtypes = [_VP, C.c_double]
     L.nb_learned_set_near_radius.argtypes = [_VP, C.c_double]
     L.nb_near_accel.argtypes = [_VP, C.c_int, C.c_double, C.c_double, _P3]
+    L.nb_near_accel_kept.argtypes = [_VP, C.c_int, C.c_double, C.c_double, _P3]
     L.nb_learned_set_hold_ratio.argtypes = [_VP, C.c_double]
     L.nb_snapshot_write.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_size_t, _D, _P3, _P3, _P3]
     L.nb_snapshot_frames.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -433,6 +434,13 @@ class Solver:
     def near_accel(self, rc: float, eps: float = 0.0, form="fp64") -> np.ndarray:
         a = np.empty((3, self.n))
         _check(self.L.nb_near_accel(self.h, _form(form), rc, eps, _planes(a)), "nb_near_accel")
+        return a
+
+    def near_accel_kept(self, rc: float, eps: float = 0.0, form="fp64") -> np.ndarray:
+        """The near field on the grid of the last build (same n and rc), searched
+        wider by the displacement since."""
+        a = np.empty((3, self.n))
+        _check(self.L.nb_near_accel_kept(self.h, _form(form), rc, eps, _planes(a)), "nb_near_accel_kept")
         return a
 
     def learned_set_mode(self, mode: int):

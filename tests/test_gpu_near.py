@@ -77,6 +77,57 @@ def test_near_accel_far_flung(solver, orc):
     assert np.linalg.norm(ao[:, 400]) > 0 and np.all(a[:, 402:] == 0.0)
 
 
+@pytest.mark.parametrize("shift", [0.0, 0.01, 0.5, 6.0])
+def test_near_accel_kept_grid_after_motion(solver, orc, shift):
+    """The grid of one build searched after every particle moved by up to
+    shift * rc (0: no motion; 0.5: a wider column and slab reach; 6: wider
+    than the scan bound at this N, so the whole array is scanned): the same
+    pairs as a new grid, to FP64 rounding."""
+    n, rc, eps = 3000, 0.1, 0.01
+    m, pos, vel, _ = nbody_amd.init_system("plummer", n, 4)
+    solver.set_state(m, pos, vel)
+    solver.near_accel(rc, eps)   # builds the grid at pos
+    rng = np.random.default_rng(17)
+    u = rng.normal(size=(3, n))
+    u *= rng.uniform(0.0, 1.0, n) ** (1.0 / 3.0) / np.linalg.norm(u, axis=0)
+    moved = np.ascontiguousarray(pos + shift * rc * u)
+    solver.set_state(m, moved, vel)
+    a = solver.near_accel_kept(rc, eps)
+    ao = orc.near_accel(m, moved, rc, eps)
+    scale = np.max(np.linalg.norm(ao, axis=0))
+    assert np.max(np.linalg.norm(a - ao, axis=0)) < 1e-12 * scale
+    fresh = solver.near_accel(rc, eps)
+    assert np.max(np.linalg.norm(a - fresh, axis=0)) < 1e-13 * scale
+    if shift == 0.0:   # nothing moved: the kept grid's sums are the new grid's
+        assert np.array_equal(a, fresh)
+
+
+def test_near_accel_kept_mixed_after_motion(solver, orc):
+    n, rc, eps = 4000, 0.1, 0.005
+    m, pos, vel, _ = nbody_amd.init_system("plummer", n, 8)
+    solver.set_state(m, pos, vel)
+    solver.near_accel(rc, eps, form="mixed")
+    moved = np.ascontiguousarray(pos + 0.05 * vel)   # |v| < 1.5: up to 0.75 rc
+    solver.set_state(m, moved, vel)
+    a = solver.near_accel_kept(rc, eps, form="mixed")
+    ao = orc.near_accel(m, moved, rc, eps)
+    scale = np.max(np.linalg.norm(ao, axis=0))
+    assert np.max(np.linalg.norm(a - ao, axis=0)) < 1e-5 * scale
+
+
+def test_near_accel_kept_rebuilds_for_another_radius_or_count(solver):
+    """No grid of the same N and rc: a new one, as nb_near_accel builds."""
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 2000, 6)
+    solver.set_state(m, pos, vel)
+    solver.near_accel(0.2, 0.01)
+    a = solver.near_accel_kept(0.1, 0.01)
+    assert np.array_equal(a, solver.near_accel(0.1, 0.01))
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 2100, 6)
+    solver.set_state(m, pos, vel)
+    a = solver.near_accel_kept(0.1, 0.01)
+    assert np.array_equal(a, solver.near_accel(0.1, 0.01))
+
+
 def test_near_accel_rejects_bad_radius(solver):
     m, pos, vel, _ = nbody_amd.init_system("plummer", 64, 1)
     solver.set_state(m, pos, vel)
