@@ -278,14 +278,64 @@ __global__ __launch_bounds__(BLK) void k_check(const P4* __restrict__ pred, cons
     }
 }
 
-// After a check: held[i] = whether particle i's extrapolation missed the
-// exact remainder by more than sqrt(thr2) (NaN counts as a miss).
-__global__ __launch_bounds__(BLK) void k_hold(const P4* __restrict__ pred, const P4* __restrict__ acc,
-                                              const P4* __restrict__ near, long n, double thr2,
-                                              unsigned char* __restrict__ held) {
+// The outcome of a check, decided on the device from k_check's block sums:
+// the hold kernel reads it there, the host from a pinned copy when it next
+// needs it (after queueing the next substep's near field).
+struct Decision {
+    double e2[2], a2[2], mx[2];   // squared error, squared exact value, largest relative error: cubic, network
+    double thr2;                  // squared miss above which a particle is held
+    int net_ok;                   // the network's extrapolation passed (it is the one judged and held against)
+    int ok;                       // the judged extrapolation passed: extrapolation may go on
+};
+
+// One wave: per set, the block sums of k_check added in a fixed order (lane
+// l takes blocks l, l + 64, ..., then a fixed tree), then the verdict:
+// the network passes when it applied and its error is within the threshold
+// (NaN fails), the check passes when the judged extrapolation's does.
+__global__ __launch_bounds__(64) void k_decide(const double* __restrict__ chk, long nb, int sets, double threshold,
+                                               double kappa, long n, Decision* __restrict__ d) {
+    const int lane = threadIdx.x;
+    Decision r{};
+    for (int k = 0; k < 2; k++) {
+        double e2 = 0.0, a2 = 0.0, mx = 0.0;
+        if (k < sets) {
+            const double* h = chk + 3 * nb * k;
+            for (long b = lane; b < nb; b += 64) {
+                e2 += h[3 * b];
+                a2 += h[3 * b + 1];
+                mx = fmax(mx, h[3 * b + 2]);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                e2 += __shfl_xor(e2, o, 64);
+                a2 += __shfl_xor(a2, o, 64);
+                mx = fmax(mx, __shfl_xor(mx, o, 64));
+            }
+        }
+        r.e2[k] = e2;
+        r.a2[k] = a2;
+        r.mx[k] = mx;
+    }
+    if (lane != 0) return;
+    auto rel = [](double e2, double a2) { return a2 > 0 ? sqrt(e2 / a2) : INFINITY; };
+    const double err_c = rel(r.e2[0], r.a2[0]);
+    const double err_n = sets > 1 ? rel(r.e2[1], r.a2[1]) : INFINITY;
+    r.net_ok = (sets > 1 && err_n <= threshold) ? 1 : 0;
+    const double err = r.net_ok ? err_n : err_c;
+    r.ok = (err <= threshold) ? 1 : 0;
+    r.thr2 = kappa * kappa * r.a2[0] / (double)(n > 1 ? n : 1);
+    *d = r;
+}
+
+// After a check: held[i] = whether particle i's judged extrapolation (the
+// network's when it passed, else the cubic) missed the exact remainder by
+// more than sqrt(d->thr2) (NaN counts as a miss).
+__global__ __launch_bounds__(BLK) void k_hold(const P4* __restrict__ cub, const P4* __restrict__ net,
+                                              const P4* __restrict__ acc, const P4* __restrict__ near, long n,
+                                              const Decision* __restrict__ d, unsigned char* __restrict__ held) {
     const long i = (long)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
-    const P4 p = pred[i], a = acc[i];
+    const double thr2 = d->thr2;
+    const P4 p = d->net_ok ? net[i] : cub[i], a = acc[i];
     const P4 q = near ? near[i] : P4{0.0, 0.0, 0.0, 0.0};
     const double dx = a.x - q.x - p.x, dy = a.y - q.y - p.y, dz = a.z - q.z - p.z;
     held[i] = (dx * dx + dy * dy + dz * dz <= thr2) ? 0 : 1;
@@ -367,6 +417,10 @@ __global__ __launch_bounds__(BLK) void k_sumsq(const float* __restrict__ dsT, lo
 
 inline dim3 blocks(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 
+// An extrapolated substep whose exact rows exceed rough_share N runs the
+// direct sum instead, and so do the substeps of the next kRoughHold periods.
+constexpr int kRoughHold = 8;
+
 }  // namespace
 
 struct Learned {
@@ -376,6 +430,7 @@ struct Learned {
     double rho = 1e-2;   // roughness ratio |D3| / |a_n| above which a particle is summed exactly
     double kappa = 1e-3; // miss at a check, in units of the RMS acceleration, above which a particle is held
     double rc = 0.0;     // near-field radius: the history holds a - a_near (0: no split)
+    double rough_share = 0.25;   // exact rows, as a share of N, above which the direct sum runs instead
     nbn::Near near;
     int exact_kform = nbk::FORM_MIXED_NR;
     int mode = 0;
@@ -410,7 +465,15 @@ struct Learned {
     bool nflag_wait = false;         // a copy is in flight: take_flag_count() waits for it
     hipStream_t side = nullptr;      // the near field runs here, beside the main stream's work
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    std::vector<double> chk_h;
+    int grid_age = 0;                // substeps since the near-field grid was last rebuilt
+    // the last check's decision: on the device, its pinned copy, the event
+    // after the copy; pending until the host has taken it (check_finish)
+    DevBuf dec;
+    Decision* dec_h = nullptr;
+    hipEvent_t dec_ev = nullptr;
+    bool dec_pending = false;
+    bool dec_net = false;            // the pending check evaluated the network
+    long rough_hold = 0;             // substeps still exact after the exact rows outgrew a quarter of N
     nb_learned_stats st{};
 };
 
@@ -512,6 +575,9 @@ nb_status ensure_particles(Learned& L, size_t n) {
     if (!L.side) LH(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking), "create stream");
     if (!L.ev_fork) LH(hipEventCreateWithFlags(&L.ev_fork, hipEventDisableTiming), "create event");
     if (!L.ev_join) LH(hipEventCreateWithFlags(&L.ev_join, hipEventDisableTiming), "create event");
+    LH(L.dec.ensure(sizeof(Decision)), "allocate check decision");
+    if (!L.dec_h) LH(hipHostMalloc((void**)&L.dec_h, sizeof(Decision), hipHostMallocDefault), "allocate pinned decision");
+    if (!L.dec_ev) LH(hipEventCreateWithFlags(&L.dec_ev, hipEventDisableTiming), "create event");
     L.n = n;
     L.count = 0;
     L.head = 3;
@@ -648,81 +714,61 @@ nb_status exact_subset_scatter(nb_ctx* c, Learned& L) {
     return NB_OK;
 }
 
-// Compare the extrapolations pred[0..sets) with the exact acc over the
-// smooth particles: per set, the squared error, the squared exact value and
-// the largest relative error, summed on the host from per-block sums.  One
-// synchronisation of the stream for all sets.
-struct CheckSums {
-    double e2 = 0.0, a2 = 0.0, mx = 0.0;
-};
-
-nb_status check_sums(nb_ctx* c, Learned& L, const P4* const* pred, int sets, CheckSums* out) {
+// Compare the extrapolations made for this substep (the cubic alone in
+// L.cub, and with the network in L.pred when it was applied) with the exact
+// acc over the particles not marked, decide on the device (k_decide) and hold
+// the particles that missed (k_hold reads the decision there); the decision
+// travels to pinned host memory behind an event.  Nothing waits here:
+// check_finish() takes the decision when the schedule needs it.
+nb_status check_enqueue(nb_ctx* c, Learned& L) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
     const long nb = (n + BLK - 1) / BLK;
+    const int sets = L.pred_net ? 2 : 1;
+    const P4* pset[2] = {(const P4*)L.cub.p, (const P4*)L.pred.p};
+    const P4* near = L.rc > 0 ? (const P4*)L.nearacc.p : nullptr;
     for (int k = 0; k < sets; k++) {
-        hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, pred[k], (const P4*)nbh::ctx_acc(c),
-                           L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, (const unsigned char*)L.mask.p, n,
-                           (double*)L.chk.p + 3 * nb * k);
+        hipLaunchKernelGGL(k_check, dim3((unsigned)nb), dim3(BLK), 0, st, pset[k], (const P4*)nbh::ctx_acc(c), near,
+                           (const unsigned char*)L.mask.p, n, (double*)L.chk.p + 3 * nb * k);
         LH(hipGetLastError(), "check kernel");
     }
-    L.chk_h.resize(3 * nb * sets);
-    LH(hipMemcpyAsync(L.chk_h.data(), L.chk.p, 3 * nb * sets * sizeof(double), hipMemcpyDeviceToHost, st),
-       "download check");
-    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
-    for (int k = 0; k < sets; k++) {
-        const double* h = L.chk_h.data() + 3 * nb * k;
-        CheckSums r;
-        for (long b = 0; b < nb; b++) {
-            r.e2 += h[3 * b];
-            r.a2 += h[3 * b + 1];
-            r.mx = std::max(r.mx, h[3 * b + 2]);
-        }
-        out[k] = r;
-    }
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, st, (const double*)L.chk.p, nb, sets, L.threshold, L.kappa, n,
+                       (Decision*)L.dec.p);
+    LH(hipGetLastError(), "decision kernel");
+    hipLaunchKernelGGL(k_hold, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const P4*)L.pred.p,
+                       (const P4*)nbh::ctx_acc(c), near, n, (const Decision*)L.dec.p, (unsigned char*)L.held.p);
+    LH(hipGetLastError(), "hold kernel");
+    LH(hipMemcpyAsync(L.dec_h, L.dec.p, sizeof(Decision), hipMemcpyDeviceToHost, st), "download decision");
+    LH(hipEventRecord(L.dec_ev, st), "record event");
+    L.dec_pending = true;
+    L.dec_net = L.pred_net;
     return NB_OK;
 }
 
-// Compare the extrapolations made for this substep (the cubic alone in
-// L.cub, and with the network in L.pred when it was applied) with the exact
-// acc over the particles not marked; update stats, the fallback state and
-// whether the network may be used in the next cycle (only after it passed a
-// check itself), and hold the particles that missed.  Synchronises the stream.
-nb_status check(nb_ctx* c, Learned& L) {
-    const long n = (long)L.n;
-    // nothing checked (every particle rough or held, or all at rest) validates nothing
+// Take the pending check's decision (waiting for its copy): the fallback
+// state, whether the network may be used until the next check, the stats.
+nb_status check_finish(Learned& L) {
+    if (!L.dec_pending) return NB_OK;
+    LH(hipEventSynchronize(L.dec_ev), "hipEventSynchronize");
+    L.dec_pending = false;
+    const Decision& r = *L.dec_h;
     auto rel = [](double e2, double a2) { return a2 > 0 ? std::sqrt(e2 / a2) : INFINITY; };
-    const P4* sets[2] = {(const P4*)L.cub.p, (const P4*)L.pred.p};
-    CheckSums r[2];
-    if (nb_status e = check_sums(c, L, sets, L.pred_net ? 2 : 1, r)) return e;
-    const double a2 = r[0].a2, mxc = r[0].mx;
-    const double err_c = rel(r[0].e2, a2);
-    double err_n = INFINITY, mxn = INFINITY;
-    if (L.pred_net) {
-        err_n = rel(r[1].e2, r[1].a2);
-        mxn = r[1].mx;
-    }
-    L.net_ok = L.pred_net && (err_n <= L.threshold);   // NaN fails
+    const double err_c = rel(r.e2[0], r.a2[0]);
+    const double err_n = L.dec_net ? rel(r.e2[1], r.a2[1]) : INFINITY;
+    L.net_ok = r.net_ok != 0;
     L.st.checks++;
-    if (L.pred_net) {
+    if (L.dec_net) {
         L.st.net_checks++;
         L.st.net_cubic_err_sum += err_c;
         L.st.net_err_sum += err_n;
     }
     const double err = L.net_ok ? err_n : err_c;
     L.st.last_check_err = err;
-    L.st.last_check_max = L.net_ok ? mxn : mxc;
+    L.st.last_check_max = L.net_ok ? r.mx[1] : r.mx[0];
     L.st.last_cubic_err = err_c;
     L.st.max_check_err = std::max(L.st.max_check_err, std::isnan(err) ? INFINITY : err);
-    const bool ok = (err <= L.threshold);
-    if (!ok) L.st.fallbacks++;
-    L.fallback = !ok;
-    // hold the particles that missed by more than kappa a_rms
-    const double thr2 = L.kappa * L.kappa * a2 / (double)std::max<long>(n, 1);
-    hipLaunchKernelGGL(k_hold, blocks(n), dim3(BLK), 0, nbh::ctx_stream(c),
-                       L.net_ok ? (const P4*)L.pred.p : (const P4*)L.cub.p, (const P4*)nbh::ctx_acc(c),
-                       L.rc > 0 ? (const P4*)L.nearacc.p : nullptr, n, thr2, (unsigned char*)L.held.p);
-    LH(hipGetLastError(), "hold kernel");
+    if (!r.ok) L.st.fallbacks++;
+    L.fallback = !r.ok;
     return NB_OK;
 }
 
@@ -751,17 +797,19 @@ nb_status near_field(nb_ctx* c, Learned& L, double eps) {
 // queued so far on the main stream; near_field_join() makes the main stream
 // wait for it.  On an exact substep it runs beside the all-pairs kernel and
 // fills the CUs that kernel's waves leave idle; on an extrapolated one beside
-// the extrapolation and the exact sums of the flagged particles.  rebuild:
-// sort the particles into a new grid (exact substeps); otherwise the grid of
-// the last exact substep is searched wider by the displacement since (the
-// same pairs, without the sort).
-nb_status near_field_fork(nb_ctx* c, Learned& L, double eps, bool rebuild) {
+// the extrapolation and the exact sums of the flagged particles.  The grid
+// is rebuilt every `period` substeps (on the exact ones of the regular
+// cycle); in between the kept grid is searched wider by the displacement
+// since its build (the same pairs, without the sort).
+nb_status near_field_fork(nb_ctx* c, Learned& L, double eps) {
     if (!(L.rc > 0)) return NB_OK;
+    const bool rebuild = L.grid_age + 1 >= L.period;
     LH(hipEventRecord(L.ev_fork, nbh::ctx_stream(c)), "record event");
     LH(hipStreamWaitEvent(L.side, L.ev_fork, 0), "stream wait");
     LH(L.near.near_accel(nbh::ctx_pm(c), (int)L.n, L.rc, eps, (P4*)L.nearacc.p, L.side,
                          L.exact_kform != nbk::FORM_FP64, rebuild),
        "near-field kernels");
+    L.grid_age = rebuild ? 0 : L.grid_age + 1;
     LH(hipEventRecord(L.ev_join, L.side), "record event");
     return NB_OK;
 }
@@ -815,6 +863,18 @@ nb_status exact_force(nb_ctx* c, int kform, double eps) {
     return NB_OK;
 }
 
+// The direct sum with its finish fused with the kick and drift of schedule
+// q, as the exact forms step (acc is written too).  The near field, which
+// reads the positions on the side stream, is joined before the finish moves
+// them.
+nb_status exact_force_kick(nb_ctx* c, Learned& L, int kform, double eps, const Schedule& q) {
+    nbh::ForceOut out;
+    LH(nbh::force_eval(c, kform, eps, out), "force kernel");
+    if (nb_status e = near_field_join(c, L)) return e;
+    LH(nbh::finish_kick_drift(c, out, q.kdt, q.c1, q.c2), "kick/drift kernel");
+    return NB_OK;
+}
+
 nb_status kick_drift(nb_ctx* c, const Schedule& q) {
     nbh::ForceOut self;
     self.sym = false;
@@ -835,16 +895,19 @@ void destroy(Learned* l) {
     if (l->ev_join) (void)hipEventDestroy(l->ev_join);
     if (l->side) (void)hipStreamDestroy(l->side);
     if (l->nflag_h) (void)hipHostFree(l->nflag_h);
+    if (l->dec_ev) (void)hipEventDestroy(l->dec_ev);
+    if (l->dec_h) (void)hipHostFree(l->dec_h);
     for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
                       &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub, &l->frame,
                       &l->pred, &l->chk, &l->mask, &l->held, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc,
-                      &l->nearacc})
+                      &l->nearacc, &l->dec})
         b->release();
     delete l;
 }
 
 void on_new_state(Learned* l) {
     if (!l) return;
+    (void)check_finish(*l);   // a check still pending counts in the stats
     l->count = 0;
     l->head = 3;
     l->since_exact = 0;
@@ -854,6 +917,8 @@ void on_new_state(Learned* l) {
     l->pred_net = l->net_ok = false;
     l->clear_held = true;
     l->nflag = 0;
+    l->rough_hold = 0;
+    l->grid_age = 1 << 20;   // the next near field sorts a new grid
 }
 
 nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc, const double* dd, size_t lf,
@@ -878,50 +943,86 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
     L.substeps = (int)(lf - 1);
     for (size_t t = 0; t < time_steps; t++) {
         for (size_t s = 0; s + 1 < lf; s++) {
+            const Schedule q = schedule(t, time_steps, s, dt, cc, dd, lf);
             const bool full = (L.count == 4);
-            const bool exact = !full || L.fallback || (L.since_exact + 1 >= L.period);
+            const bool held_exact = L.rough_hold > 0;
+            if (held_exact) L.rough_hold--;
+            // a held substep with four more after it neither needs the near
+            // field nor enters the history: those four replace its sample
+            // before anything is extrapolated
+            const bool quiet = held_exact && L.rough_hold >= 4;
+            // exact on the schedule's own terms, known without the pending check
+            const bool sched = !full || held_exact || (L.since_exact + 1 >= L.period);
+            // the last check's decision is taken after this substep's near
+            // field is queued, so the GPU works while the host waits for it
+            bool forked = false;
+            if (L.rc > 0 && L.dec_pending && !quiet) {
+                if (nb_status e = near_field_fork(c, L, eps)) return e;
+                forked = true;
+            }
+            if (nb_status e = check_finish(L)) return e;
+            const bool exact = sched || L.fallback;
             if (exact) {
-                if (full) {
+                const bool checking = full && !held_exact;
+                if (checking) {
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, false, true)) return e;
                 }
-                if (nb_status e = near_field_fork(c, L, eps, true)) return e;
-                if (nb_status e = exact_force(c, L.exact_kform, eps)) return e;
-                if (nb_status e = near_field_join(c, L)) return e;
+                if (quiet) {
+                    L.grid_age = 1 << 20;   // the next near field sorts a new grid
+                } else if (!forked) {
+                    if (nb_status e = near_field_fork(c, L, eps)) return e;
+                }
+                if (nb_status e = exact_force_kick(c, L, L.exact_kform, eps, q)) return e;
                 L.st.exact_evals++;
                 // extrapolation starts only after a check has passed, so a
                 // network that produces garbage never reaches the integrator
-                if (full) {
-                    if (nb_status e = check(c, L)) return e;
+                if (checking) {
+                    if (nb_status e = check_enqueue(c, L)) return e;
                 }
-                if (nb_status e = push_history(c, L)) return e;
+                if (!quiet) {
+                    if (nb_status e = push_history(c, L)) return e;
+                }
                 L.since_exact = 0;
             } else {
                 // the near field runs on the side stream while the main
                 // stream extrapolates, reads the flag count back and sums the
                 // flagged particles exactly
                 if (L.rc > 0) {
-                    if (nb_status e = near_field_fork(c, L, eps, false)) return e;
+                    if (!forked) {
+                        if (nb_status e = near_field_fork(c, L, eps)) return e;
+                    }
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, true, L.net_ok)) return e;
-                    if (nb_status e = take_flag_count(L)) return e;
-                    if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
-                    if (nb_status e = near_field_join(c, L)) return e;
-                    hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.pred.p,
-                                       (const P4*)L.nearacc.p, (long)n, nbh::ctx_acc(c));
-                    LH(hipGetLastError(), "add kernel");
                 } else {
                     if (nb_status e = predict(c, L, nbh::ctx_acc(c), true, L.net_ok)) return e;
-                    if (nb_status e = take_flag_count(L)) return e;
-                    if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
                 }
-                if (nb_status e = exact_subset_scatter(c, L)) return e;
-                L.st.predicted_evals++;
-                if (L.pred_net) L.st.net_evals++;
-                L.st.subset_rows += (uint64_t)L.nflag;
-                L.since_exact++;
+                if (nb_status e = take_flag_count(L)) return e;
+                if ((double)L.nflag > L.rough_share * (double)n) {
+                    // so many exact rows (each a full row of N, the direct
+                    // sum's two) cost about what the direct sum does: it runs
+                    // instead, and on the next kRoughHold periods' substeps
+                    if (nb_status e = exact_force_kick(c, L, L.exact_kform, eps, q)) return e;
+                    if (nb_status e = push_history(c, L)) return e;
+                    L.st.exact_evals++;
+                    L.st.rough_exact++;
+                    L.since_exact = 0;
+                    L.rough_hold = (long)kRoughHold * L.period;
+                } else {
+                    if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
+                    if (L.rc > 0) {
+                        if (nb_status e = near_field_join(c, L)) return e;
+                        hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c),
+                                           (const P4*)L.pred.p, (const P4*)L.nearacc.p, (long)n, nbh::ctx_acc(c));
+                        LH(hipGetLastError(), "add kernel");
+                    }
+                    if (nb_status e = exact_subset_scatter(c, L)) return e;
+                    if (nb_status e = kick_drift(c, q)) return e;
+                    L.st.predicted_evals++;
+                    if (L.pred_net) L.st.net_evals++;
+                    L.st.subset_rows += (uint64_t)L.nflag;
+                    L.since_exact++;
+                }
             }
             L.sub++;
-            const Schedule q = schedule(t, time_steps, s, dt, cc, dd, lf);
-            if (nb_status e = kick_drift(c, q)) return e;
             L.clock += (q.c1 + q.c2) / dt;
         }
     }
@@ -973,11 +1074,11 @@ nb_status nb_learned_configure(nb_ctx* c, int period, double threshold, int exac
     if (kf < 0) return nbh::set_error(NB_ERR_ARG, "nb_learned_configure: exact_form must be FP64 or a mixed form");
     Learned* L = nbl::get(c);
     if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_configure: out of host memory");
+    nbl::on_new_state(L);   // takes a pending check first, so the stats below start from zero
     L->period = period;
     L->threshold = threshold;
     L->exact_kform = kf;
     L->st = nb_learned_stats{};
-    nbl::on_new_state(L);
     return NB_OK;
 }
 
@@ -987,6 +1088,15 @@ nb_status nb_learned_set_flag_ratio(nb_ctx* c, double rho) {
     Learned* L = nbl::get(c);
     if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_set_flag_ratio: out of host memory");
     L->rho = rho;
+    return NB_OK;
+}
+
+nb_status nb_learned_set_rough_share(nb_ctx* c, double share) {
+    if (!c) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_rough_share: null context");
+    if (!(share >= 0.0)) return nbh::set_error(NB_ERR_ARG, "nb_learned_set_rough_share: share must be >= 0");
+    Learned* L = nbl::get(c);
+    if (!L) return nbh::set_error(NB_ERR_ALLOC, "nb_learned_set_rough_share: out of host memory");
+    L->rough_share = share;
     return NB_OK;
 }
 
@@ -1157,6 +1267,9 @@ nb_status nb_learned_reset(nb_ctx* c, uint64_t seed) {
 nb_status nb_learned_stats_read(nb_ctx* c, nb_learned_stats* out) {
     if (!c || !out) return nbh::set_error(NB_ERR_ARG, "nb_learned_stats_read: null argument");
     Learned* L = nbh::ctx_learned(c);
+    if (L) {
+        if (nb_status e = nbl::check_finish(*L)) return e;   // a check still pending counts
+    }
     *out = L ? L->st : nb_learned_stats{};
     return NB_OK;
 }

@@ -39,7 +39,8 @@ EXPORTS = (
     "nb_learned_set_mode", "nb_learned_collect", "nb_learned_train", "nb_learned_stats_read",
     "nb_mlp_param_count", "nb_mlp_get_params", "nb_mlp_set_params", "nb_mlp_forward", "nb_mlp_grad",
     "nb_set_kernel", "nb_learned_reset", "nb_learned_set_flag_ratio", "nb_learned_set_near_radius",
-    "nb_near_accel", "nb_near_accel_kept", "nb_learned_set_hold_ratio", "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
+    "nb_near_accel", "nb_near_accel_kept", "nb_learned_set_hold_ratio", "nb_learned_set_rough_share",
+    "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
     "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
     "nb_ring_predict", "nb_kernel_info_read", "nb_mlp_adam_step", "nb_forces_ring_round",
     "nb_forces_ring_round_device", "nb_accel_block_device_noalloc",
@@ -64,7 +65,7 @@ class LearnedStats(C.Structure):
                 ("subset_rows", C.c_uint64), ("last_check_err", C.c_double), ("max_check_err", C.c_double),
                 ("net_evals", C.c_uint64), ("last_cubic_err", C.c_double), ("last_check_max", C.c_double),
                 ("checks", C.c_uint64), ("net_checks", C.c_uint64), ("net_cubic_err_sum", C.c_double),
-                ("net_err_sum", C.c_double)]
+                ("net_err_sum", C.c_double), ("rough_exact", C.c_uint64)]
 
 
 class KernelInfo(C.Structure):
@@ -112,6 +113,7 @@ def load_library(path: str = LIB_PATH):
     L.nb_near_accel.argtypes = [_VP, C.c_int, C.c_double, C.c_double, _P3]
     L.nb_near_accel_kept.argtypes = [_VP, C.c_int, C.c_double, C.c_double, _P3]
     L.nb_learned_set_hold_ratio.argtypes = [_VP, C.c_double]
+    L.nb_learned_set_rough_share.argtypes = [_VP, C.c_double]
     L.nb_snapshot_write.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_size_t, _D, _P3, _P3, _P3]
     L.nb_snapshot_frames.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.nb_snapshot_read.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, _D, _D, _P3, _P3, _P3]
@@ -430,6 +432,9 @@ class Solver:
 
     def learned_set_hold_ratio(self, kappa: float):
         _check(self.L.nb_learned_set_hold_ratio(self.h, kappa), "nb_learned_set_hold_ratio")
+
+    def learned_set_rough_share(self, share: float):
+        _check(self.L.nb_learned_set_rough_share(self.h, share), "nb_learned_set_rough_share")
 
     def near_accel(self, rc: float, eps: float = 0.0, form="fp64") -> np.ndarray:
         a = np.empty((3, self.n))

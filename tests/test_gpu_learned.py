@@ -104,7 +104,7 @@ def test_adam_steps_vs_oracle(solver, orc):
 # --- learned stepping ----------------------------------------------------------
 
 def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, params=None, tau=1.0, rc=0.0,
-                     kappa=1e-3):
+                     kappa=1e-3, rough_share=0.25):
     """numpy restatement of the learned form's schedule (order-2 DKD): the
     history holds the last four exact samples with their substeps, an
     extrapolated substep takes the cubic through them (Lagrange weights from
@@ -115,13 +115,16 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
     With rc > 0 the history holds
     a - a_near, the near field is summed exactly on every substep and added to
     the extrapolated remainder.  Particles rough, or held after missing the
-    last check by more than kappa a_rms, are summed exactly."""
+    last check by more than kappa a_rms, are summed exactly.  A substep due
+    for extrapolation with more than rough_share N such rows runs the direct
+    sum instead, and so do the substeps of the next 8 periods (without
+    checks)."""
     pos, vel = pos.copy(), vel.copy()
     n = m.shape[0]
     hist, htime = [], []
-    since, fallback, sub, net_ok = 0, True, 0, False
+    since, fallback, sub, net_ok, hold = 0, True, 0, False, 0
     held = np.zeros(n, dtype=bool)
-    st = {"exact": 0, "predicted": 0, "subset": 0}
+    st = {"exact": 0, "predicted": 0, "subset": 0, "rough": 0}
 
     def predict(allow_net):
         t = np.array(htime[::-1], dtype=float)       # newest first
@@ -148,12 +151,19 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
     pos += 0.5 * dt * vel
     for t in range(steps):
         full = len(hist) == 4
-        exact = (not full) or fallback or since + 1 >= period
+        held_exact = hold > 0
+        hold = max(hold - 1, 0)
+        exact = (not full) or held_exact or fallback or since + 1 >= period
         near = orc.near_accel(m, pos, rc, eps) if rc > 0 else np.zeros_like(pos)
+        if not exact:
+            _, pred, _, rough = predict(net_ok)
+            if float(rough.sum()) > rough_share * float(n):
+                exact, held_exact, hold = True, True, 8 * period
+                st["rough"] += 1
         if exact:
             acc = orc.accel(m, pos, eps)
             st["exact"] += 1
-            if full:
+            if full and not held_exact:
                 cub, pred, used, rough = predict(True)
                 smooth = ~rough
                 a2 = np.sum(acc[:, smooth] ** 2)
@@ -170,7 +180,6 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
             hist, htime = hist[-4:], htime[-4:]
             since = 0
         else:
-            _, pred, _, rough = predict(net_ok)
             acc = pred + near
             if rough.any():
                 ex = orc.accel(m, pos, eps)
@@ -184,6 +193,33 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
         if t + 1 < steps:
             pos += 0.5 * dt * vel
     return pos, vel, st
+
+
+def test_rough_guard_runs_the_direct_sum(solver):
+    """More than N/4 exact rows on a substep due for extrapolation (here every
+    particle is held after the first check: kappa = 1e-12): the direct sum
+    runs instead, and on the next 8 periods' substeps, so the run is the exact
+    run bit for bit; the guard fires at substeps 5, 38 and 71."""
+    m, pos, vel, _ = nbody_amd.init_system("plummer", 2000, 9)
+    steps, dt, eps = 80, 1.0 / 256, 0.01
+    solver.set_state(m, pos, vel)
+    solver.step(steps, dt, 2, "fp64", eps)
+    p_exact, v_exact = solver.get_state()
+    solver.learned_configure(period=4, threshold=1e-2, exact_form="fp64")
+    solver.learned_set_mode(1)
+    solver.learned_set_near_radius(0.0)
+    solver.learned_set_flag_ratio(0.1)
+    solver.learned_set_hold_ratio(1e-12)
+    try:
+        solver.set_state(m, pos, vel)
+        solver.step(steps, dt, 2, "learned", eps)
+        p, v = solver.get_state()
+        st = solver.learned_stats()
+    finally:
+        solver.learned_set_hold_ratio(1e-3)
+    assert st["checks"] == 1 and st["fallbacks"] == 0
+    assert st["rough_exact"] == 3 and st["predicted_evals"] == 0 and st["exact_evals"] == steps
+    assert np.array_equal(p, p_exact) and np.array_equal(v, v_exact)
 
 
 def test_threshold_zero_is_all_exact_bitwise(solver):
@@ -316,6 +352,7 @@ def test_learned_fourth_order_uses_real_substep_times(solver):
     solver.learned_configure(period=4, threshold=1e-2, exact_form="fp64")
     solver.learned_set_mode(1)
     solver.learned_set_near_radius(0.3)
+    solver.learned_set_rough_share(1.0)   # extrapolate however many rows are exact (dt = 1/128 flags many)
     try:
         solver.set_state(m, pos, vel)
         solver.step(steps, dt, 4, "learned", eps)
@@ -323,6 +360,7 @@ def test_learned_fourth_order_uses_real_substep_times(solver):
         st = solver.learned_stats()
     finally:
         solver.learned_set_near_radius(0.0)
+        solver.learned_set_rough_share(0.25)
     assert st["predicted_evals"] > 40 and st["fallbacks"] <= 2, st
     assert st["max_check_err"] < 1e-2
     assert np.max(np.abs(p - p_exact)) < 1e-6 * np.max(np.abs(p_exact))
