@@ -339,9 +339,9 @@ typedef struct nb_learned_stats {
     uint64_t net_checks;       /* of those, checks that also evaluated the network's extrapolation */
     double net_cubic_err_sum;  /* sum over net_checks of the cubic's RMS relative error */
     double net_err_sum;        /* sum over net_checks of the network's RMS relative error */
-    uint64_t rough_exact;      /* substeps due for extrapolation whose exact rows exceeded the rough share
-                                  of N: the direct sum ran instead (and on the next 8 periods' substeps);
-                                  counted in exact_evals */
+    uint64_t rough_exact;      /* substeps due for extrapolation after one whose exact rows exceeded the
+                                  rough share of N: the direct sum ran instead (and on the following
+                                  periods' substeps); counted in exact_evals */
 } nb_learned_stats;
 
 /* period >= 2 (exact every period-th substep), threshold > 0 (0 = always
@@ -358,10 +358,11 @@ NB_API nb_status nb_learned_set_flag_ratio(nb_ctx* ctx, double rho);
  * never holds on this ground). */
 NB_API nb_status nb_learned_set_hold_ratio(nb_ctx* ctx, double kappa);
 
-/* share >= 0: when a substep due for extrapolation has more than share * N
- * particles to sum exactly (rough or held), those rows cost about what the
- * direct sum does: the direct sum runs instead, and on the substeps of the
- * next 8 periods (default 0.25; 1 or more: never). */
+/* share >= 0: when an extrapolated substep had more than share * N
+ * particles to sum exactly (rough or held), such rows cost about what the
+ * direct sum does: the next substep runs the direct sum instead, and so do
+ * the substeps of the next 8 periods, twice as many each further time in a
+ * row up to 64 periods (default 0.25; 1 or more: never). */
 NB_API nb_status nb_learned_set_rough_share(nb_ctx* ctx, double share);
 
 /* rc >= 0: near-field radius.  With rc > 0 pairs closer than rc (tapered

@@ -54,6 +54,27 @@ struct AccelLaunch {
 
 hipError_t launch_accel(const AccelLaunch& L, hipStream_t st);
 
+// Rows [0, min(*count, ni_max)) of pi against pj[0, n), the row count read on
+// the device (no host round trip): a resident grid of `grid` blocks, splits
+// over j chosen on the device, partial sums in `part` (at least
+// accel_rows_part_records(grid, ni_max) records), their in-order sums in acc.
+struct RowsLaunch {
+    int form = FORM_FP64;
+    const P4* pi = nullptr;
+    const int* count = nullptr;
+    int ni_max = 0;
+    const P4* pj = nullptr;
+    int n = 0;
+    double eps2 = 0.0, G = 1.0;
+    int grid = 0;
+    P4* part = nullptr;
+    int4* shape = nullptr;   // (m, splits) for the reduction
+    P4* acc = nullptr;
+};
+
+hipError_t launch_accel_rows(const RowsLaunch& L, hipStream_t st);
+size_t accel_rows_part_records(int grid, int ni_max);
+
 // Symmetric kernel (nbody_sym.hip): blocks of S = 64*K particles, work items
 // (A, b0, b1, action slot) = one wave sweeping j-blocks [b0, b1) against
 // i-block A (slot layout described in nbody_sym.hip).

@@ -18,6 +18,7 @@
 //   a_i += m_j (x_j - x_i) / (|x_j - x_i|^2 + eps^2)^(3/2)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "nbody_kernels.h"
@@ -107,12 +108,13 @@ struct __attribute__((aligned(16))) P4f {
 // ---------------------------------------------------------------------------
 // Force kernel (asymmetric: all ordered pairs of i-set x j-range)
 // ---------------------------------------------------------------------------
+// One work item: rows [ibase, ibase + BLOCK IPT) of pi (those below ni)
+// against pj[jbeg, jend), written to o[i].  Every thread of the block takes
+// part (the j-tiles are staged through LDS with barriers).
 template <int FORM, bool EPS0, int IPT>
-__global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int ni,
-                                                 const P4* __restrict__ pj, int jbeg_all,
-                                                 int jend_all, int jchunk, double eps2, double G,
-                                                 double cshift, double coff, P4* __restrict__ out,
-                                                 long out_stride, int split_base) {
+__device__ __forceinline__ void accel_item(const P4* __restrict__ pi, int ni, const P4* __restrict__ pj, int jbeg,
+                                           int jend, double eps2, double G, double cshift, double coff,
+                                           P4* __restrict__ o, int ibase) {
     constexpr bool F32 = (FORM == FORM_FP32);
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR) || F32;
@@ -122,10 +124,6 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
     __shared__ Rec tile[2][BLOCK];
 
     const int t = threadIdx.x;
-    const int split = blockIdx.y;
-    const int jbeg = jbeg_all + split * jchunk;
-    const int jend = min(jend_all, jbeg + jchunk);
-    const int ibase = blockIdx.x * BLOCK * IPT;
 
     double xi[IPT], yi[IPT], zi[IPT];
     double ax[IPT], ay[IPT], az[IPT];
@@ -213,11 +211,50 @@ __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int 
             ax[k] = (double)gx[k]; ay[k] = (double)gy[k]; az[k] = (double)gz[k];
         }
     }
-    P4* o = out + (long)(split_base + split) * out_stride;
 #pragma unroll
     for (int k = 0; k < IPT; k++) {
         const int i = ibase + k * BLOCK + t;
         if (i < ni) o[i] = P4{G * ax[k], G * ay[k], G * az[k], 0.0};
+    }
+}
+
+template <int FORM, bool EPS0, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int ni,
+                                                 const P4* __restrict__ pj, int jbeg_all,
+                                                 int jend_all, int jchunk, double eps2, double G,
+                                                 double cshift, double coff, P4* __restrict__ out,
+                                                 long out_stride, int split_base) {
+    const int split = blockIdx.y;
+    const int jbeg = jbeg_all + split * jchunk;
+    const int jend = min(jend_all, jbeg + jchunk);
+    accel_item<FORM, EPS0, IPT>(pi, ni, pj, jbeg, jend, eps2, G, cshift, coff,
+                                out + (long)(split_base + split) * out_stride, blockIdx.x * BLOCK * IPT);
+}
+
+// Rows [0, m) of pi, m = min(*count, ni_max) read on the device, against all
+// of pj[0, n): a resident grid takes the items (row block, j-split) in turn.
+// The split count adapts to m -- enough splits that the items fill the grid,
+// each split a multiple of 64 particles and at least 256 -- and is written
+// with m to *shape (x = m, y = splits) for the reduction; partial sums go to
+// part[split * m + row].
+template <int FORM, bool EPS0>
+__global__ __launch_bounds__(BLOCK) void k_accel_rows(const P4* __restrict__ pi, const int* __restrict__ count,
+                                                      int ni_max, const P4* __restrict__ pj, int n, double eps2,
+                                                      double G, double cshift, double coff, P4* __restrict__ part,
+                                                      int4* __restrict__ shape) {
+    const int m = min(max(*count, 0), ni_max);
+    const int ng = (m + BLOCK - 1) / BLOCK;
+    const int smax = max(1, n / 256);
+    const int seff = ng > 0 ? min(max((int)gridDim.x / ng, 1), smax) : 1;
+    const int jchunk = ((n + seff - 1) / seff + 63) & ~63;
+    const int splits = (n + jchunk - 1) / jchunk;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *shape = make_int4(m, splits, 0, 0);
+    for (int it = blockIdx.x; it < ng * splits; it += gridDim.x) {   // uniform per block
+        const int sp = it % splits;
+        const int jbeg = sp * jchunk;
+        accel_item<FORM, EPS0, 1>(pi, m, pj, jbeg, min(n, jbeg + jchunk), eps2, G, cshift, coff,
+                                  part + (long)sp * m, (it / splits) * BLOCK);
+        __syncthreads();   // the next item restages the LDS tiles
     }
 }
 
@@ -557,6 +594,69 @@ hipError_t launch_accel(const AccelLaunch& L, hipStream_t st) {
     case FORM_FP32: return launch_accel_eps<FORM_FP32>(L, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// acc[i] = sum over s of part[s * m + i] in order s = 0..S-1 for i < m,
+// (m, S) = shape->x, shape->y from k_accel_rows: 16 waves per block take
+// every 16th split of a 64-row block, then wave 0 adds the 16 sums in order;
+// the blocks take the row blocks in turn.
+__global__ __launch_bounds__(64 * kWideWaves) void k_parts_rows(const P4* __restrict__ part,
+                                                                const int4* __restrict__ shape,
+                                                                P4* __restrict__ acc) {
+    __shared__ double sh[kWideWaves][3][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m = shape->x, S = shape->y;
+    for (int rb = blockIdx.x; rb * 64 < m; rb += gridDim.x) {   // uniform per block
+        const int i = rb * 64 + lane;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (i < m)
+            for (int q = w; q < S; q += kWideWaves) {
+                const P4 p = part[(long)q * m + i];
+                ax += p.x; ay += p.y; az += p.z;
+            }
+        sh[w][0][lane] = ax;
+        sh[w][1][lane] = ay;
+        sh[w][2][lane] = az;
+        __syncthreads();
+        if (w == 0 && i < m) {
+            ax = ay = az = 0.0;
+            for (int q = 0; q < kWideWaves; q++) {
+                ax += sh[q][0][lane];
+                ay += sh[q][1][lane];
+                az += sh[q][2][lane];
+            }
+            acc[i] = P4{ax, ay, az, 0.0};
+        }
+        __syncthreads();   // sh is rewritten for the next row block
+    }
+}
+
+template <int FORM>
+static hipError_t launch_accel_rows_t(const RowsLaunch& L, hipStream_t st) {
+    auto k = (L.eps2 == 0.0) ? k_accel_rows<FORM, true> : k_accel_rows<FORM, false>;
+    hipLaunchKernelGGL(k, dim3(L.grid), dim3(BLOCK), 0, st, L.pi, L.count, L.ni_max, L.pj, L.n, L.eps2, L.G,
+                       kCubicShift, kCubicOffset, L.part, L.shape);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int rblocks = std::max(1, std::min(L.grid, (L.ni_max + 63) / 64));
+    hipLaunchKernelGGL(k_parts_rows, dim3(rblocks), dim3(64 * kWideWaves), 0, st, (const P4*)L.part,
+                       (const int4*)L.shape, L.acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_accel_rows(const RowsLaunch& L, hipStream_t st) {
+    if (L.ni_max <= 0 || L.n <= 0 || L.grid <= 0) return hipSuccess;
+    switch (L.form) {
+    case FORM_FP64: return launch_accel_rows_t<FORM_FP64>(L, st);
+    case FORM_MIXED: return launch_accel_rows_t<FORM_MIXED>(L, st);
+    case FORM_MIXED_NR: return launch_accel_rows_t<FORM_MIXED_NR>(L, st);
+    case FORM_FP32: return launch_accel_rows_t<FORM_FP32>(L, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+size_t accel_rows_part_records(int grid, int ni_max) {
+    return std::max((size_t)grid * BLOCK, (size_t)std::max(ni_max, 1));
 }
 
 hipError_t launch_reduce(const P4* part, int S, long stride, int n, P4* acc, hipStream_t st) {

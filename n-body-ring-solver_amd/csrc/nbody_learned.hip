@@ -363,16 +363,18 @@ __global__ __launch_bounds__(BLK) void k_add(const P4* __restrict__ a, const P4*
     dst[i] = P4{u.x + v.x, u.y + v.y, u.z + v.z, 0.0};
 }
 
-__global__ __launch_bounds__(BLK) void k_gather(const P4* __restrict__ src, const int* __restrict__ idx, int m,
-                                                P4* __restrict__ dst) {
+// The flagged rows, m = min(*count, cap) read on the device (grids sized
+// for cap): gathered into a dense set, and their exact sums scattered back.
+__global__ __launch_bounds__(BLK) void k_gather(const P4* __restrict__ src, const int* __restrict__ idx,
+                                                const int* __restrict__ count, int cap, P4* __restrict__ dst) {
     const int k = blockIdx.x * BLK + threadIdx.x;
-    if (k < m) dst[k] = src[idx[k]];
+    if (k < min(*count, cap)) dst[k] = src[idx[k]];
 }
 
-__global__ __launch_bounds__(BLK) void k_scatter(const P4* __restrict__ src, const int* __restrict__ idx, int m,
-                                                 P4* __restrict__ dst) {
+__global__ __launch_bounds__(BLK) void k_scatter(const P4* __restrict__ src, const int* __restrict__ idx,
+                                                 const int* __restrict__ count, int cap, P4* __restrict__ dst) {
     const int k = blockIdx.x * BLK + threadIdx.x;
-    if (k < m) dst[idx[k]] = src[k];
+    if (k < min(*count, cap)) dst[idx[k]] = src[k];
 }
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
@@ -417,9 +419,11 @@ __global__ __launch_bounds__(BLK) void k_sumsq(const float* __restrict__ dsT, lo
 
 inline dim3 blocks(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 
-// An extrapolated substep whose exact rows exceed rough_share N runs the
-// direct sum instead, and so do the substeps of the next kRoughHold periods.
+// After an extrapolated substep whose exact rows exceed rough_share N the
+// direct sum runs instead, and on the substeps of the next kRoughHold
+// periods, twice as many for each further time in a row (up to 8 times).
 constexpr int kRoughHold = 8;
+constexpr int kRoughDoublings = 3;
 
 }  // namespace
 
@@ -447,7 +451,8 @@ struct Learned {
     DevBuf bX, bT, bY, bZ, bH, bD, dbpart, losspart;
     // per-particle state
     size_t n = 0;
-    DevBuf hist, X, Y, cub, frame, pred, chk, mask, held, flags, nflags, sub_p, sub_part, sub_acc, nearacc;
+    DevBuf hist, X, Y, cub, frame, pred, chk, mask, held, flags, nflags, sub_p, sub_part, sub_acc, sub_shape, nearacc;
+    int rows_grid = 0;         // resident blocks of the flagged rows' force kernel
     int head = 3, count = 0;   // ring of the last four exact samples (head newest)
     double htime[4] = {0, 0, 0, 0};   // their times, in steps
     long sub = 0;              // substeps since the state was set
@@ -459,10 +464,10 @@ struct Learned {
     bool pred_net = false;     // the last extrapolation applied the network
     bool net_ok = false;       // the network passed the last check: use it until the next
     bool clear_held = true;    // held to be cleared before the next substep
-    int nflag = 0;
+    long prev_rough = -1;            // flagged rows of the previous substep when it was extrapolated, else -1
     int* nflag_h = nullptr;          // pinned host copy of the flag count
     hipEvent_t nflag_ev = nullptr;   // recorded after that copy
-    bool nflag_wait = false;         // a copy is in flight: take_flag_count() waits for it
+    bool nflag_wait = false;         // a copy not yet taken by take_flag_count()
     hipStream_t side = nullptr;      // the near field runs here, beside the main stream's work
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int grid_age = 0;                // substeps since the near-field grid was last rebuilt
@@ -473,7 +478,8 @@ struct Learned {
     hipEvent_t dec_ev = nullptr;
     bool dec_pending = false;
     bool dec_net = false;            // the pending check evaluated the network
-    long rough_hold = 0;             // substeps still exact after the exact rows outgrew a quarter of N
+    long rough_hold = 0;             // substeps still exact after the exact rows outgrew their share of N
+    int rough_streak = 0;            // times in a row the exact rows outgrew it
     nb_learned_stats st{};
 };
 
@@ -632,9 +638,8 @@ inline int4 ring_slots(const Learned& L) {
 // for the current substep into `out` (the cubic alone stays in L.cub); marks
 // and lists the particles to be summed exactly.  The network is applied when
 // `allow_net`, it is on and trained, and the history is evenly spaced (noted
-// in L.pred_net).  With `count`, queues the copy of how many are marked to the
-// host; take_flag_count() waits for it, so work queued in between keeps the
-// device busy while the host waits.
+// in L.pred_net).  With `count`, queues the copy of how many are marked to
+// pinned host memory; take_flag_count() takes it at the next substep.
 nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
@@ -660,56 +665,58 @@ nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
     return NB_OK;
 }
 
-// The flag count of the last predict(count = true), into L.nflag.
+// The flag count of the last predict(count = true), if not yet taken: into
+// L.prev_rough and the stats.  Taken at the top of the next substep, when the
+// device has long passed the copy (the work queued behind it keeps it busy).
 nb_status take_flag_count(Learned& L) {
     if (!L.nflag_wait) return NB_OK;
     LH(hipEventSynchronize(L.nflag_ev), "hipEventSynchronize");
     L.nflag_wait = false;
-    L.nflag = std::min<int>(std::max(*L.nflag_h, 0), (int)L.n);
+    L.prev_rough = std::min<long>(std::max(*L.nflag_h, 0), (long)L.n);
+    L.st.subset_rows += (uint64_t)L.prev_rough;
     return NB_OK;
 }
 
 // Exact sums of the flagged particles into L.sub_acc; exact_subset_scatter()
-// writes them into acc.
+// writes them into acc.  Their count stays on the device: the kernels read
+// it and size the work themselves (grids for the worst case, every row).
 nb_status exact_subset_compute(nb_ctx* c, Learned& L, double eps, int kform) {
-    if (L.nflag <= 0) return NB_OK;
     hipStream_t st = nbh::ctx_stream(c);
-    const long n = (long)L.n, m = L.nflag;
-    const int ipt = 1;
-    int S = nbh::choose_splits(c, m, n, ipt);
-    long jchunk = (n + S - 1) / S;
-    jchunk = (jchunk + 63) & ~63L;
-    S = int((n + jchunk - 1) / jchunk);
-    LH(L.sub_p.ensure(m * sizeof(P4)), "allocate subset");
-    LH(L.sub_part.ensure((size_t)S * m * sizeof(P4)), "allocate subset partials");
-    LH(L.sub_acc.ensure(m * sizeof(P4)), "allocate subset output");
-    hipLaunchKernelGGL(k_gather, blocks(m), dim3(BLK), 0, st, (const P4*)nbh::ctx_pm(c), (const int*)L.flags.p, (int)m,
-                       (P4*)L.sub_p.p);
+    const int n = (int)L.n;
+    if (L.rows_grid <= 0) {
+        int dev = 0, ncu = 0;
+        LH(hipGetDevice(&dev), "hipGetDevice");
+        LH(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+        L.rows_grid = std::max(1, 4 * ncu);
+    }
+    LH(L.sub_p.ensure((size_t)n * sizeof(P4)), "allocate subset");
+    LH(L.sub_part.ensure(nbk::accel_rows_part_records(L.rows_grid, n) * sizeof(P4)), "allocate subset partials");
+    LH(L.sub_acc.ensure((size_t)n * sizeof(P4)), "allocate subset output");
+    LH(L.sub_shape.ensure(sizeof(int4)), "allocate subset shape");
+    hipLaunchKernelGGL(k_gather, blocks(n), dim3(BLK), 0, st, (const P4*)nbh::ctx_pm(c), (const int*)L.flags.p,
+                       (const int*)L.nflags.p, n, (P4*)L.sub_p.p);
     LH(hipGetLastError(), "gather kernel");
-    nbk::AccelLaunch A;
+    nbk::RowsLaunch A;
     A.form = kform;
-    A.ipt = ipt;
     A.pi = (const P4*)L.sub_p.p;
-    A.ni = (int)m;
+    A.count = (const int*)L.nflags.p;
+    A.ni_max = n;
     A.pj = (const P4*)nbh::ctx_pm(c);
-    A.jbeg = 0;
-    A.jend = (int)n;
-    A.jchunk = (int)jchunk;
-    A.splits = S;
+    A.n = n;
     A.eps2 = eps * eps;
     A.G = 1.0;
-    A.out = (P4*)L.sub_part.p;
-    A.out_stride = m;
-    LH(nbk::launch_accel(A, st), "subset force kernel");
-    LH(nbk::launch_reduce((const P4*)L.sub_part.p, S, m, (int)m, (P4*)L.sub_acc.p, st), "subset reduce kernel");
+    A.grid = L.rows_grid;
+    A.part = (P4*)L.sub_part.p;
+    A.shape = (int4*)L.sub_shape.p;
+    A.acc = (P4*)L.sub_acc.p;
+    LH(nbk::launch_accel_rows(A, st), "subset force kernels");
     return NB_OK;
 }
 
 nb_status exact_subset_scatter(nb_ctx* c, Learned& L) {
-    if (L.nflag <= 0) return NB_OK;
-    const long m = L.nflag;
-    hipLaunchKernelGGL(k_scatter, blocks(m), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.sub_acc.p,
-                       (const int*)L.flags.p, (int)m, nbh::ctx_acc(c));
+    const int n = (int)L.n;
+    hipLaunchKernelGGL(k_scatter, blocks(n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.sub_acc.p,
+                       (const int*)L.flags.p, (const int*)L.nflags.p, n, nbh::ctx_acc(c));
     LH(hipGetLastError(), "scatter kernel");
     return NB_OK;
 }
@@ -900,14 +907,15 @@ void destroy(Learned* l) {
     for (DevBuf* b : {&l->params, &l->am, &l->av, &l->grad, &l->netmem, &l->lossbuf, &l->dsX, &l->dsT, &l->bX, &l->bT,
                       &l->bY, &l->bZ, &l->bH, &l->bD, &l->dbpart, &l->losspart, &l->hist, &l->X, &l->Y, &l->cub, &l->frame,
                       &l->pred, &l->chk, &l->mask, &l->held, &l->flags, &l->nflags, &l->sub_p, &l->sub_part, &l->sub_acc,
-                      &l->nearacc, &l->dec})
+                      &l->sub_shape, &l->nearacc, &l->dec})
         b->release();
     delete l;
 }
 
 void on_new_state(Learned* l) {
     if (!l) return;
-    (void)check_finish(*l);   // a check still pending counts in the stats
+    (void)check_finish(*l);   // a check or flag count still pending counts in the stats
+    (void)take_flag_count(*l);
     l->count = 0;
     l->head = 3;
     l->since_exact = 0;
@@ -916,8 +924,9 @@ void on_new_state(Learned* l) {
     l->fallback = true;
     l->pred_net = l->net_ok = false;
     l->clear_held = true;
-    l->nflag = 0;
+    l->prev_rough = -1;
     l->rough_hold = 0;
+    l->rough_streak = 0;
     l->grid_age = 1 << 20;   // the next near field sorts a new grid
 }
 
@@ -953,25 +962,25 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
             const bool quiet = held_exact && L.rough_hold >= 4;
             // exact on the schedule's own terms, known without the pending check
             const bool sched = !full || held_exact || (L.since_exact + 1 >= L.period);
-            // the last check's decision is taken after this substep's near
-            // field is queued, so the GPU works while the host waits for it
-            bool forked = false;
-            if (L.rc > 0 && L.dec_pending && !quiet) {
+            // this substep's near field is queued first: the GPU works on it
+            // while the host takes the last check's decision and the last
+            // flag count (copies the device passed earlier)
+            if (!quiet) {
                 if (nb_status e = near_field_fork(c, L, eps)) return e;
-                forked = true;
             }
             if (nb_status e = check_finish(L)) return e;
+            if (nb_status e = take_flag_count(L)) return e;
+            const long prev_rough = L.prev_rough;   // the previous substep's flagged rows, if it extrapolated
+            L.prev_rough = -1;
+            const bool rough = prev_rough >= 0 && (double)prev_rough > L.rough_share * (double)n;
+            if (prev_rough >= 0 && !rough) L.rough_streak = 0;
             const bool exact = sched || L.fallback;
             if (exact) {
                 const bool checking = full && !held_exact;
                 if (checking) {
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, false, true)) return e;
                 }
-                if (quiet) {
-                    L.grid_age = 1 << 20;   // the next near field sorts a new grid
-                } else if (!forked) {
-                    if (nb_status e = near_field_fork(c, L, eps)) return e;
-                }
+                if (quiet) L.grid_age = 1 << 20;   // the next near field sorts a new grid
                 if (nb_status e = exact_force_kick(c, L, L.exact_kform, eps, q)) return e;
                 L.st.exact_evals++;
                 // extrapolation starts only after a check has passed, so a
@@ -983,44 +992,40 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                     if (nb_status e = push_history(c, L)) return e;
                 }
                 L.since_exact = 0;
+            } else if (rough) {
+                // the previous substep had so many exact rows (each a full
+                // row of N, the direct sum's two) that they cost about what
+                // the direct sum does: it runs instead, here and on the
+                // substeps of the next kRoughHold periods (doubled for each
+                // further time in a row)
+                if (nb_status e = exact_force_kick(c, L, L.exact_kform, eps, q)) return e;
+                if (nb_status e = push_history(c, L)) return e;
+                L.st.exact_evals++;
+                L.st.rough_exact++;
+                L.since_exact = 0;
+                L.rough_hold = ((long)kRoughHold * L.period) << std::min(L.rough_streak, kRoughDoublings);
+                L.rough_streak++;
             } else {
                 // the near field runs on the side stream while the main
-                // stream extrapolates, reads the flag count back and sums the
-                // flagged particles exactly
+                // stream extrapolates and sums the flagged particles exactly
+                // (their count stays on the device)
                 if (L.rc > 0) {
-                    if (!forked) {
-                        if (nb_status e = near_field_fork(c, L, eps)) return e;
-                    }
                     if (nb_status e = predict(c, L, (P4*)L.pred.p, true, L.net_ok)) return e;
                 } else {
                     if (nb_status e = predict(c, L, nbh::ctx_acc(c), true, L.net_ok)) return e;
                 }
-                if (nb_status e = take_flag_count(L)) return e;
-                if ((double)L.nflag > L.rough_share * (double)n) {
-                    // so many exact rows (each a full row of N, the direct
-                    // sum's two) cost about what the direct sum does: it runs
-                    // instead, and on the next kRoughHold periods' substeps
-                    if (nb_status e = exact_force_kick(c, L, L.exact_kform, eps, q)) return e;
-                    if (nb_status e = push_history(c, L)) return e;
-                    L.st.exact_evals++;
-                    L.st.rough_exact++;
-                    L.since_exact = 0;
-                    L.rough_hold = (long)kRoughHold * L.period;
-                } else {
-                    if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
-                    if (L.rc > 0) {
-                        if (nb_status e = near_field_join(c, L)) return e;
-                        hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c),
-                                           (const P4*)L.pred.p, (const P4*)L.nearacc.p, (long)n, nbh::ctx_acc(c));
-                        LH(hipGetLastError(), "add kernel");
-                    }
-                    if (nb_status e = exact_subset_scatter(c, L)) return e;
-                    if (nb_status e = kick_drift(c, q)) return e;
-                    L.st.predicted_evals++;
-                    if (L.pred_net) L.st.net_evals++;
-                    L.st.subset_rows += (uint64_t)L.nflag;
-                    L.since_exact++;
+                if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
+                if (L.rc > 0) {
+                    if (nb_status e = near_field_join(c, L)) return e;
+                    hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.pred.p,
+                                       (const P4*)L.nearacc.p, (long)n, nbh::ctx_acc(c));
+                    LH(hipGetLastError(), "add kernel");
                 }
+                if (nb_status e = exact_subset_scatter(c, L)) return e;
+                if (nb_status e = kick_drift(c, q)) return e;
+                L.st.predicted_evals++;
+                if (L.pred_net) L.st.net_evals++;
+                L.since_exact++;
             }
             L.sub++;
             L.clock += (q.c1 + q.c2) / dt;
@@ -1267,8 +1272,9 @@ nb_status nb_learned_reset(nb_ctx* c, uint64_t seed) {
 nb_status nb_learned_stats_read(nb_ctx* c, nb_learned_stats* out) {
     if (!c || !out) return nbh::set_error(NB_ERR_ARG, "nb_learned_stats_read: null argument");
     Learned* L = nbh::ctx_learned(c);
-    if (L) {
-        if (nb_status e = nbl::check_finish(*L)) return e;   // a check still pending counts
+    if (L) {   // a check or flag count still pending counts
+        if (nb_status e = nbl::check_finish(*L)) return e;
+        if (nb_status e = nbl::take_flag_count(*L)) return e;
     }
     *out = L ? L->st : nb_learned_stats{};
     return NB_OK;

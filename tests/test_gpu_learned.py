@@ -116,13 +116,14 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
     a - a_near, the near field is summed exactly on every substep and added to
     the extrapolated remainder.  Particles rough, or held after missing the
     last check by more than kappa a_rms, are summed exactly.  A substep due
-    for extrapolation with more than rough_share N such rows runs the direct
-    sum instead, and so do the substeps of the next 8 periods (without
-    checks)."""
+    for extrapolation right after one that extrapolated with more than
+    rough_share N such rows runs the direct sum instead, and so do the
+    substeps of the next 8 periods (without checks), twice as many for each
+    further time in a row, up to 64 periods."""
     pos, vel = pos.copy(), vel.copy()
     n = m.shape[0]
     hist, htime = [], []
-    since, fallback, sub, net_ok, hold = 0, True, 0, False, 0
+    since, fallback, sub, net_ok, hold, prev, streak = 0, True, 0, False, 0, -1, 0
     held = np.zeros(n, dtype=bool)
     st = {"exact": 0, "predicted": 0, "subset": 0, "rough": 0}
 
@@ -155,11 +156,14 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
         hold = max(hold - 1, 0)
         exact = (not full) or held_exact or fallback or since + 1 >= period
         near = orc.near_accel(m, pos, rc, eps) if rc > 0 else np.zeros_like(pos)
-        if not exact:
-            _, pred, _, rough = predict(net_ok)
-            if float(rough.sum()) > rough_share * float(n):
-                exact, held_exact, hold = True, True, 8 * period
-                st["rough"] += 1
+        prev_rough, prev = prev, -1
+        rough_prev = prev_rough >= 0 and float(prev_rough) > rough_share * float(n)
+        if prev_rough >= 0 and not rough_prev:
+            streak = 0
+        if not exact and rough_prev:
+            exact, held_exact, hold = True, True, (8 * period) << min(streak, 3)
+            streak += 1
+            st["rough"] += 1
         if exact:
             acc = orc.accel(m, pos, eps)
             st["exact"] += 1
@@ -180,12 +184,14 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
             hist, htime = hist[-4:], htime[-4:]
             since = 0
         else:
+            _, pred, _, rough = predict(net_ok)
             acc = pred + near
             if rough.any():
                 ex = orc.accel(m, pos, eps)
                 acc[:, rough] = ex[:, rough]
             st["predicted"] += 1
             st["subset"] += int(rough.sum())
+            prev = int(rough.sum())
             since += 1
         sub += 1
         vel += dt * acc
@@ -196,10 +202,12 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
 
 
 def test_rough_guard_runs_the_direct_sum(solver):
-    """More than N/4 exact rows on a substep due for extrapolation (here every
-    particle is held after the first check: kappa = 1e-12): the direct sum
-    runs instead, and on the next 8 periods' substeps, so the run is the exact
-    run bit for bit; the guard fires at substeps 5, 38 and 71."""
+    """More than N/4 exact rows on an extrapolated substep (here nearly every
+    particle is held after the first check: kappa = 1e-12): the next substep
+    runs the direct sum instead, and so do the next 8 periods' substeps, then
+    16 periods' the second time in a row.  Substep 4 checks; 5 and 39
+    extrapolate (nearly every row summed exactly); 6 and 40 are the guard's;
+    the rest are exact."""
     m, pos, vel, _ = nbody_amd.init_system("plummer", 2000, 9)
     steps, dt, eps = 80, 1.0 / 256, 0.01
     solver.set_state(m, pos, vel)
@@ -218,8 +226,12 @@ def test_rough_guard_runs_the_direct_sum(solver):
     finally:
         solver.learned_set_hold_ratio(1e-3)
     assert st["checks"] == 1 and st["fallbacks"] == 0
-    assert st["rough_exact"] == 3 and st["predicted_evals"] == 0 and st["exact_evals"] == steps
-    assert np.array_equal(p, p_exact) and np.array_equal(v, v_exact)
+    assert st["rough_exact"] == 2 and st["predicted_evals"] == 2 and st["exact_evals"] == steps - 2
+    # (all but the far, smoothest particles miss by more than 1e-12 a_rms)
+    assert 2 * m.shape[0] // 4 < st["subset_rows"] <= 2 * m.shape[0]
+    # every force was summed exactly (three substeps row by row): the exact run to rounding
+    assert np.max(np.abs(p - p_exact)) < 1e-12 * np.max(np.abs(p_exact))
+    assert np.max(np.abs(v - v_exact)) < 1e-10 * np.max(np.abs(v_exact))
 
 
 def test_threshold_zero_is_all_exact_bitwise(solver):
