@@ -424,6 +424,8 @@ inline dim3 blocks(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 // periods, twice as many for each further time in a row (up to 8 times).
 constexpr int kRoughHold = 8;
 constexpr int kRoughDoublings = 3;
+// The near-field grid is sorted anew after at most this many substeps.
+constexpr int kMaxGridAge = 4;
 
 }  // namespace
 
@@ -805,12 +807,15 @@ nb_status near_field(nb_ctx* c, Learned& L, double eps) {
 // wait for it.  On an exact substep it runs beside the all-pairs kernel and
 // fills the CUs that kernel's waves leave idle; on an extrapolated one beside
 // the extrapolation and the exact sums of the flagged particles.  The grid
-// is rebuilt every `period` substeps (on the exact ones of the regular
-// cycle); in between the kept grid is searched wider by the displacement
-// since its build (the same pairs, without the sort).
+// is rebuilt every min(period, kMaxGridAge) substeps (with period 4 on the
+// exact ones of the regular cycle); in between the kept grid is searched
+// wider by the displacement since its build (the same pairs, without the
+// sort; the wider search outgrows the sort it saves after ~4 substeps at
+// rc = 0.35: period 8 3.40 ms per step rebuilding every 8th, 3.28 every 4th,
+// 3.30 every 2nd).
 nb_status near_field_fork(nb_ctx* c, Learned& L, double eps) {
     if (!(L.rc > 0)) return NB_OK;
-    const bool rebuild = L.grid_age + 1 >= L.period;
+    const bool rebuild = L.grid_age + 1 >= std::min(L.period, kMaxGridAge);
     LH(hipEventRecord(L.ev_fork, nbh::ctx_stream(c)), "record event");
     LH(hipStreamWaitEvent(L.side, L.ev_fork, 0), "stream wait");
     LH(L.near.near_accel(nbh::ctx_pm(c), (int)L.n, L.rc, eps, (P4*)L.nearacc.p, L.side,
