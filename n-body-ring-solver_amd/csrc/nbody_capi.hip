@@ -81,6 +81,7 @@ struct nb_ctx {
     int num_cus = 256;
     size_t n = 0;
     std::vector<double> mass_h;
+    double eq_mass = 0.0;   // the common mass when every particle has it (> 0, finite), else 0
     DevBuf pm, vel, acc, part, rows, eblk;
     DevBuf bi, bj, bout, bout2;
     DevBuf rk;   // Runge-Kutta stage records: x0, kv[s], ka[s]
@@ -369,7 +370,7 @@ static int public_form(int kform) {
 
 // Record what a force evaluation of the resident system ran (nb_kernel_info_read).
 static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int block, uint64_t items,
-                        uint64_t scratch_bytes, bool grouped = false) {
+                        uint64_t scratch_bytes, bool grouped = false, bool eqm = false) {
     nb_kernel_info& k = c->kinfo;
     k.kernel = kernel;
     k.form = public_form(kform);
@@ -377,19 +378,33 @@ static void note_kernel(nb_ctx* c, int kernel, int kform, int per_lane, int bloc
     k.block = block;
     k.items = items;
     k.scratch_bytes = scratch_bytes;
+    const char* eq = eqm ? ", equal masses" : "";
     if (kernel == NB_KERNEL_SYMMETRIC && grouped)
-        std::snprintf(k.name, sizeof k.name, "k_accel_symg<%s, K=%d, W=%d>", form_label(kform), per_lane,
-                      nbk::kGroupWaves);
+        std::snprintf(k.name, sizeof k.name, "k_accel_symg<%s, K=%d, W=%d%s>", form_label(kform), per_lane,
+                      nbk::kGroupWaves, eq);
     else if (kernel == NB_KERNEL_SYMMETRIC)
-        std::snprintf(k.name, sizeof k.name, "k_accel_sym<%s, K=%d>", form_label(kform), per_lane);
+        std::snprintf(k.name, sizeof k.name, "k_accel_sym<%s, K=%d%s>", form_label(kform), per_lane, eq);
     else
         std::snprintf(k.name, sizeof k.name, "k_accel<%s, IPT=%d>", form_label(kform), per_lane);
 }
 
 // One force evaluation of the resident state; `out` says where the partial
 // sums are and how to add them.
+// The mass an equal-mass evaluation leaves out of its sums: the common mass
+// times the form's factor on the staged masses (0: the masses stay in).
+static double eq_scale(const nb_ctx* c, int kform) {
+    if (c->eq_mass == 0.0) return 0.0;
+    switch (kform) {
+    case nbk::FORM_FP64: return c->eq_mass * nbk::kCubicMass;
+    case nbk::FORM_MIXED_NR: return c->eq_mass * nbk::kNewtonMass;
+    case nbk::FORM_MIXED: return c->eq_mass;
+    default: return 0.0;
+    }
+}
+
 hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     const long n = long(c->n);
+    const double scale = eq_scale(c, kform);
     int K;
     if (use_symmetric(c, kform, K) && grouped_size(n)) {
         const nb_ctx::SymgPlan* plan = nullptr;
@@ -412,6 +427,7 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         L.act = (P4*)c->part.p;
         if (kform == nbk::FORM_FP64) L.react64 = (P4*)react;
         else L.react32 = (float4*)react;
+        L.eqm = scale != 0.0;
         out.sym = true;
         out.part = L.act;
         out.react64 = L.react64;
@@ -419,7 +435,9 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         out.nB = plan->nB;
         out.S = int(S);
         out.rowseg = (const int2*)plan->rowseg.p;
-        note_kernel(c, NB_KERNEL_SYMMETRIC, kform, K, int(S), uint64_t(plan->nwg), R.act_bytes + R.react_bytes, true);
+        out.scale = L.eqm ? scale : 1.0;
+        note_kernel(c, NB_KERNEL_SYMMETRIC, kform, K, int(S), uint64_t(plan->nwg), R.act_bytes + R.react_bytes, true,
+                    L.eqm);
         const long ev = timing_begin(c);
         e = nbk::launch_accel_symg(L, c->stream);
         timing_end(c, ev);
@@ -447,6 +465,7 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         L.part = (P4*)c->part.p;
         if (kform == nbk::FORM_FP64) L.react64 = (P4*)react;
         else L.react32 = (float4*)react;
+        L.eqm = scale != 0.0;
         out.sym = true;
         out.part = L.part;
         out.react64 = L.react64;
@@ -454,7 +473,9 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         out.nB = P.nB;
         out.S = 64 * K;
         out.L = P.L;
-        note_kernel(c, NB_KERNEL_SYMMETRIC, kform, K, int(S), uint64_t(P.nitems), R.act_bytes + R.react_bytes);
+        out.scale = L.eqm ? scale : 1.0;
+        note_kernel(c, NB_KERNEL_SYMMETRIC, kform, K, int(S), uint64_t(P.nitems), R.act_bytes + R.react_bytes, false,
+                    L.eqm);
         const long ev = timing_begin(c);
         e = nbk::launch_accel_sym(L, c->stream);
         timing_end(c, ev);
@@ -507,6 +528,7 @@ static nbk::SymgSums symg_sums(const ForceOut& out) {
     P.react32 = out.react32;
     P.nB = out.nB;
     P.S = out.S;
+    P.scale = out.scale;
     return P;
 }
 
@@ -518,6 +540,7 @@ static nbk::SymSums sym_sums(const ForceOut& out) {
     P.nB = out.nB;
     P.S = out.S;
     P.L = out.L;
+    P.scale = out.scale;
     return P;
 }
 
@@ -691,6 +714,9 @@ nb_status nb_set_state(nb_ctx* c, size_t n, const double* mass, const double* co
         NB_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     }
     c->mass_h.assign(mass, mass + n);
+    c->eq_mass = (n > 0 && mass[0] > 0.0 && std::isfinite(mass[0])) ? mass[0] : 0.0;
+    for (size_t i = 1; i < n && c->eq_mass != 0.0; i++)
+        if (mass[i] != mass[0]) c->eq_mass = 0.0;
     c->n = n;
     c->kinfo = nb_kernel_info{};
     nbl::on_new_state(c->learned);

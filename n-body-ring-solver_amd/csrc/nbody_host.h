@@ -42,6 +42,7 @@ struct ForceOut {
     int S = 1;          // tile kernel: number of j-splits; symmetric: block size
     int nB = 0, L = 1;  // symmetric: blocks, tiles per work item
     const int2* rowseg = nullptr;   // grouped symmetric kernel: action segments per group row (else null)
+    double scale = 1.0;             // symmetric: factor of the sums (the mass an equal-mass evaluation left out)
 };
 
 // Context internals used by the learned-form module.

@@ -94,6 +94,9 @@ struct SymLaunch {
     // pairs q > k (q >= k with incl); nB more reaction slots for the diagonal
     const P4* pj = nullptr;
     bool incl = false;
+    // every mass equal (one form only): the sums leave out m times the
+    // form's mass factor, which the finish applies (SymSums::scale)
+    bool eqm = false;
 };
 
 hipError_t launch_accel_sym(const SymLaunch& L, hipStream_t st);
@@ -105,6 +108,7 @@ struct SymSums {
     const P4* react64 = nullptr;
     const float4* react32 = nullptr;
     int nB = 0, S = 0, L = 1;
+    double scale = 1.0;   // factor of the sums: the mass factored out of an equal-mass evaluation, else 1
 };
 hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st);
 hipError_t launch_sym_kick_drift(const SymSums& P, int n, P4* pm, P4* vel, P4* acc, double kdt, double c1dt,
@@ -131,6 +135,7 @@ struct SymgLaunch {
     P4* act = nullptr;
     P4* react64 = nullptr;
     float4* react32 = nullptr;
+    bool eqm = false;   // every mass equal: as SymLaunch::eqm
 };
 hipError_t launch_accel_symg(const SymgLaunch& L, hipStream_t st);
 
@@ -142,6 +147,7 @@ struct SymgSums {
     const P4* react64 = nullptr;
     const float4* react32 = nullptr;
     int nB = 0, S = 0;
+    double scale = 1.0;   // as SymSums::scale
 };
 // acc = the sums (kick: then v += kdt a, x += c1dt v + c2dt v as the kick/drift kernels)
 hipError_t launch_symg_finish(const SymgSums& P, int n, P4* pm, P4* vel, P4* acc, bool kick, double kdt, double c1dt,

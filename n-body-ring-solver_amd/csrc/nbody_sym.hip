@@ -65,8 +65,9 @@ __device__ __forceinline__ float rotate_down(float v) {
 // v_rsq_f64 and the second-order correction (1-e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2
 // written as ((s + cshift)^2 + coff), s = r2 y^2 (kCubicShift, kCubicOffset),
 // with the 15/8 already in both masses; a_i += m_j w d, b_j -= m_i w d.
-// `keep` zeroes masked pairs.
-template <bool EPS0, bool MASK>
+// `keep` zeroes masked pairs.  MUL false (every mass equal, factored out of
+// the sums): a_i += w d, b_j -= w d, the masses unused.
+template <bool EPS0, bool MASK, bool MUL = true>
 __device__ __forceinline__ void sym64(double xi, double yi, double zi, double mi, double xj, double yj,
                                       double zj, double mj, double eps2, double cshift, double coff, bool keep,
                                       double& ax, double& ay, double& az, double& bx, double& by,
@@ -80,7 +81,7 @@ __device__ __forceinline__ void sym64(double xi, double yi, double zi, double mi
     double w = (t * y) * p;
     if (EPS0) w = (r2 > 0.0) ? w : 0.0;
     if (MASK) w = keep ? w : 0.0;
-    const double si = mj * w, sj = mi * w;
+    const double si = MUL ? mj * w : w, sj = MUL ? mi * w : w;
     ax = __builtin_fma(si, dx, ax);
     ay = __builtin_fma(si, dy, ay);
     az = __builtin_fma(si, dz, az);
@@ -96,7 +97,7 @@ __device__ __forceinline__ void sym64(double xi, double yi, double zi, double mi
 // gain as refining y by y (3 - r2 y^2)/2 and cubing, for two instructions
 // fewer: 1 + 3/2 e = c (F - r2 y^2) with F = kNewtonOffset and c = kNewtonMass
 // already in both masses.
-template <bool EPS0, bool MASK, bool NR>
+template <bool EPS0, bool MASK, bool NR, bool MUL = true>
 __device__ __forceinline__ void sym32(double xi, double yi, double zi, float mi, double xj, double yj,
                                       double zj, float mj, float eps2, bool keep, float& ax, float& ay,
                                       float& az, float& bx, float& by, float& bz) {
@@ -108,7 +109,7 @@ __device__ __forceinline__ void sym32(double xi, double yi, double zi, float mi,
     if (NR) w = w * __builtin_fmaf(-r2, t, kNewtonOffset);
     if (EPS0) w = (r2 > 0.0f) ? w : 0.0f;
     if (MASK) w = keep ? w : 0.0f;
-    const float si = mj * w, sj = mi * w;
+    const float si = MUL ? mj * w : w, sj = MUL ? mi * w : w;
     ax = __builtin_fmaf(si, dx, ax);
     ay = __builtin_fmaf(si, dy, ay);
     az = __builtin_fmaf(si, dz, az);
@@ -132,7 +133,58 @@ __device__ __forceinline__ P4 load_or_zero(const P4* __restrict__ p, int j, int 
 
 }  // namespace
 
-template <int FORM, bool EPS0, int K, bool TWO>
+// One chunk (64 records staged twice in lds) against the lane's K particles;
+// the chunk's reaction sums end in their home lanes.  DIAG: the chunk is the
+// c-th of the wave's own block (pairs i < j only; i <= j with incl).  MUL
+// false: the pair terms without masses (equal masses, padding-free chunk).
+template <int FORM, bool EPS0, int K, bool DIAG, bool MUL = true>
+__device__ __forceinline__ void sweep_chunk(const P4* __restrict__ lds, int lane, int c, int tmin,
+                                            const double (&xi)[K], const double (&yi)[K], const double (&zi)[K],
+                                            const double (&mi)[K], const float (&mif)[K], double eps2,
+                                            double cshift, double coff, double (&ax)[K], double (&ay)[K],
+                                            double (&az)[K], double& bx, double& by, double& bz) {
+    constexpr bool MIXED = (FORM != FORM_FP64);
+    constexpr bool NR = (FORM == FORM_MIXED_NR);
+    if constexpr (!MIXED) {
+        bx = by = bz = 0.0;
+#pragma unroll 2
+        for (int t = 0; t < 64; t++) {
+            const P4 J = lds[lane + t];
+            const bool same_ok = (t >= tmin) && (lane + t < 64);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                sym64<EPS0, DIAG, MUL>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
+                                       (k < c) || (k == c && same_ok), ax[k], ay[k], az[k], bx, by, bz);
+            bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
+        }
+    } else {
+        const float eps2f = (float)eps2;
+        float fbx = 0.f, fby = 0.f, fbz = 0.f;
+        float fax[K], fay[K], faz[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) fax[k] = fay[k] = faz[k] = 0.f;
+#pragma unroll 1
+        for (int t = 0; t < 64; t++) {
+            const P4 J = lds[lane + t];
+            const float mj = (float)J.m;
+            const bool same_ok = (t >= tmin) && (lane + t < 64);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                sym32<EPS0, DIAG, NR, MUL>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f,
+                                           (k < c) || (k == c && same_ok), fax[k], fay[k], faz[k], fbx, fby, fbz);
+            fbx = rotate_down(fbx); fby = rotate_down(fby); fbz = rotate_down(fbz);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            ax[k] += (double)fax[k];
+            ay[k] += (double)fay[k];
+            az[k] += (double)faz[k];
+        }
+        bx = (double)fbx; by = (double)fby; bz = (double)fbz;
+    }
+}
+
+template <int FORM, bool EPS0, int K, bool TWO, bool EQM>
 __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __restrict__ pm,
                                                                    const P4* __restrict__ pj, int n,
                                                                    const int4* __restrict__ items, int nitems,
@@ -143,8 +195,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR);
     constexpr int S = 64 * K;
-    // the factor of the correction polynomial carried by the masses
+    // the factor of the correction polynomial carried by the masses; EQM
+    // (every mass equal, m > 0): the sums leave the factor m MSCALE to the
+    // finish, a "mass" is 1 for a particle and 0 for padding, and the chunks
+    // with no padding on either side take the pair terms without it
     constexpr double MSCALE = MIXED ? (NR ? kNewtonMass : 1.0) : kCubicMass;
+    auto mass_of = [](double m) { return EQM ? (m != 0.0 ? 1.0 : 0.0) : m * MSCALE; };
     __shared__ P4 jt[kWavesPerGroup][128];   // each wave's chunk, stored twice
 
     const int wave = threadIdx.x >> 6;
@@ -159,10 +215,9 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const P4 p = load_or_zero(pm, A * S + k * 64 + lane, n);
-        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.m * MSCALE;
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = mass_of(p.m);
         ax[k] = ay[k] = az[k] = 0.0;
     }
-    const float eps2f = (float)eps2;
     float mif[K];
 #pragma unroll
     for (int k = 0; k < K; k++) mif[k] = (float)mi[k];
@@ -171,72 +226,22 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
         const bool diag = (B == A);
         for (int c = 0; c < K; c++) {
             P4 q = load_or_zero(TWO ? pj : pm, B * S + c * 64 + lane, n);
-            q.m *= MSCALE;
+            q.m = mass_of(q.m);
             lds[lane] = q;
             lds[lane + 64] = q;
             wave_sync();
-            double bx, by, bz;
-            if constexpr (!MIXED) {
-                bx = by = bz = 0.0;
-                if (!diag) {
-#pragma unroll 2
-                    for (int t = 0; t < 64; t++) {
-                        const P4 J = lds[lane + t];
-#pragma unroll
-                        for (int k = 0; k < K; k++)
-                            sym64<EPS0, false>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
-                                               true, ax[k], ay[k], az[k], bx, by, bz);
-                        bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
-                    }
-                } else {
-                    for (int t = 0; t < 64; t++) {
-                        const P4 J = lds[lane + t];
-                        // i = c*64+lane < j = c*64+lane+t (<= in the two-set form with incl)
-                        const bool same_ok = (t >= (TWO ? 1 - incl : 1)) && (lane + t < 64);
-#pragma unroll
-                        for (int k = 0; k < K; k++)
-                            sym64<EPS0, true>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
-                                              (k < c) || (k == c && same_ok), ax[k], ay[k], az[k], bx, by, bz);
-                        bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
-                    }
-                }
-            } else {
-                float fbx = 0.f, fby = 0.f, fbz = 0.f;
-                float fax[K], fay[K], faz[K];
-#pragma unroll
-                for (int k = 0; k < K; k++) fax[k] = fay[k] = faz[k] = 0.f;
-                if (!diag) {
-#pragma unroll 1
-                    for (int t = 0; t < 64; t++) {
-                        const P4 J = lds[lane + t];
-                        const float mj = (float)J.m;
-#pragma unroll
-                        for (int k = 0; k < K; k++)
-                            sym32<EPS0, false, NR>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f, true,
-                                                   fax[k], fay[k], faz[k], fbx, fby, fbz);
-                        fbx = rotate_down(fbx); fby = rotate_down(fby); fbz = rotate_down(fbz);
-                    }
-                } else {
-                    for (int t = 0; t < 64; t++) {
-                        const P4 J = lds[lane + t];
-                        const float mj = (float)J.m;
-                        const bool same_ok = (t >= (TWO ? 1 - incl : 1)) && (lane + t < 64);
-#pragma unroll
-                        for (int k = 0; k < K; k++)
-                            sym32<EPS0, true, NR>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f,
-                                                  (k < c) || (k == c && same_ok), fax[k], fay[k], faz[k], fbx,
-                                                  fby, fbz);
-                        fbx = rotate_down(fbx); fby = rotate_down(fby); fbz = rotate_down(fbz);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    ax[k] += (double)fax[k];
-                    ay[k] += (double)fay[k];
-                    az[k] += (double)faz[k];
-                }
-                bx = (double)fbx; by = (double)fby; bz = (double)fbz;
-            }
+            double bx = 0.0, by = 0.0, bz = 0.0;
+            // padding: in the wave's own block (the last) or in this chunk
+            const bool pad = (A + 1) * S > n || B * S + (c + 1) * 64 > n;
+            if (diag)
+                sweep_chunk<FORM, EPS0, K, true>(lds, lane, c, TWO ? 1 - incl : 1, xi, yi, zi, mi, mif, eps2, cshift,
+                                                 coff, ax, ay, az, bx, by, bz);
+            else if (EQM && !pad)
+                sweep_chunk<FORM, EPS0, K, false, false>(lds, lane, c, 1, xi, yi, zi, mi, mif, eps2, cshift, coff, ax,
+                                                         ay, az, bx, by, bz);
+            else
+                sweep_chunk<FORM, EPS0, K, false>(lds, lane, c, 1, xi, yi, zi, mi, mif, eps2, cshift, coff, ax, ay,
+                                                  az, bx, by, bz);
             if (diag && !TWO) {
                 // the chunk is this wave's own particles k == c: fold the reactions in
 #pragma unroll
@@ -276,57 +281,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
 // order: results are bitwise reproducible.
 // ---------------------------------------------------------------------------
 
-// One chunk (64 records staged twice in lds) against the lane's K particles;
-// the chunk's reaction sums end in their home lanes.  DIAG: the chunk is the
-// c-th of the wave's own block (pairs i < j only; i <= j with incl).
-template <int FORM, bool EPS0, int K, bool DIAG>
-__device__ __forceinline__ void sweep_chunk(const P4* __restrict__ lds, int lane, int c, int tmin,
-                                            const double (&xi)[K], const double (&yi)[K], const double (&zi)[K],
-                                            const double (&mi)[K], const float (&mif)[K], double eps2,
-                                            double cshift, double coff, double (&ax)[K], double (&ay)[K],
-                                            double (&az)[K], double& bx, double& by, double& bz) {
-    constexpr bool MIXED = (FORM != FORM_FP64);
-    constexpr bool NR = (FORM == FORM_MIXED_NR);
-    if constexpr (!MIXED) {
-        bx = by = bz = 0.0;
-#pragma unroll 2
-        for (int t = 0; t < 64; t++) {
-            const P4 J = lds[lane + t];
-            const bool same_ok = (t >= tmin) && (lane + t < 64);
-#pragma unroll
-            for (int k = 0; k < K; k++)
-                sym64<EPS0, DIAG>(xi[k], yi[k], zi[k], mi[k], J.x, J.y, J.z, J.m, eps2, cshift, coff,
-                                  (k < c) || (k == c && same_ok), ax[k], ay[k], az[k], bx, by, bz);
-            bx = rotate_down(bx); by = rotate_down(by); bz = rotate_down(bz);
-        }
-    } else {
-        const float eps2f = (float)eps2;
-        float fbx = 0.f, fby = 0.f, fbz = 0.f;
-        float fax[K], fay[K], faz[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) fax[k] = fay[k] = faz[k] = 0.f;
-#pragma unroll 1
-        for (int t = 0; t < 64; t++) {
-            const P4 J = lds[lane + t];
-            const float mj = (float)J.m;
-            const bool same_ok = (t >= tmin) && (lane + t < 64);
-#pragma unroll
-            for (int k = 0; k < K; k++)
-                sym32<EPS0, DIAG, NR>(xi[k], yi[k], zi[k], mif[k], J.x, J.y, J.z, mj, eps2f,
-                                      (k < c) || (k == c && same_ok), fax[k], fay[k], faz[k], fbx, fby, fbz);
-            fbx = rotate_down(fbx); fby = rotate_down(fby); fbz = rotate_down(fbz);
-        }
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            ax[k] += (double)fax[k];
-            ay[k] += (double)fay[k];
-            az[k] += (double)faz[k];
-        }
-        bx = (double)fbx; by = (double)fby; bz = (double)fbz;
-    }
-}
-
-template <int FORM, bool EPS0, int K, int W, bool TWO>
+template <int FORM, bool EPS0, int K, int W, bool TWO, bool EQM>
 __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm, const P4* __restrict__ pj, int n,
                                                        const int4* __restrict__ ranges, int nB, double eps2,
                                                        double cshift, double coff, int incl, P4* __restrict__ act,
@@ -335,6 +290,10 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
     constexpr bool NR = (FORM == FORM_MIXED_NR);
     constexpr int S = 64 * K;
     constexpr double MSCALE = MIXED ? (NR ? kNewtonMass : 1.0) : kCubicMass;
+    // EQM (every mass equal, m > 0): the sums leave the factor m MSCALE to the
+    // finish; a staged "mass" is 1 for a particle and 0 for padding, and the
+    // chunks with no padding on either side take the pair terms without it
+    auto mass_of = [](double m) { return EQM ? (m != 0.0 ? 1.0 : 0.0) : m * MSCALE; };
     __shared__ P4 lds[128];             // the chunk, stored twice so lane + t never wraps
     __shared__ double red[W][3][64];    // the waves' reaction sums of the chunk
 
@@ -354,7 +313,7 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const P4 p = load_or_zero(pm, A * S + k * 64 + lane, n);
-            xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = p.m * MSCALE; mif[k] = (float)mi[k];
+            xi[k] = p.x; yi[k] = p.y; zi[k] = p.z; mi[k] = mass_of(p.m); mif[k] = (float)mi[k];
             ax[k] = ay[k] = az[k] = 0.0;
         }
         // settle these loads here (once per row): otherwise the sweeps, which
@@ -400,7 +359,7 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
             const int idx = tid + d * 64 * W;
             if (idx < 256) {
                 double* l = (double*)lds;
-                const double v = (idx & 3) == 3 ? nxt[d] * MSCALE : nxt[d];
+                const double v = (idx & 3) == 3 ? mass_of(nxt[d]) : nxt[d];
                 l[idx] = v;
                 l[idx + 256] = v;
             }
@@ -416,8 +375,14 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
         fetch(B2, c2, nxt);
         if (wave == 0) store_pending();
         double bx = 0.0, by = 0.0, bz = 0.0;
-        if (B > A) sweep_chunk<FORM, EPS0, K, false>(lds, lane, c, tmin, xi, yi, zi, mi, mif, eps2, cshift, coff, ax,
+        // padding: in the wave's own block (the last) or in this chunk
+        const bool pad = (A + 1) * S > n || B * S + (c + 1) * 64 > n;
+        if (B > A && EQM && !pad)
+            sweep_chunk<FORM, EPS0, K, false, false>(lds, lane, c, tmin, xi, yi, zi, mi, mif, eps2, cshift, coff, ax,
                                                      ay, az, bx, by, bz);
+        else if (B > A)
+            sweep_chunk<FORM, EPS0, K, false>(lds, lane, c, tmin, xi, yi, zi, mi, mif, eps2, cshift, coff, ax, ay, az,
+                                              bx, by, bz);
         else if (B == A) sweep_chunk<FORM, EPS0, K, true>(lds, lane, c, tmin, xi, yi, zi, mi, mif, eps2, cshift, coff,
                                                           ax, ay, az, bx, by, bz);
         red[wave][0][lane] = bx;
@@ -502,14 +467,16 @@ template <bool F32R, int W, bool KICK>
 __global__ __launch_bounds__(BLOCK) void k_symg_finish(const P4* __restrict__ act, const int2* __restrict__ rowseg,
                                                        const P4* __restrict__ react64,
                                                        const float4* __restrict__ react32, int nB, int S, int n,
-                                                       P4* __restrict__ pm, P4* __restrict__ vel,
+                                                       double scale, P4* __restrict__ pm, P4* __restrict__ vel,
                                                        P4* __restrict__ acc, double kdt, double c1dt, double c2dt) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double bx, by, bz, ax, ay, az;
     symg_react<F32R, W>(react64, react32, nB, S, i, bx, by, bz);
     symg_act<W>(act, rowseg, S, i, ax, ay, az);
-    ax += bx; ay += by; az += bz;
+    ax = scale * (ax + bx);   // scale: the factored-out mass (1 when the sums hold it)
+    ay = scale * (ay + by);
+    az = scale * (az + bz);
     acc[i] = P4{ax, ay, az, 0.0};
     if constexpr (KICK) {
         P4 v = vel[i];
@@ -664,25 +631,28 @@ __global__ __launch_bounds__(BLOCK) void k_ring_finish(const P4* __restrict__ pa
 template <bool F32R>
 __global__ __launch_bounds__(BLOCK) void k_sym_reduce(const P4* __restrict__ part, const P4* __restrict__ react64,
                                                       const float4* __restrict__ react32, int nB, int S, int L,
-                                                      int n, P4* __restrict__ acc) {
+                                                      int n, double scale, P4* __restrict__ acc) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double ax, ay, az;
     sym_sum<F32R>(part, react64, react32, nB, S, L, i, ax, ay, az);
-    acc[i] = P4{ax, ay, az, 0.0};
+    acc[i] = P4{scale * ax, scale * ay, scale * az, 0.0};   // scale: the factored-out mass (else 1)
 }
 
 template <bool F32R>
 __global__ __launch_bounds__(BLOCK) void k_sym_kick_drift(const P4* __restrict__ part,
                                                           const P4* __restrict__ react64,
                                                           const float4* __restrict__ react32, int nB, int S, int L,
-                                                          int n, P4* __restrict__ pm, P4* __restrict__ vel,
-                                                          P4* __restrict__ acc, double kdt, double c1dt,
-                                                          double c2dt) {
+                                                          int n, double scale, P4* __restrict__ pm,
+                                                          P4* __restrict__ vel, P4* __restrict__ acc, double kdt,
+                                                          double c1dt, double c2dt) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     double ax, ay, az;
     sym_sum<F32R>(part, react64, react32, nB, S, L, i, ax, ay, az);
+    ax *= scale;   // the factored-out mass (else 1)
+    ay *= scale;
+    az *= scale;
     acc[i] = P4{ax, ay, az, 0.0};
     P4 v = vel[i];
     v.x = __builtin_fma(kdt, ax, v.x);
@@ -703,12 +673,16 @@ template <int FORM, bool EPS0, int K>
 static hipError_t launch_sym_t(const SymLaunch& L, hipStream_t st) {
     const int groups = (L.nitems + kWavesPerGroup - 1) / kWavesPerGroup;
     if (L.pj)
-        hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K, true>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm,
-                           L.pj, L.n, L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, L.incl ? 1 : 0,
-                           L.part, L.react64, L.react32);
+        hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K, true, false>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st,
+                           L.pm, L.pj, L.n, L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset,
+                           L.incl ? 1 : 0, L.part, L.react64, L.react32);
+    else if (L.eqm)
+        hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K, false, true>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st,
+                           L.pm, L.pm, L.n, L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, 0, L.part,
+                           L.react64, L.react32);
     else
-        hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K, false>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st, L.pm,
-                           L.pm, L.n, L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, 0, L.part,
+        hipLaunchKernelGGL((k_accel_sym<FORM, EPS0, K, false, false>), dim3(groups), dim3(64 * kWavesPerGroup), 0, st,
+                           L.pm, L.pm, L.n, L.items, L.nitems, L.nB, L.eps2, kCubicShift, kCubicOffset, 0, L.part,
                            L.react64, L.react32);
     return hipGetLastError();
 }
@@ -744,12 +718,16 @@ template <int FORM, bool EPS0, int K>
 static hipError_t launch_symg_k(const SymgLaunch& L, hipStream_t st) {
     constexpr int W = kGroupWaves;
     if (L.pj)
-        hipLaunchKernelGGL((k_accel_symg<FORM, EPS0, K, W, true>), dim3(L.nwg), dim3(64 * W), 0, st, L.pm, L.pj, L.n,
-                           L.ranges, L.nB, L.eps2, kCubicShift, kCubicOffset, L.incl ? 1 : 0, L.act, L.react64,
+        hipLaunchKernelGGL((k_accel_symg<FORM, EPS0, K, W, true, false>), dim3(L.nwg), dim3(64 * W), 0, st, L.pm, L.pj,
+                           L.n, L.ranges, L.nB, L.eps2, kCubicShift, kCubicOffset, L.incl ? 1 : 0, L.act, L.react64,
                            L.react32);
-    else
-        hipLaunchKernelGGL((k_accel_symg<FORM, EPS0, K, W, false>), dim3(L.nwg), dim3(64 * W), 0, st, L.pm, L.pm,
+    else if (L.eqm)
+        hipLaunchKernelGGL((k_accel_symg<FORM, EPS0, K, W, false, true>), dim3(L.nwg), dim3(64 * W), 0, st, L.pm, L.pm,
                            L.n, L.ranges, L.nB, L.eps2, kCubicShift, kCubicOffset, 0, L.act, L.react64, L.react32);
+    else
+        hipLaunchKernelGGL((k_accel_symg<FORM, EPS0, K, W, false, false>), dim3(L.nwg), dim3(64 * W), 0, st, L.pm,
+                           L.pm, L.n, L.ranges, L.nB, L.eps2, kCubicShift, kCubicOffset, 0, L.act, L.react64,
+                           L.react32);
     return hipGetLastError();
 }
 
@@ -787,7 +765,7 @@ hipError_t launch_symg_finish(const SymgSums& P, int n, P4* pm, P4* vel, P4* acc
     const dim3 grid((n + BLOCK - 1) / BLOCK);
 #define NB_SYMG_FINISH(F32R, KICK)                                                                                 \
     hipLaunchKernelGGL((k_symg_finish<F32R, W, KICK>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,        \
-                       P.react32, P.nB, P.S, n, pm, vel, acc, kdt, c1dt, c2dt)
+                       P.react32, P.nB, P.S, n, P.scale, pm, vel, acc, kdt, c1dt, c2dt)
     if (P.react32) {
         if (kick) NB_SYMG_FINISH(true, true);
         else NB_SYMG_FINISH(true, false);
@@ -820,10 +798,10 @@ hipError_t launch_sym_reduce(const SymSums& P, int n, P4* acc, hipStream_t st) {
     const dim3 grid((n + BLOCK - 1) / BLOCK);
     if (P.react32)
         hipLaunchKernelGGL(k_sym_reduce<true>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB, P.S,
-                           P.L, n, acc);
+                           P.L, n, P.scale, acc);
     else
         hipLaunchKernelGGL(k_sym_reduce<false>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB, P.S,
-                           P.L, n, acc);
+                           P.L, n, P.scale, acc);
     return hipGetLastError();
 }
 
@@ -847,10 +825,10 @@ hipError_t launch_sym_kick_drift(const SymSums& P, int n, P4* pm, P4* vel, P4* a
     const dim3 grid((n + BLOCK - 1) / BLOCK);
     if (P.react32)
         hipLaunchKernelGGL(k_sym_kick_drift<true>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB,
-                           P.S, P.L, n, pm, vel, acc, kdt, c1dt, c2dt);
+                           P.S, P.L, n, P.scale, pm, vel, acc, kdt, c1dt, c2dt);
     else
         hipLaunchKernelGGL(k_sym_kick_drift<false>, grid, dim3(BLOCK), 0, st, P.part, P.react64, P.react32, P.nB,
-                           P.S, P.L, n, pm, vel, acc, kdt, c1dt, c2dt);
+                           P.S, P.L, n, P.scale, pm, vel, acc, kdt, c1dt, c2dt);
     return hipGetLastError();
 }
 

@@ -331,10 +331,10 @@ def test_ring_share_full_size_sampled(solver, orc):
 @pytest.mark.parametrize("form,n,kernel,name", [
     ("fp64", 2048, "tile", "k_accel<FP64, IPT=1>"),
     ("mixed", 16383, "tile", "k_accel<MIXED_NR, IPT=1>"),
-    ("fp64", 16384, "symmetric", "k_accel_sym<FP64, K=4>"),
-    ("mixed-raw", 40000, "symmetric", "k_accel_sym<MIXED, K=4>"),
-    ("fp64", 262144, "symmetric", "k_accel_symg<FP64, K=8, W=4>"),
-    ("mixed", 262144, "symmetric", "k_accel_symg<MIXED_NR, K=6, W=4>"),
+    ("fp64", 16384, "symmetric", "k_accel_sym<FP64, K=4, equal masses>"),
+    ("mixed-raw", 40000, "symmetric", "k_accel_sym<MIXED, K=4, equal masses>"),
+    ("fp64", 262144, "symmetric", "k_accel_symg<FP64, K=8, W=4, equal masses>"),
+    ("mixed", 262144, "symmetric", "k_accel_symg<MIXED_NR, K=6, W=4, equal masses>"),
 ])
 def test_kernel_info_names_the_kernel_that_ran(solver, form, n, kernel, name):
     """nb_kernel_info_read reports the kernel the library chose for the last
@@ -353,3 +353,30 @@ def test_kernel_info_names_the_kernel_that_ran(solver, form, n, kernel, name):
         assert solver.kernel_info()["kernel"] == "tile"
     finally:
         solver.set_kernel("auto")
+
+
+@pytest.mark.parametrize("n", [40000 + 13, 131072 + 77])
+@pytest.mark.parametrize("form", ["fp64", "mixed", "mixed-raw"])
+def test_equal_mass_sums_match_the_general_path(solver, orc, n, form):
+    """Every mass equal: the symmetric kernels leave the mass out of the pair
+    terms and the finish multiplies it in.  Against the same system with one
+    mass changed by 2^-50 (the general path: masses in every pair term) and
+    the oracle on sampled rows; a ragged last block (padding) at both sizes
+    (item kernel K = 4 at 40013, grouped kernel at 131149)."""
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 12, [1.0, 20.0])
+    tol = FP64_TOL if form == "fp64" else MIXED_TOL
+    solver.set_state(m, pos)
+    a_eq = solver.accel(form, 0.005)
+    assert solver.kernel_info()["name"].endswith("equal masses>")
+    m2 = m.copy()
+    m2[n // 2] *= 1.0 + 2.0 ** -50
+    assert m2[n // 2] != m[0]
+    solver.set_state(m2, pos)
+    a_gen = solver.accel(form, 0.005)
+    assert "equal" not in solver.kernel_info()["name"]
+    assert rel_err(a_eq, a_gen) < tol   # mixed: two FP32 roundings of each term, each within its bar
+    rows = np.random.default_rng(n).choice(n, 256, replace=False)
+    rows[0] = n - 1   # a particle of the padded last block
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    ref = orc.accel_block(pj[rows], pj, 0.005).T
+    assert rel_err(a_eq[:, rows], ref) < tol
