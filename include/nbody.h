@@ -141,9 +141,10 @@ NB_API nb_status nb_accel_block_device(nb_ctx* ctx, int form, double eps, size_t
  * reference; for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
  * same kernels and summation order, so the results equal
- * nb_accel_block_device's bit for bit, but the j-split partial sums go to a
- * static 256 MiB device array that the library holds on every device it runs
- * on until the process ends: calls whose partial sums need more (more than
+ * nb_accel_block_device's bit for bit, but the j-split partial sums (and
+ * the check whether every j mass is equal) go to a static device array of
+ * 256 MiB + 256 bytes that the library holds on every device it runs on
+ * until the process ends: calls whose partial sums need more (more than
  * the 131072 x 1048576 ring share in j-blocks of 131072) fail with NB_ERR_ARG
  * before anything is queued.  Every call on one device shares that array, so
  * calls that overlap on one device (two contexts, or two host threads) race;

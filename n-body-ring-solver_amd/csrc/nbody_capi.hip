@@ -38,14 +38,16 @@ nb_status hip_fail(hipError_t e, const char* what) {
 
 constexpr size_t kMaxN = 100000000;   // the reference's bound (CPU/main.cpp:271)
 
-// Partial sums of nb_accel_block_device_noalloc: a fixed array in device
-// memory that every GPU the library runs on holds until the process ends
-// (256 MiB: the 131072 x 1048576 ring share in j-blocks of 131072 needs
-// exactly that).  Host code never passes the array's name to a kernel or a
-// copy: it uses the address hipGetSymbolAddress returns for the device
-// (nb_ctx::scratch_ptr, taken in nb_create with the device current).
-constexpr size_t kStaticScratchBytes = size_t(256) << 20;
-static __device__ __align__(256) P4 scratch[kStaticScratchBytes / sizeof(P4)];
+// Scratch of nb_accel_block_device_noalloc: a fixed array in device memory
+// that every GPU the library runs on holds until the process ends -- a
+// 256-byte head (the j masses' check) and 256 MiB of partial sums (the
+// 131072 x 1048576 ring share in j-blocks of 131072 needs exactly that).
+// Host code never passes the array's name to a kernel or a copy: it uses the
+// address hipGetSymbolAddress returns for the device (nb_ctx::scratch_ptr,
+// taken in nb_create with the device current).
+constexpr size_t kStaticScratchBytes = size_t(256) << 20;   // partial sums
+constexpr size_t kScratchHead = 256;
+static __device__ __align__(256) P4 scratch[(kScratchHead + kStaticScratchBytes) / sizeof(P4)];
 
 __global__ void k_zero_records(P4* __restrict__ p, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,7 +85,7 @@ struct nb_ctx {
     std::vector<double> mass_h;
     double eq_mass = 0.0;   // the common mass when every particle has it (> 0, finite), else 0
     DevBuf pm, vel, acc, part, rows, eblk;
-    DevBuf bi, bj, bout, bout2;
+    DevBuf bi, bj, bout, bout2, eqstate;   // ring-share staging; eqstate: the j masses' check
     DevBuf rk;   // Runge-Kutta stage records: x0, kv[s], ka[s]
     // timing
     bool timing = false;
@@ -484,6 +486,7 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     nbk::AccelLaunch L;
     L.form = kform;
     L.ipt = choose_ipt(c, n);
+    L.eq_mass = c->eq_mass;
     int S = choose_splits(c, n, n, L.ipt);
     long jchunk = (n + S - 1) / S;
     jchunk = (jchunk + 63) & ~63L;
@@ -639,6 +642,7 @@ nb_status nb_destroy(nb_ctx* c) {
     nbl::destroy(c->learned);
     c->learned = nullptr;
     for (DevBuf* b : {&c->pm, &c->vel, &c->acc, &c->part, &c->rows, &c->eblk, &c->bi, &c->bj, &c->bout, &c->bout2,
+                      &c->eqstate,
                       &c->rk, &c->symg[0].ranges, &c->symg[0].rowseg, &c->symg[1].ranges, &c->symg[1].rowseg,
                       &c->sym[0].items, &c->sym[1].items})
         b->release();
@@ -801,8 +805,12 @@ nb_status nb_accel_device(nb_ctx* c, int form, double eps) {
 
 // Shared body of the ring-share entry points: d_pi (ni records) against d_pj
 // (nj records) in j-blocks of jb, partials then one ordered reduction.
+// Equal j masses (the pair terms then leave the mass out): known on the host
+// (eq_mass > 0) or found on the device into *eq_dev (when set) before the
+// force kernels, which read it there.
 static nb_status block_eval(nb_ctx* c, int kf, double eps, long ni, const P4* d_pi, long nj,
-                            const P4* d_pj, long jb, P4* d_out, P4* fixed = nullptr, size_t fixed_bytes = 0) {
+                            const P4* d_pj, long jb, P4* d_out, P4* fixed = nullptr, size_t fixed_bytes = 0,
+                            double eq_mass = 0.0, nbk::EqState* eq_dev = nullptr) {
     const int ipt = choose_ipt(c, ni);
     const long nb = (nj + jb - 1) / jb;
     int S = choose_splits(c, ni, jb, ipt);
@@ -824,6 +832,7 @@ static nb_status block_eval(nb_ctx* c, int kf, double eps, long ni, const P4* d_
             part = (P4*)c->part.p;
         }
     }
+    if (eq_dev) NB_HIP(nbk::launch_mass_check(d_pj, int(nj), eq_dev, c->stream), "mass check kernels");
     for (long b = 0; b < nb; b++) {
         nbk::AccelLaunch L;
         L.form = kf;
@@ -840,6 +849,8 @@ static nb_status block_eval(nb_ctx* c, int kf, double eps, long ni, const P4* d_
         L.out = part;
         L.out_stride = ni;
         L.split_base = int(b * S);
+        L.eq_mass = eq_mass;
+        L.eq_dev = eq_dev;
         const long ev = fixed ? -1 : timing_begin(c);
         hipError_t e = nbk::launch_accel(L, c->stream);
         timing_end(c, ev);
@@ -872,10 +883,15 @@ nb_status nb_accel_block(nb_ctx* c, int form, double eps, size_t ni, const doubl
     NB_HIP(c->bout.ensure(ni * sizeof(P4)), "allocate output");
     NB_HIP(hipMemcpyAsync(c->bi.p, pi, ni * sizeof(P4), hipMemcpyHostToDevice, c->stream), "upload i block");
     if (nj) NB_HIP(hipMemcpyAsync(c->bj.p, pj, nj * sizeof(P4), hipMemcpyHostToDevice, c->stream), "upload j set");
+    // the j records are at hand: when their masses are all equal the pair
+    // terms leave the mass out (the device entry points keep it in)
+    double eq = (nj > 0 && pj[3] > 0.0 && std::isfinite(pj[3])) ? pj[3] : 0.0;
+    for (size_t j = 1; j < nj && eq != 0.0; j++)
+        if (pj[4 * j + 3] != pj[3]) eq = 0.0;
     if (nj == 0) {
         NB_HIP(hipMemsetAsync(c->bout.p, 0, ni * sizeof(P4), c->stream), "clear output");
     } else if (nb_status s = block_eval(c, kf, eps, long(ni), (const P4*)c->bi.p, long(nj), (const P4*)c->bj.p, jb,
-                                        (P4*)c->bout.p)) {
+                                        (P4*)c->bout.p, nullptr, 0, eq)) {
         return s;
     }
     std::vector<P4> h(ni);
@@ -900,7 +916,9 @@ nb_status nb_accel_block_device(nb_ctx* c, int form, double eps, size_t ni, cons
         return NB_OK;
     }
     const long jb = (j_block == 0 || j_block > nj) ? long(nj) : long(j_block);
-    return block_eval(c, kf, eps, long(ni), (const P4*)d_pi, long(nj), (const P4*)d_pj, jb, (P4*)d_acc4);
+    NB_HIP(c->eqstate.ensure(sizeof(nbk::EqState)), "allocate mass check");
+    return block_eval(c, kf, eps, long(ni), (const P4*)d_pi, long(nj), (const P4*)d_pj, jb, (P4*)d_acc4, nullptr, 0,
+                      0.0, (nbk::EqState*)c->eqstate.p);
 }
 
 // One ring round on the device: the local set d_l against the received set
@@ -1010,8 +1028,10 @@ nb_status nb_accel_block_device_noalloc(nb_ctx* c, int form, double eps, size_t 
         return NB_OK;
     }
     const long jb = (j_block == 0 || j_block > nj) ? long(nj) : long(j_block);
+    // the scratch's first 256 bytes hold the mass check, the partial sums follow
     return block_eval(c, kf, eps, long(ni), (const P4*)d_pi, long(nj), (const P4*)d_pj, jb, (P4*)d_acc4,
-                      c->scratch_ptr, kStaticScratchBytes);
+                      c->scratch_ptr + kScratchHead / sizeof(P4), kStaticScratchBytes, 0.0,
+                      (nbk::EqState*)c->scratch_ptr);
 }
 
 nb_status nb_forces_ring_round(nb_ctx* c, int form, double eps, size_t local_n, const double* mass,

@@ -39,6 +39,14 @@ constexpr double kNewtonMass = 1.0 / ((double)kNewtonOffset - 1.0);
 // Launch description of the force kernel: accelerations of pi[0..ni) from
 // pj[jbeg..jend), the j-range cut into `splits` chunks of `jchunk` (one per
 // blockIdx.y); chunk s writes out[(split_base + s) * out_stride + i].
+// Whether every mass of a j set is equal (ok) and the value (m0), found on
+// the device by launch_mass_check.
+struct EqState {
+    double m0;
+    int ok, pad;
+};
+hipError_t launch_mass_check(const P4* pj, int nj, EqState* st, hipStream_t stream);
+
 struct AccelLaunch {
     int form = FORM_FP64;
     int ipt = 1;              // i-particles per lane (1 or 4)
@@ -50,6 +58,11 @@ struct AccelLaunch {
     P4* out = nullptr;
     long out_stride = 0;
     int split_base = 0;
+    // every j mass equal: the pair terms leave it out and the final scale
+    // takes it; known on the host (eq_mass > 0) or decided on the device
+    // (eq_dev, from launch_mass_check; takes precedence)
+    double eq_mass = 0.0;
+    const EqState* eq_dev = nullptr;
 };
 
 hipError_t launch_accel(const AccelLaunch& L, hipStream_t st);

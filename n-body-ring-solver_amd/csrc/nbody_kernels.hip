@@ -47,7 +47,7 @@ __device__ __forceinline__ P4 load_p4(const P4* __restrict__ p, int j, int jend)
 // and mj carries its factor 15/8 (staged with kCubicMass).  The constants
 // arrive as kernel arguments so they stay in registers (as literals the
 // compiler rebuilds them with two moves per pair).
-template <bool EPS0>
+template <bool EPS0, bool MUL = true>
 __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, double xj, double yj,
                                           double zj, double mj, double eps2, double cshift, double coff,
                                           double& ax, double& ay, double& az) {
@@ -58,7 +58,7 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
     const double u = __builtin_fma(r2, t, cshift);
     const double y3 = t * y;
     const double p = __builtin_fma(u, u, coff);
-    double s = (mj * y3) * p;
+    double s = (MUL ? mj * y3 : y3) * p;   // MUL false: every mass equal, multiplied in afterwards
     if (EPS0) s = (r2 > 0.0) ? s : 0.0;   // zero separation (self pair) contributes 0
     ax = __builtin_fma(s, dx, ax);
     ay = __builtin_fma(s, dy, ay);
@@ -69,7 +69,7 @@ __device__ __forceinline__ void pair_fp64(double xi, double yi, double zi, doubl
 // Newton-Raphson step applied to the cube, y^3 (1 + 3/2 (1 - r2 y^2))
 // = c y^3 (F - r2 y^2) (F = kNewtonOffset; mj carries c = kNewtonMass),
 // FP32 pair arithmetic.
-template <bool EPS0, bool NR>
+template <bool EPS0, bool NR, bool MUL = true>
 __device__ __forceinline__ void pair_mixed(double xi, double yi, double zi, double xj, double yj,
                                            double zj, float mj, float eps2, float& ax, float& ay,
                                            float& az) {
@@ -77,7 +77,7 @@ __device__ __forceinline__ void pair_mixed(double xi, double yi, double zi, doub
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
     const float y = __builtin_amdgcn_rsqf(r2);
     const float y2 = y * y;
-    float s = (mj * y) * y2;
+    float s = (MUL ? mj * y : y) * y2;
     if (NR) s = s * __builtin_fmaf(-r2, y2, kNewtonOffset);
     if (EPS0) s = (r2 > 0.0f) ? s : 0.0f;
     ax = __builtin_fmaf(s, dx, ax);
@@ -87,13 +87,14 @@ __device__ __forceinline__ void pair_mixed(double xi, double yi, double zi, doub
 
 // FP32 throughout (FORM_FP32): FP32 positions and differences, the mixed
 // form's rsqrt and Newton step, FP32 sums.
+template <bool MUL = true>
 __device__ __forceinline__ void pair_fp32(float xi, float yi, float zi, float xj, float yj, float zj, float mj,
                                           float eps2, bool eps0, float& ax, float& ay, float& az) {
     const float dx = xj - xi, dy = yj - yi, dz = zj - zi;
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
     const float y = __builtin_amdgcn_rsqf(r2);
     const float y2 = y * y;
-    float s = (mj * y) * y2 * __builtin_fmaf(-r2, y2, kNewtonOffset);
+    float s = (MUL ? mj * y : y) * y2 * __builtin_fmaf(-r2, y2, kNewtonOffset);
     if (eps0) s = (r2 > 0.0f) ? s : 0.0f;
     ax = __builtin_fmaf(s, dx, ax);
     ay = __builtin_fmaf(s, dy, ay);
@@ -110,16 +111,21 @@ struct __attribute__((aligned(16))) P4f {
 // ---------------------------------------------------------------------------
 // One work item: rows [ibase, ibase + BLOCK IPT) of pi (those below ni)
 // against pj[jbeg, jend), written to o[i].  Every thread of the block takes
-// part (the j-tiles are staged through LDS with barriers).
+// part (the j-tiles are staged through LDS with barriers).  eq > 0 (every j
+// mass equals eq): a staged mass is 1 for a particle and 0 for padding, the
+// full tiles take the pair terms without it, and eq times the form's mass
+// factor joins G in the final scale.
 template <int FORM, bool EPS0, int IPT>
 __device__ __forceinline__ void accel_item(const P4* __restrict__ pi, int ni, const P4* __restrict__ pj, int jbeg,
                                            int jend, double eps2, double G, double cshift, double coff,
-                                           P4* __restrict__ o, int ibase) {
+                                           P4* __restrict__ o, int ibase, double eq = 0.0) {
     constexpr bool F32 = (FORM == FORM_FP32);
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR) || F32;
     // the factor of the correction polynomial carried by the staged masses
     constexpr double MSCALE = MIXED ? (NR ? kNewtonMass : 1.0) : kCubicMass;
+    const bool eqm = eq > 0.0;
+    G = eqm ? G * (eq * MSCALE) : G;
     using Rec = typename std::conditional<F32, P4f, typename std::conditional<MIXED, P4m, P4>::type>::type;
     __shared__ Rec tile[2][BLOCK];
 
@@ -146,21 +152,9 @@ __device__ __forceinline__ void accel_item(const P4* __restrict__ pi, int ni, co
         gx[k] = gy[k] = gz[k] = 0.0f;
     }
 
-    for (int tt = 0; tt < ntiles; tt++) {
-        const int buf = tt & 1;
-        if constexpr (F32) {
-            tile[buf][t] = P4f{(float)nxt.x, (float)nxt.y, (float)nxt.z, (float)(nxt.m * MSCALE)};
-        } else if constexpr (MIXED) {
-            tile[buf][t] = P4m{nxt.x, nxt.y, nxt.z, (float)(nxt.m * MSCALE), 0.0f};
-        } else {
-            tile[buf][t] = P4{nxt.x, nxt.y, nxt.z, nxt.m * MSCALE};
-        }
-        __syncthreads();
-        const int j0 = jbeg + tt * BLOCK;
-        if (tt + 1 < ntiles) nxt = load_p4(pj, j0 + BLOCK + t, jend);
-        const int cnt = min(BLOCK, jend - j0);
-        const int cnt8 = (cnt + 7) & ~7;   // entries past jend are zero-mass records
-        const Rec* tl = tile[buf];
+    // the pair loop over one staged tile: MUL false leaves the masses out
+    auto sweep = [&](const Rec* tl, int cnt8, auto mul) {
+        constexpr bool MUL = decltype(mul)::value;
         if constexpr (F32) {
             for (int jj = 0; jj < cnt8; jj += 8) {
 #pragma unroll
@@ -168,7 +162,7 @@ __device__ __forceinline__ void accel_item(const P4* __restrict__ pi, int ni, co
                     const P4f q = tl[jj + u];
 #pragma unroll
                     for (int k = 0; k < IPT; k++)
-                        pair_fp32(xs[k], ys[k], zs[k], q.x, q.y, q.z, q.m, eps2f, EPS0, gx[k], gy[k], gz[k]);
+                        pair_fp32<MUL>(xs[k], ys[k], zs[k], q.x, q.y, q.z, q.m, eps2f, EPS0, gx[k], gy[k], gz[k]);
                 }
             }
         } else if constexpr (MIXED) {
@@ -181,8 +175,8 @@ __device__ __forceinline__ void accel_item(const P4* __restrict__ pi, int ni, co
                     const P4m q = tl[jj + u];
 #pragma unroll
                     for (int k = 0; k < IPT; k++)
-                        pair_mixed<EPS0, NR>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2f,
-                                             fx[k], fy[k], fz[k]);
+                        pair_mixed<EPS0, NR, MUL>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2f,
+                                                  fx[k], fy[k], fz[k]);
                 }
             }
 #pragma unroll
@@ -198,11 +192,30 @@ __device__ __forceinline__ void accel_item(const P4* __restrict__ pi, int ni, co
                     const P4 q = tl[jj + u];
 #pragma unroll
                     for (int k = 0; k < IPT; k++)
-                        pair_fp64<EPS0>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2, cshift, coff,
-                                        ax[k], ay[k], az[k]);
+                        pair_fp64<EPS0, MUL>(xi[k], yi[k], zi[k], q.x, q.y, q.z, q.m, eps2, cshift, coff,
+                                             ax[k], ay[k], az[k]);
                 }
             }
         }
+    };
+
+    for (int tt = 0; tt < ntiles; tt++) {
+        const int buf = tt & 1;
+        const double mj = eqm ? (nxt.m != 0.0 ? 1.0 : 0.0) : nxt.m * MSCALE;
+        if constexpr (F32) {
+            tile[buf][t] = P4f{(float)nxt.x, (float)nxt.y, (float)nxt.z, (float)mj};
+        } else if constexpr (MIXED) {
+            tile[buf][t] = P4m{nxt.x, nxt.y, nxt.z, (float)mj, 0.0f};
+        } else {
+            tile[buf][t] = P4{nxt.x, nxt.y, nxt.z, mj};
+        }
+        __syncthreads();
+        const int j0 = jbeg + tt * BLOCK;
+        if (tt + 1 < ntiles) nxt = load_p4(pj, j0 + BLOCK + t, jend);
+        const int cnt = min(BLOCK, jend - j0);
+        const int cnt8 = (cnt + 7) & ~7;   // entries past jend are zero-mass records
+        if (eqm && cnt == BLOCK) sweep(tile[buf], cnt8, std::integral_constant<bool, false>{});
+        else sweep(tile[buf], cnt8, std::integral_constant<bool, true>{});
     }
 
     if constexpr (F32) {
@@ -218,17 +231,45 @@ __device__ __forceinline__ void accel_item(const P4* __restrict__ pi, int ni, co
     }
 }
 
+// eq_dev (if set) says on the device whether every j mass is equal
+// (k_mass_check); else eq_host does (0: not known equal).
 template <int FORM, bool EPS0, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_accel(const P4* __restrict__ pi, int ni,
                                                  const P4* __restrict__ pj, int jbeg_all,
                                                  int jend_all, int jchunk, double eps2, double G,
                                                  double cshift, double coff, P4* __restrict__ out,
-                                                 long out_stride, int split_base) {
+                                                 long out_stride, int split_base, double eq_host,
+                                                 const EqState* __restrict__ eq_dev) {
     const int split = blockIdx.y;
     const int jbeg = jbeg_all + split * jchunk;
     const int jend = min(jend_all, jbeg + jchunk);
+    const double eq = eq_dev ? (eq_dev->ok ? eq_dev->m0 : 0.0) : eq_host;
     accel_item<FORM, EPS0, IPT>(pi, ni, pj, jbeg, jend, eps2, G, cshift, coff,
-                                out + (long)(split_base + split) * out_stride, blockIdx.x * BLOCK * IPT);
+                                out + (long)(split_base + split) * out_stride, blockIdx.x * BLOCK * IPT, eq);
+}
+
+// Whether every mass of pj[0, nj) equals the first (> 0, finite): the
+// first kernel takes it as the candidate, the second clears `ok` at any
+// other mass (plain stores of one value; no atomics needed).
+__global__ void k_mass_check_init(const P4* __restrict__ pj, EqState* __restrict__ st) {
+    const double m0 = pj[0].m;
+    st->m0 = m0;
+    st->ok = (m0 > 0.0 && m0 < INFINITY) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_mass_check(const P4* __restrict__ pj, int nj, EqState* __restrict__ st) {
+    const double m0 = st->m0;
+    bool same = true;
+    for (int j = blockIdx.x * BLOCK + threadIdx.x; j < nj; j += gridDim.x * BLOCK) same = same && pj[j].m == m0;
+    if (!same) st->ok = 0;
+}
+
+hipError_t launch_mass_check(const P4* pj, int nj, EqState* st, hipStream_t stream) {
+    if (nj <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mass_check_init, dim3(1), dim3(1), 0, stream, pj, st);
+    hipLaunchKernelGGL(k_mass_check, dim3(std::min(1024, (nj + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, pj, nj,
+                       st);
+    return hipGetLastError();
 }
 
 // Rows [0, m) of pi, m = min(*count, ni_max) read on the device, against all
@@ -565,9 +606,9 @@ __global__ __launch_bounds__(BLOCK) void k_energy_reduce(const DD* __restrict__ 
 template <int FORM, bool EPS0, int IPT>
 static hipError_t launch_accel_t(const AccelLaunch& L, hipStream_t st) {
     const dim3 grid((L.ni + BLOCK * IPT - 1) / (BLOCK * IPT), L.splits);
-    hipLaunchKernelGGL((k_accel<FORM, EPS0, IPT>), grid, dim3(BLOCK), 0, st, L.pi, L.ni, L.pj,
-                       L.jbeg, L.jend, L.jchunk, L.eps2, L.G, kCubicShift, kCubicOffset, L.out, L.out_stride,
-                       L.split_base);
+    hipLaunchKernelGGL((k_accel<FORM, EPS0, IPT>), grid, dim3(BLOCK), 0, st, L.pi, L.ni, L.pj, L.jbeg, L.jend,
+                       L.jchunk, L.eps2, L.G, kCubicShift, kCubicOffset, L.out, L.out_stride, L.split_base, L.eq_mass,
+                       L.eq_dev);
     return hipGetLastError();
 }
 

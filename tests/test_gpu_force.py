@@ -380,3 +380,29 @@ def test_equal_mass_sums_match_the_general_path(solver, orc, n, form):
     pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
     ref = orc.accel_block(pj[rows], pj, 0.005).T
     assert rel_err(a_eq[:, rows], ref) < tol
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed", "fp32"])
+def test_tile_kernel_equal_masses(solver, orc, form):
+    """The tile kernel with every j mass equal leaves the mass out of the
+    pair terms (full j-tiles) and puts it into the final scale: against the
+    same j set with one mass changed by 2^-50 (the general path) and the
+    oracle; j-blocks with a partial last tile, the resident system too."""
+    tol = {"fp64": FP64_TOL, "mixed": MIXED_TOL, "fp32": 1e-3}[form]
+    m, pos, _, _ = nbody_amd.init_system("plummer", 5000, 14, [1.0, 20.0])
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    pi = pj[::7].copy()
+    ref = orc.accel_block(pi, pj, 0.005).T
+    a_eq = solver.accel_block(pi, pj, form, 0.005, j_block=1300).T
+    pj2 = pj.copy()
+    pj2[2500, 3] *= 1.0 + 2.0 ** -50
+    a_gen = solver.accel_block(pi, pj2, form, 0.005, j_block=1300).T
+    assert rel_err(a_eq, ref) < tol and rel_err(a_gen, ref) < tol
+    assert rel_err(a_eq, a_gen) < tol
+    solver.set_kernel("tile")
+    try:
+        solver.set_state(m, pos)
+        a_res = solver.accel(form, 0.005)
+    finally:
+        solver.set_kernel("auto")
+    assert rel_err(a_res[:, ::7], ref) < tol
