@@ -41,7 +41,15 @@
 // from a sorted cell grid, and only the smooth remainder a - a_near is
 // extrapolated (the history holds it), which removes the close-encounter
 // noise the extrapolation cannot follow.
+// When an extrapolated substep had to sum more than a share of the rows
+// exactly (rough_share, a quarter by default: such rows cost about what the
+// direct sum does), the next substeps run the direct sum for 8 periods,
+// doubling for each further time in a row (up to 64).
 // Mode 1 drops the network (f = 0): the method without it.
+// No substep waits for its own work on the host: the flagged rows' count
+// stays on the device (their kernels size themselves from it), a check is
+// decided on the device, and the host takes both from pinned copies one
+// substep later, after queueing that substep's near field.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
