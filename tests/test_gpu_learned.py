@@ -201,13 +201,15 @@ def restated_learned(orc, m, pos, vel, steps, dt, eps, period, threshold, rho, p
     return pos, vel, st
 
 
-def test_rough_guard_runs_the_direct_sum(solver):
+@pytest.mark.parametrize("rc", [0.0, 0.3])
+def test_rough_guard_runs_the_direct_sum(solver, rc):
     """More than N/4 exact rows on an extrapolated substep (here nearly every
     particle is held after the first check: kappa = 1e-12): the next substep
     runs the direct sum instead, and so do the next 8 periods' substeps, then
     16 periods' the second time in a row.  Substep 4 checks; 5 and 39
     extrapolate (nearly every row summed exactly); 6 and 40 are the guard's;
-    the rest are exact."""
+    the rest are exact.  With a near-field split (rc = 0.3) the held
+    substeps that cannot reach the history skip the near field."""
     m, pos, vel, _ = nbody_amd.init_system("plummer", 2000, 9)
     steps, dt, eps = 80, 1.0 / 256, 0.01
     solver.set_state(m, pos, vel)
@@ -215,7 +217,7 @@ def test_rough_guard_runs_the_direct_sum(solver):
     p_exact, v_exact = solver.get_state()
     solver.learned_configure(period=4, threshold=1e-2, exact_form="fp64")
     solver.learned_set_mode(1)
-    solver.learned_set_near_radius(0.0)
+    solver.learned_set_near_radius(rc)
     solver.learned_set_flag_ratio(0.1)
     solver.learned_set_hold_ratio(1e-12)
     try:
@@ -225,6 +227,7 @@ def test_rough_guard_runs_the_direct_sum(solver):
         st = solver.learned_stats()
     finally:
         solver.learned_set_hold_ratio(1e-3)
+        solver.learned_set_near_radius(0.0)
     assert st["checks"] == 1 and st["fallbacks"] == 0
     assert st["rough_exact"] == 2 and st["predicted_evals"] == 2 and st["exact_evals"] == steps - 2
     # (all but the far, smoothest particles miss by more than 1e-12 a_rms)
