@@ -133,7 +133,9 @@ NB_API nb_status nb_accel_block(nb_ctx* ctx, int form, double eps, size_t ni, co
 
 /* As nb_accel_block with DEVICE pointers: d_pi (ni x 4), d_pj (nj x 4) and
  * d_acc4 (ni x 4 doubles: ax, ay, az, 0) in HBM, work queued on the context's
- * stream without synchronising.  For inputs already resident on the GPU. */
+ * stream without synchronising.  For inputs already resident on the GPU.
+ * Whether every j mass is equal (then the pair terms leave it out) is found
+ * by two small kernels on the same stream, not on the host. */
 NB_API nb_status nb_accel_block_device(nb_ctx* ctx, int form, double eps, size_t ni, const double* d_pi,
                                 size_t nj, const double* d_pj, size_t j_block, double* d_acc4);
 
