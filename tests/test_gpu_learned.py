@@ -237,6 +237,35 @@ def test_rough_guard_runs_the_direct_sum(solver, rc):
     assert np.max(np.abs(v - v_exact)) < 1e-10 * np.max(np.abs(v_exact))
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 65])
+@pytest.mark.parametrize("rc", [0.0, 0.3])
+def test_learned_tiny_systems(solver, orc, n, rc):
+    """Systems smaller than a wave: the schedule, the device-sized exact rows
+    and the near field still follow the restatement (one particle: nothing to
+    check, every substep exact)."""
+    m, pos, vel, _ = nbody_amd.init_system("plummer", max(n, 2), 5)
+    m, pos, vel = m[:n] * 0 + 1.0 / n, np.ascontiguousarray(pos[:, :n]), np.ascontiguousarray(vel[:, :n])
+    steps, dt, eps = 24, 1.0 / 256, 0.05
+    solver.learned_configure(period=4, threshold=1e-2, exact_form="fp64")
+    solver.learned_set_mode(1)
+    solver.learned_set_flag_ratio(0.1)
+    solver.learned_set_near_radius(rc)
+    try:
+        solver.set_state(m, pos, vel)
+        solver.step(steps, dt, 2, "learned", eps)
+        p, v = solver.get_state()
+        st = solver.learned_stats()
+    finally:
+        solver.learned_set_near_radius(0.0)
+    pr, vr, sr = restated_learned(orc, m, pos, vel, steps, dt, eps, 4, 1e-2, 0.1, rc=rc)
+    assert st["exact_evals"] == sr["exact"] and st["predicted_evals"] == sr["predicted"]
+    assert st["subset_rows"] == sr["subset"] and st["rough_exact"] == sr["rough"]
+    assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
+    assert np.max(np.abs(p - pr)) < 1e-11 * max(1.0, np.max(np.abs(pr)))
+    if n == 1:
+        assert st["predicted_evals"] == 0
+
+
 def test_threshold_zero_is_all_exact_bitwise(solver):
     m, pos, vel, _ = nbody_amd.init_system("plummer", 2000, 9)
     solver.set_state(m, pos, vel)
