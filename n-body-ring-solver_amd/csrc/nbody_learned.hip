@@ -645,12 +645,14 @@ inline int4 ring_slots(const Learned& L) {
 }
 
 // Extrapolated accelerations (remainders, with the split) of every particle
-// for the current substep into `out` (the cubic alone stays in L.cub); marks
-// and lists the particles to be summed exactly.  The network is applied when
-// `allow_net`, it is on and trained, and the history is evenly spaced (noted
-// in L.pred_net).  With `count`, queues the copy of how many are marked to
-// pinned host memory; take_flag_count() takes it at the next substep.
-nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
+// for the current substep, in two parts so that work needing only the flags
+// can run between them.  predict_features: the features, the cubic (in
+// L.cub), the particles to be summed exactly (marked and listed); with
+// `count`, queues the copy of how many are marked to pinned host memory
+// (take_flag_count() takes it at the next substep).  predict_finish: the
+// network's correction when `allow_net`, the network is on and trained and
+// the history is evenly spaced (noted in L.pred_net), combined into `out`.
+nb_status predict_features(nb_ctx* c, Learned& L, bool count) {
     hipStream_t st = nbh::ctx_stream(c);
     const long n = (long)L.n;
     const Weights W = weights(L, L.clock);
@@ -660,13 +662,6 @@ nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
                        (P4*)L.cub.p, (float4*)L.frame.p, L.rho, (unsigned char*)L.mask.p, (int*)L.flags.p,
                        (int*)L.nflags.p, (const unsigned char*)L.held.p);
     LH(hipGetLastError(), "features kernel");
-    const bool use_net = allow_net && L.mode == 0 && L.trained && W.wnet != 0.0;
-    L.pred_net = use_net;
-    if (use_net) LH(mlp::forward(L.net, (const float*)L.X.p, (int)n, (float*)L.Y.p, false, nullptr, nullptr, 0, st), "network forward");
-    hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float4*)L.frame.p,
-                       (const float*)L.Y.p, (const P4*)nbh::ctx_vel(c), (float)(L.tau * W.wnet), use_net ? 1 : 0, n,
-                       out);
-    LH(hipGetLastError(), "combine kernel");
     if (count) {
         LH(hipMemcpyAsync(L.nflag_h, L.nflags.p, sizeof(int), hipMemcpyDeviceToHost, st), "download flag count");
         LH(hipEventRecord(L.nflag_ev, st), "record event");
@@ -675,7 +670,21 @@ nb_status predict(nb_ctx* c, Learned& L, P4* out, bool count, bool allow_net) {
     return NB_OK;
 }
 
-// The flag count of the last predict(count = true), if not yet taken: into
+nb_status predict_finish(nb_ctx* c, Learned& L, P4* out, bool allow_net) {
+    hipStream_t st = nbh::ctx_stream(c);
+    const long n = (long)L.n;
+    const Weights W = weights(L, L.clock);
+    const bool use_net = allow_net && L.mode == 0 && L.trained && W.wnet != 0.0;
+    L.pred_net = use_net;
+    if (use_net) LH(mlp::forward(L.net, (const float*)L.X.p, (int)n, (float*)L.Y.p, false, nullptr, nullptr, 0, st), "network forward");
+    hipLaunchKernelGGL(k_combine, blocks(n), dim3(BLK), 0, st, (const P4*)L.cub.p, (const float4*)L.frame.p,
+                       (const float*)L.Y.p, (const P4*)nbh::ctx_vel(c), (float)(L.tau * W.wnet), use_net ? 1 : 0, n,
+                       out);
+    LH(hipGetLastError(), "combine kernel");
+    return NB_OK;
+}
+
+// The flag count of the last predict_features(count = true), if not yet taken: into
 // L.prev_rough and the stats.  Taken at the top of the next substep, when the
 // device has long passed the copy (the work queued behind it keeps it busy).
 nb_status take_flag_count(Learned& L) {
@@ -991,7 +1000,8 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
             if (exact) {
                 const bool checking = full && !held_exact;
                 if (checking) {
-                    if (nb_status e = predict(c, L, (P4*)L.pred.p, false, true)) return e;
+                    if (nb_status e = predict_features(c, L, false)) return e;
+                    if (nb_status e = predict_finish(c, L, (P4*)L.pred.p, true)) return e;
                 }
                 if (quiet) L.grid_age = 1 << 20;   // the next near field sorts a new grid
                 if (nb_status e = exact_force_kick(c, L, L.exact_kform, eps, q)) return e;
@@ -1020,14 +1030,13 @@ nb_status step_learned(nb_ctx* c, size_t time_steps, double dt, const double* cc
                 L.rough_streak++;
             } else {
                 // the near field runs on the side stream while the main
-                // stream extrapolates and sums the flagged particles exactly
-                // (their count stays on the device)
-                if (L.rc > 0) {
-                    if (nb_status e = predict(c, L, (P4*)L.pred.p, true, L.net_ok)) return e;
-                } else {
-                    if (nb_status e = predict(c, L, nbh::ctx_acc(c), true, L.net_ok)) return e;
-                }
+                // stream sums the flagged particles exactly (their count
+                // stays on the device) and extrapolates the others -- the
+                // exact rows first: they need only the flags, and so run
+                // beside the near field rather than after the network
+                if (nb_status e = predict_features(c, L, true)) return e;
                 if (nb_status e = exact_subset_compute(c, L, eps, L.exact_kform)) return e;
+                if (nb_status e = predict_finish(c, L, L.rc > 0 ? (P4*)L.pred.p : nbh::ctx_acc(c), L.net_ok)) return e;
                 if (L.rc > 0) {
                     if (nb_status e = near_field_join(c, L)) return e;
                     hipLaunchKernelGGL(k_add, blocks((long)n), dim3(BLK), 0, nbh::ctx_stream(c), (const P4*)L.pred.p,
