@@ -262,30 +262,45 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(Net net, const float* __restric
 }
 
 // ---------------------------------------------------------------------------
-// dW_l = delta_l^T H_{l-1}, K = ld (the padded batch): one wave per 16x16
-// output tile.  Tiles: layer 1 (8 x 1), layer 2 (8 x 8), layer 3 (8 x 8),
-// layer 4 (1 x 8).  Writes fp32 into the packed gradient (W row-major
-// [out][in], W1 with in = 15, W4 with out = 3).
+// dW_l = delta_l^T H_{l-1}, K = ld (the padded batch): one workgroup of
+// kDwWaves waves per 16x16 output tile.  Tiles: layer 1 (8 x 1), layer 2
+// (8 x 8), layer 3 (8 x 8), layer 4 (1 x 8).  Wave w takes the batch steps
+// k = 16 (w + kDwWaves i) and wave 0 adds the waves' accumulators in wave
+// order (the batch split so that its sum, 256 MFMA steps for a batch of
+// 4096, is not one latency-bound chain).  Writes fp32 into the packed
+// gradient (W row-major [out][in], W1 with in = 15, W4 with out = 3).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mlp_dw(const unsigned short* __restrict__ H, const unsigned short* __restrict__ D,
-                                                long ld, float* __restrict__ grad) {
-    const int lane = threadIdx.x & 63;
+constexpr int kDwWaves = 8;
+
+__global__ __launch_bounds__(64 * kDwWaves) void k_mlp_dw(const unsigned short* __restrict__ H,
+                                                          const unsigned short* __restrict__ D, long ld,
+                                                          float* __restrict__ grad) {
+    __shared__ f32x4 part[kDwWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int t = blockIdx.x;   // 0 .. TILES - 1
     // layer, tile (mt, nt), operand bases
     int layer_id, mt, nt;
     if (t < 8) { layer_id = 0; mt = t; nt = 0; }
     else if (t < 72) { layer_id = 1; mt = (t - 8) / 8; nt = (t - 8) % 8; }
     else if (t < 136) { layer_id = 2; mt = (t - 72) / 8; nt = (t - 72) % 8; }
-    else if (t < 144) { layer_id = 3; mt = 0; nt = t - 136; }
-    else return;
+    else { layer_id = 3; mt = 0; nt = t - 136; }
     // delta of layer l: D layout [D4 16 rows][D3 128][D2 128][D1 128]
     const long drow = (layer_id == 3) ? 0 : 16 + (long)(2 - layer_id) * 128;
     const long hrow = (layer_id == 0) ? 0 : 16 + (long)(layer_id - 1) * 128;
     const unsigned short* A = D + (drow + 16 * mt + r) * ld;   // A[m][k] = delta[k][out]
     const unsigned short* B = H + (hrow + 16 * nt + r) * ld;   // B[k][n] = H[k][in]
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (long k = 0; k < ld; k += 16) acc = mfma(load4(A + k + 4 * g), load4(B + k + 4 * g), acc);
+    for (long k = 16L * w; k < ld; k += 16L * kDwWaves) acc = mfma(load4(A + k + 4 * g), load4(B + k + 4 * g), acc);
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w != 0) return;
+    acc = part[0][lane];
+    for (int q = 1; q < kDwWaves; q++) {
+        const f32x4 v = part[q][lane];
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] += v[j];
+    }
     const int in_dim = (layer_id == 0) ? 15 : 128;
     const int out_dim = (layer_id == 3) ? 3 : 128;
     const long off = (layer_id == 0) ? 0 : (layer_id == 1) ? OFF_W2 : (layer_id == 2) ? OFF_W3 : OFF_W4;
@@ -297,23 +312,34 @@ __global__ __launch_bounds__(256) void k_mlp_dw(const unsigned short* __restrict
     }
 }
 
-// Bias gradients and loss: sums over tiles in tile order.
-__global__ __launch_bounds__(256) void k_mlp_db(const float* __restrict__ dbpart, const float* __restrict__ losspart,
-                                                int tiles, float* __restrict__ grad, float* __restrict__ loss) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < 16 + 3 * 128) {
-        float s = 0.f;
-        for (int t = 0; t < tiles; t++) s += dbpart[(long)t * (16 + 3 * 128) + i];
-        // dbpart rows: [0,16) output, [16,144) layer 1, [144,272) layer 2, [272,400) layer 3
-        if (i < 3) grad[OFF_B4 + i] = s;
-        else if (i >= 16 && i < 144) grad[OFF_B1 + (i - 16)] = s;
-        else if (i >= 144 && i < 272) grad[OFF_B2 + (i - 144)] = s;
-        else if (i >= 272) grad[OFF_B3 + (i - 272)] = s;
-    } else if (i == 16 + 3 * 128) {
-        float s = 0.f;
-        for (int t = 0; t < tiles; t++) s += losspart[t];
-        *loss = s;
-    }
+// Bias gradients and loss: sums over the batch's tiles.  A workgroup of
+// kDbWaves waves per 64 columns (column 400: the loss): wave w takes tiles
+// w, w + kDbWaves, ..., and wave 0 adds the waves' sums in wave order.
+constexpr int kDbWaves = 16;
+constexpr int kDbCols = 16 + 3 * 128 + 1;   // dbpart rows, then the loss
+
+__global__ __launch_bounds__(64 * kDbWaves) void k_mlp_db(const float* __restrict__ dbpart,
+                                                          const float* __restrict__ losspart, int tiles,
+                                                          float* __restrict__ grad, float* __restrict__ loss) {
+    __shared__ float part[kDbWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    float s = 0.f;
+    if (i < kDbCols - 1)
+        for (int t = w; t < tiles; t += kDbWaves) s += dbpart[(long)t * (16 + 3 * 128) + i];
+    else if (i == kDbCols - 1)
+        for (int t = w; t < tiles; t += kDbWaves) s += losspart[t];
+    part[w][lane] = s;
+    __syncthreads();
+    if (w != 0 || i >= kDbCols) return;
+    s = part[0][lane];
+    for (int q = 1; q < kDbWaves; q++) s += part[q][lane];
+    // dbpart rows: [0,16) output, [16,144) layer 1, [144,272) layer 2, [272,400) layer 3
+    if (i < 3) grad[OFF_B4 + i] = s;
+    else if (i >= 16 && i < 144) grad[OFF_B1 + (i - 16)] = s;
+    else if (i >= 144 && i < 272) grad[OFF_B2 + (i - 144)] = s;
+    else if (i >= 272 && i < 400) grad[OFF_B3 + (i - 272)] = s;
+    else if (i == 400) *loss = s;
 }
 
 // Adam (Kingma & Ba 2015) with bias correction on the FP32 master weights.
@@ -411,10 +437,11 @@ hipError_t backward(const Net& net, const float* Y, const float* T, int rows, co
     hipLaunchKernelGGL(k_mlp_bwd, dim3(grid_for(rows)), dim3(256), 0, st, net, Y, T, rows, Z, D, ld, dbpart, losspart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp_dw, dim3(36), dim3(256), 0, st, H, (const unsigned short*)D, ld, grad);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(TILES), dim3(64 * kDwWaves), 0, st, H, (const unsigned short*)D, ld, grad);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp_db, dim3(2), dim3(256), 0, st, dbpart, losspart, (rows + 15) / 16, grad, loss);
+    hipLaunchKernelGGL(k_mlp_db, dim3((kDbCols + 63) / 64), dim3(64 * kDbWaves), 0, st, dbpart, losspart,
+                       (rows + 15) / 16, grad, loss);
     return hipGetLastError();
 }
 
