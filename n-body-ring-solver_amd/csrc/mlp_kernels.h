@@ -20,11 +20,21 @@ constexpr long NPARAM = OFF_B4 + NOUT;
 
 // Device copies the kernels read: bf16 weights (padded, plus transposes for
 // the backward sweep) and fp32 biases.
-// MFMA operand tiles of the forward weights: tile t holds, for lane
-// l = r + 16 g, the 4 bf16 A-operand values W[16 m + r][16 kt + 4 g + j] of
-// one 16 x 16 block (m, kt); tiles 0..7 are W1 (m), 8..71 W2 (8 + 8 m + kt),
-// 72..135 W3, 136..143 W4 (kt).  Read linearly, 8 bytes per lane.
-constexpr int TILES = 144, T_W1 = 0, T_W2 = 8, T_W3 = 72, T_W4 = 136;
+// MFMA operand tiles of the forward weights (Wtile, bf16), read linearly by
+// the inference kernel:
+//  - W1: 8 tiles of 64 lanes x 4 values (16 x 16 blocks, K = 16): tile m
+//    holds W1[16 m + r][4 g + j] at lane l = r + 16 g, element j;
+//  - W2, W3, W4: 68 tiles of 64 lanes x 8 values (16 x 32 blocks, K = 32)
+//    after the W1 tiles: tile u = 4 m + kb (W2, u < 32; W3 at 32 + 4 m + kb;
+//    W4 at 64 + kb) holds, at lane r + 16 g, element e, the weight of output
+//    16 m + r and input 32 kb + 4 g + e (e < 4) or 32 kb + 16 + 4 g + e - 4
+//    (e >= 4): the two 16-feature blocks 2 kb, 2 kb + 1 that one lane holds of
+//    the layer's input, in that order.
+// TILES counts 16 x 16 blocks (W1 8, W2 64, W3 64, W4 8): also the weight
+// gradient's tiles.
+constexpr int TILES = 144;
+constexpr int W1_TILES = 8, WIDE_TILES = 68, U_W2 = 0, U_W3 = 32, U_W4 = 64;
+static_assert(W1_TILES * 64 * 4 + WIDE_TILES * 64 * 8 == TILES * 64 * 4, "tile storage");
 
 struct Net {
     const unsigned short *W1, *W2, *W3, *W4;    // [128][16], [128][128] x2, [16][128]

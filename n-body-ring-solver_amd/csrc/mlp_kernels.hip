@@ -2,14 +2,16 @@
 // extrapolator: a 15 -> 128 -> 128 -> 128 -> 3 MLP with SiLU hidden
 // activations, FP32 accumulation, trained with Adam on FP32 master weights.
 //
-// Every matrix product runs on v_mfma_f32_16x16x16_bf16 in "transposed" form:
-// a layer computes D^T = W X^T with the weights as the A operand (rows =
-// output features) and 16 particles as the columns of the B operand.  The
-// accumulator layout of one layer -- lane (r, g), r = lane % 16, g = lane / 16,
-// holds features 4g..4g+3 of particle r -- is exactly the B-operand layout the
-// next layer needs, so activations never leave registers inside the forward or
-// the backward sweep.  Weight gradients dW = delta^T H are a GEMM over the
-// batch; the sweeps store delta and H feature-major (bf16) for it.
+// Every matrix product runs on MFMA (v_mfma_f32_16x16x16_bf16; the
+// inference kernel's layers 2-4 on gfx950's v_mfma_f32_16x16x32_bf16) in
+// "transposed" form: a layer computes D^T = W X^T with the weights as the A
+// operand (rows = output features) and 16 particles as the columns of the B
+// operand.  The accumulator layout of one layer -- lane (r, g), r = lane % 16,
+// g = lane / 16, holds features 4g..4g+3 of particle r -- is exactly the
+// B-operand layout the next layer needs, so activations never leave registers
+// inside the forward or the backward sweep.  Weight gradients dW = delta^T H
+// are a GEMM over the batch; the sweeps store delta and H feature-major (bf16)
+// for it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -120,28 +122,39 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(Net net, const float* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Inference forward (no saved activations): the same arithmetic as k_mlp_fwd,
-// with every weight tile and bias staged once per workgroup in LDS (72 KiB +
-// 1.6 KiB, two workgroups per CU) and read back linearly, 8 bytes per lane,
-// at LDS latency, instead of from memory for every MFMA (144 dependent-latency
-// loads per 16-particle tile at one wave per SIMD).  An empty asm with a
-// memory clobber at the top of the tile loop keeps the compiler from hoisting
-// all 144 tiles into registers (which would cost the occupancy).  Workgroups
-// of kInferWaves = 6 waves: two per CU fill the three waves per SIMD that the
-// kernel's 132 registers allow.
+// Inference forward (no saved activations): the arithmetic of k_mlp_fwd (bf16
+// operands, FP32 accumulation) in fewer, wider instructions.  Layers 2-4 run
+// on gfx950's v_mfma_f32_16x16x32_bf16: K = 32 per instruction, the lane's
+// two 4-feature pieces of blocks 2 kb and 2 kb + 1 of the previous layer's
+// output as one 8-value B operand, the weights stored in that order (Wtile),
+// so activations still never leave registers: 76 MFMAs per 16-particle tile
+// instead of 144.  Each accumulator starts from the layer's bias, so the
+// bias costs no VALU instruction.  Every weight tile and bias is staged once
+// per workgroup in LDS (72 KiB + 1.6 KiB, two workgroups per CU) and read
+// back linearly, 8 or 16 bytes per lane.  An empty asm with a memory clobber
+// at the top of the tile loop keeps the compiler from hoisting the weight
+// tiles into registers (which would cost the occupancy).
 // ---------------------------------------------------------------------------
-constexpr int kInferWaves = 6;
+constexpr int kInferWaves = 8;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
 
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(Net net, const float* __restrict__ X, int rows,
                                                                 float* __restrict__ Y) {
-    __shared__ uint2 wt[TILES * 64];
-    __shared__ float bias[3 * 128 + 16];
+    __shared__ uint4 wt[TILES * 32];
+    __shared__ f32x4 bias[(3 * 128 + 16) / 4];
     {
         const uint4* src = (const uint4*)net.Wtile;
-        uint4* dst = (uint4*)wt;
-        for (int k = threadIdx.x; k < TILES * 32; k += 64 * kInferWaves) dst[k] = src[k];
+        for (int k = threadIdx.x; k < TILES * 32; k += 64 * kInferWaves) wt[k] = src[k];
+        float* b = (float*)bias;
         for (int k = threadIdx.x; k < 3 * 128 + 16; k += 64 * kInferWaves)
-            bias[k] = k < 128 ? net.b1[k] : k < 256 ? net.b2[k - 128] : k < 384 ? net.b3[k - 256] : net.b4[k - 384];
+            b[k] = k < 128 ? net.b1[k] : k < 256 ? net.b2[k - 128] : k < 384 ? net.b3[k - 256] : net.b4[k - 384];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -149,9 +162,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(Net net, const f
     const int tiles = (rows + 15) / 16;
     const int wave = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * kInferWaves;
+    const uint2* w1 = (const uint2*)wt;                 // W1 tiles: 8 bytes per lane
+    const uint4* wide = wt + W1_TILES * 64 / 2;         // then 16 bytes per lane
     for (int tile = wave; tile < tiles; tile += nwaves) {
         asm volatile("" ::: "memory");
-        auto A = [&](int t) { return __builtin_bit_cast(bf16x4, wt[t * 64 + lane]); };
         const long p = (long)tile * 16 + r;
         const bool valid = p < rows;
         const float4 xin = valid ? *(const float4*)(X + p * 16 + 4 * g) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -159,32 +173,30 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(Net net, const f
         bf16x4 h[8];
         f32x4 z[8];
 #pragma unroll
-        for (int m = 0; m < 8; m++) z[m] = mfma(A(T_W1 + m), h0, f32x4{0.f, 0.f, 0.f, 0.f});
+        for (int m = 0; m < 8; m++) z[m] = mfma(__builtin_bit_cast(bf16x4, w1[m * 64 + lane]), h0, bias[4 * m + g]);
 #pragma unroll
-        for (int l = 0; l < 3; l++) {   // bias + SiLU of layer l+1, then layer l+2's product
+        for (int l = 0; l < 3; l++) {   // SiLU of layer l+1, then layer l+2's product
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                f32x4 zz = z[m];
-#pragma unroll
-                for (int j = 0; j < 4; j++) zz[j] += bias[128 * l + 16 * m + 4 * g + j];
-                h[m] = pack4(silu(zz[0]), silu(zz[1]), silu(zz[2]), silu(zz[3]));
-            }
+            for (int m = 0; m < 8; m++) h[m] = pack4(silu(z[m][0]), silu(z[m][1]), silu(z[m][2]), silu(z[m][3]));
             if (l < 2) {
-                const int t0 = l == 0 ? T_W2 : T_W3;
+                const int u0 = l == 0 ? U_W2 : U_W3;
 #pragma unroll
                 for (int m = 0; m < 8; m++) {
-                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                    f32x4 acc = bias[32 * (l + 1) + 4 * m + g];
 #pragma unroll
-                    for (int kt = 0; kt < 8; kt++) acc = mfma(A(t0 + 8 * m + kt), h[kt], acc);
+                    for (int kb = 0; kb < 4; kb++)
+                        acc = mfma32(__builtin_bit_cast(s16x8, wide[(u0 + 4 * m + kb) * 64 + lane]),
+                                     __builtin_shufflevector(h[2 * kb], h[2 * kb + 1], 0, 1, 2, 3, 4, 5, 6, 7), acc);
                     z[m] = acc;
                 }
             }
         }
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc = bias[96 + g];   // b4 padded with zeros to 16
 #pragma unroll
-        for (int kt = 0; kt < 8; kt++) acc = mfma(A(T_W4 + kt), h[kt], acc);
-        if (g == 0 && valid)
-            *(float4*)(Y + p * 4) = make_float4(acc[0] + bias[384], acc[1] + bias[385], acc[2] + bias[386], 0.f);
+        for (int kb = 0; kb < 4; kb++)
+            acc = mfma32(__builtin_bit_cast(s16x8, wide[(U_W4 + kb) * 64 + lane]),
+                         __builtin_shufflevector(h[2 * kb], h[2 * kb + 1], 0, 1, 2, 3, 4, 5, 6, 7), acc);
+        if (g == 0 && valid) *(float4*)(Y + p * 4) = make_float4(acc[0], acc[1], acc[2], 0.f);
     }
 }
 
@@ -390,17 +402,18 @@ __global__ __launch_bounds__(256) void k_publish(const float* __restrict__ p, Ne
     }
     if (i < 16) ((float*)net.b4)[i] = (i < 3) ? p[OFF_B4 + i] : 0.f;
     if (i < TILES * 256) {   // MFMA operand tiles (see Net)
-        const int tile = i / 256, w = i % 256, ln = w / 4, j = w % 4, r = ln % 16, g = ln / 16;
         float v;
-        if (tile < T_W2) {
+        if (i < W1_TILES * 256) {
+            const int m = i / 256, w = i % 256, ln = w / 4, j = w % 4, r = ln % 16, g = ln / 16;
             const int col = 4 * g + j;
-            v = col < NIN ? p[OFF_W1 + (16 * tile + r) * NIN + col] : 0.f;
-        } else if (tile < T_W4) {
-            const int t = tile < T_W3 ? tile - T_W2 : tile - T_W3, m = t / 8, kt = t % 8;
-            v = p[(tile < T_W3 ? OFF_W2 : OFF_W3) + (16 * m + r) * NH + 16 * kt + 4 * g + j];
+            v = col < NIN ? p[OFF_W1 + (16 * m + r) * NIN + col] : 0.f;
         } else {
-            const int kt = tile - T_W4;
-            v = r < NOUT ? p[OFF_W4 + r * NH + 16 * kt + 4 * g + j] : 0.f;
+            const int k = i - W1_TILES * 256;
+            const int u = k / 512, w = k % 512, ln = w / 8, e = w % 8, r = ln % 16, g = ln / 16;
+            const int kb = u % 4, m = (u % 32) / 4;
+            const int in = 32 * kb + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+            if (u < U_W4) v = p[(u < U_W3 ? OFF_W2 : OFF_W3) + (16 * m + r) * NH + in];
+            else v = r < NOUT ? p[OFF_W4 + r * NH + in] : 0.f;
         }
         ((unsigned short*)net.Wtile)[i] = to_bf16(v);
     }
