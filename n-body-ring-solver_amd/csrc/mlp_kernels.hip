@@ -63,61 +63,62 @@ __device__ __forceinline__ void store_fm(unsigned short* __restrict__ T, long ld
     for (int j = 0; j < 4; j++) T[(long)(f0 + 4 * g + j) * ld + p] = (unsigned short)v[j];
 }
 
-// One hidden layer: out[m] (8 tiles of 16 features) = W (128 x 16*KT) times in[KT].
-template <int KT>
-__device__ __forceinline__ void layer(const unsigned short* __restrict__ W, const bf16x4* in, f32x4* out, int r,
-                                      int g) {
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kt = 0; kt < KT; kt++) acc = mfma(load4(W + (long)(16 * m + r) * (16 * KT) + 16 * kt + 4 * g), in[kt], acc);
-        out[m] = acc;
-    }
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Forward: Y = f(X) for rows [0, rows); with `save` the pre-activations Z_l
+// Training forward: Y = f(X) for rows [0, rows) with the pre-activations Z_l
 // (row-major [rows][128] fp32) and the layer inputs H_l (feature-major bf16,
-// H0 [16][ld], H1..H3 [128][ld]) are kept for the backward pass.  Rows past
+// H0 [16][ld], H1..H3 [128][ld]) kept for the backward pass.  Rows past
 // `rows` (up to the 16-row tile) read zero inputs and are not written.
+// A workgroup of kTrainWaves = 8 waves takes one 16-particle tile at a time:
+// wave w computes the 16 output features 16 w .. 16 w + 15 of every hidden
+// layer (the same MFMA chain, in the same order, as one wave computing all
+// eight), and the layer's activations reach the other waves through LDS
+// (two alternating buffers, one barrier per layer), so a tile's latency is
+// three short chains instead of 144 dependent products, and a training batch
+// of 4096 rows spreads over 256 workgroups.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mlp_fwd(Net net, const float* __restrict__ X, int rows, float* __restrict__ Y,
-                                                 int save, float* __restrict__ Z, unsigned short* __restrict__ H, long ld) {
-    const int lane = threadIdx.x & 63;
+constexpr int kTrainWaves = 8;
+
+__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_fwd(Net net, const float* __restrict__ X, int rows,
+                                                              float* __restrict__ Y, float* __restrict__ Z,
+                                                              unsigned short* __restrict__ H, long ld) {
+    __shared__ bf16x4 hs[2][8][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     const int tiles = (rows + 15) / 16;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * 4;
-    for (int tile = wave; tile < tiles; tile += nwaves) {
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const long p = (long)tile * 16 + r;
         const bool valid = p < rows;
-        float4 xin = valid ? *(const float4*)(X + p * 16 + 4 * g) : make_float4(0.f, 0.f, 0.f, 0.f);
-        bf16x4 h0[1] = {pack4(xin.x, xin.y, xin.z, xin.w)};
-        if (save) store_fm(H, ld, 0, g, p, h0[0]);
-        bf16x4 h[8];
-        f32x4 z[8];
-        layer<1>(net.W1, h0, z, r, g);
+        const float4 xin = valid ? *(const float4*)(X + p * 16 + 4 * g) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bf16x4 h0 = pack4(xin.x, xin.y, xin.z, xin.w);
+        if (w == 0) store_fm(H, ld, 0, g, p, h0);
+        f32x4 acc = mfma(load4(net.W1 + (long)(16 * w + r) * 16 + 4 * g), h0, f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
-        for (int l = 0; l < 3; l++) {   // bias + SiLU of layer l+1, then layer l+2's product
+        for (int l = 0; l < 3; l++) {   // bias + SiLU of layer l+1, then (waves' share of) layer l+2's product
             const float* b = (l == 0) ? net.b1 : (l == 1) ? net.b2 : net.b3;
+            f32x4 zz = acc;
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                f32x4 zz = z[m];
+            for (int j = 0; j < 4; j++) zz[j] += b[16 * w + 4 * g + j];
+            if (valid) *(f32x4*)(Z + (long)l * ld * 128 + p * 128 + 16 * w + 4 * g) = zz;
+            const bf16x4 hv = pack4(silu(zz[0]), silu(zz[1]), silu(zz[2]), silu(zz[3]));
+            store_fm(H + 16 * ld + (long)l * 128 * ld, ld, 16 * w, g, p, hv);
+            hs[l & 1][w][lane] = hv;
+            __syncthreads();
+            if (l < 2) {
+                const unsigned short* W = (l == 0) ? net.W2 : net.W3;
+                acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int j = 0; j < 4; j++) zz[j] += b[16 * m + 4 * g + j];
-                if (save && valid) *(f32x4*)(Z + (long)l * ld * 128 + p * 128 + 16 * m + 4 * g) = zz;
-                h[m] = pack4(silu(zz[0]), silu(zz[1]), silu(zz[2]), silu(zz[3]));
-                if (save) store_fm(H + 16 * ld + (long)l * 128 * ld, ld, 16 * m, g, p, h[m]);
+                for (int kt = 0; kt < 8; kt++) acc = mfma(load4(W + (long)(16 * w + r) * 128 + 16 * kt + 4 * g), hs[l & 1][kt][lane], acc);
             }
-            if (l < 2) layer<8>(l == 0 ? net.W2 : net.W3, h, z, r, g);
         }
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        if (w == 0) {   // output layer from layer 3's activations (buffer 0)
+            f32x4 y = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < 8; kt++) acc = mfma(load4(net.W4 + (long)r * 128 + 16 * kt + 4 * g), h[kt], acc);
-        if (g == 0 && valid) *(float4*)(Y + p * 4) = make_float4(acc[0] + net.b4[0], acc[1] + net.b4[1], acc[2] + net.b4[2], 0.f);
+            for (int kt = 0; kt < 8; kt++) y = mfma(load4(net.W4 + (long)r * 128 + 16 * kt + 4 * g), hs[0][kt][lane], y);
+            if (g == 0 && valid) *(float4*)(Y + p * 4) = make_float4(y[0] + net.b4[0], y[1] + net.b4[1], y[2] + net.b4[2], 0.f);
+        }
+        __syncthreads();   // buffer 0 is rewritten by the next tile's first layer
     }
 }
 
@@ -203,18 +204,27 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(Net net, const f
 // ---------------------------------------------------------------------------
 // Backward of L = sum (y - t)^2 / (2 rows): deltas of every layer, stored
 // feature-major bf16 (D4 [16][ld], D3..D1 [128][ld]); per-tile column sums of
-// the deltas (bias gradients, fp32) and of the squared error.
+// the deltas (bias gradients, fp32) and of the squared error.  Split like the
+// training forward: wave w of a workgroup computes features 16 w .. 16 w + 15
+// of each hidden layer's delta, the layer's deltas pass to the other waves
+// through LDS (two alternating buffers, one barrier per layer).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mlp_bwd(Net net, const float* __restrict__ Y, const float* __restrict__ T,
-                                                 int rows, const float* __restrict__ Z, unsigned short* __restrict__ D,
-                                                 long ld, float* __restrict__ dbpart, float* __restrict__ losspart) {
-    const int lane = threadIdx.x & 63;
+__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_bwd(Net net, const float* __restrict__ Y,
+                                                              const float* __restrict__ T, int rows,
+                                                              const float* __restrict__ Z, unsigned short* __restrict__ D,
+                                                              long ld, float* __restrict__ dbpart,
+                                                              float* __restrict__ losspart) {
+    __shared__ bf16x4 dsh[2][8][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     const int tiles = (rows + 15) / 16;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * 4;
     const float inv = 1.0f / (float)rows;
-    for (int tile = wave; tile < tiles; tile += nwaves) {
+    // column sums over the 16 particles (lanes with equal g) by xor-shuffles
+    auto colsum = [&](float v) {
+        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    };
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const long p = (long)tile * 16 + r;
         const bool valid = p < rows;
         // output delta: features 0..2 live in the g == 0 lanes
@@ -224,52 +234,50 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(Net net, const float* __restric
             const float4 t = *(const float4*)(T + p * 4);
             e[0] = y.x - t.x; e[1] = y.y - t.y; e[2] = y.z - t.z;
         }
-        f32x4 d4 = {e[0] * inv, e[1] * inv, e[2] * inv, 0.f};
-        float sq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
-        bf16x4 dl[8];
-        dl[0] = pack4(d4[0], d4[1], d4[2], d4[3]);
-        store_fm(D, ld, 0, g, p, dl[0]);
+        const f32x4 d4 = {e[0] * inv, e[1] * inv, e[2] * inv, 0.f};
+        const bf16x4 dl0 = pack4(d4[0], d4[1], d4[2], d4[3]);
         float* dbt = dbpart + (long)tile * (16 + 3 * 128);
-        // column sums over the 16 particles (lanes with equal g) by xor-shuffles
-        auto colsum = [&](float v) {
-            for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-            return v;
-        };
+        if (w == 0) {
+            store_fm(D, ld, 0, g, p, dl0);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float s = colsum(d4[j]);
-            if (r == 0) dbt[4 * g + j] = s;
+            for (int j = 0; j < 4; j++) {
+                const float s = colsum(d4[j]);
+                if (r == 0) dbt[4 * g + j] = s;
+            }
+            const float sq = colsum(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            if (lane == 0) losspart[tile] = sq;
         }
-        sq = colsum(sq);
-        if (lane == 0) losspart[tile] = sq;
-        // hidden layers 3, 2, 1
-        f32x4 acc[8];
+        // hidden layers 3, 2, 1: this wave's 16 features of each
 #pragma unroll
         for (int l = 2; l >= 0; l--) {
-            const unsigned short* WT = (l == 2) ? net.W4T : (l == 1) ? net.W3T : net.W2T;
-            const int KT = (l == 2) ? 1 : 8;
+            f32x4 a = {0.f, 0.f, 0.f, 0.f};
+            if (l == 2) {
+                a = mfma(load4(net.W4T + (long)(16 * w + r) * 16 + 4 * g), dl0, a);
+            } else {
+                const unsigned short* WT = (l == 1) ? net.W3T : net.W2T;
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                f32x4 a = {0.f, 0.f, 0.f, 0.f};
-                for (int kt = 0; kt < KT; kt++) a = mfma(load4(WT + (long)(16 * m + r) * (16 * KT) + 16 * kt + 4 * g), dl[kt], a);
-                acc[m] = a;
+                for (int kt = 0; kt < 8; kt++) a = mfma(load4(WT + (long)(16 * w + r) * 128 + 16 * kt + 4 * g), dsh[(l + 1) & 1][kt][lane], a);
             }
+            const f32x4 zz = valid ? *(const f32x4*)(Z + (long)l * ld * 128 + p * 128 + 16 * w + 4 * g) : f32x4{0.f, 0.f, 0.f, 0.f};
+            f32x4 d;
+#pragma unroll
+            for (int j = 0; j < 4; j++) d[j] = valid ? a[j] * silu_d(zz[j]) : 0.f;
+            const bf16x4 dv = pack4(d[0], d[1], d[2], d[3]);
+            store_fm(D + 16 * ld + (long)(2 - l) * 128 * ld, ld, 16 * w, g, p, dv);
             float* dbl = dbt + 16 + (long)l * 128;
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                f32x4 zz = valid ? *(const f32x4*)(Z + (long)l * ld * 128 + p * 128 + 16 * m + 4 * g) : f32x4{0.f, 0.f, 0.f, 0.f};
-                f32x4 d;
-#pragma unroll
-                for (int j = 0; j < 4; j++) d[j] = valid ? acc[m][j] * silu_d(zz[j]) : 0.f;
-                dl[m] = pack4(d[0], d[1], d[2], d[3]);
-                store_fm(D + 16 * ld + (long)(2 - l) * 128 * ld, ld, 16 * m, g, p, dl[m]);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const float s = colsum(d[j]);
-                    if (r == 0) dbl[16 * m + 4 * g + j] = s;
-                }
+            for (int j = 0; j < 4; j++) {
+                const float s = colsum(d[j]);
+                if (r == 0) dbl[16 * w + 4 * g + j] = s;
+            }
+            if (l > 0) {
+                dsh[l & 1][w][lane] = dv;
+                __syncthreads();
             }
         }
+        // no barrier at the end of a tile: buffer 0 was last read before this
+        // tile's l = 1 barrier, and buffer 1 is rewritten only after the next
+        // tile's l = 2 barrier, which every wave reaches after reading it
     }
 }
 
@@ -303,7 +311,25 @@ __global__ __launch_bounds__(64 * kDwWaves) void k_mlp_dw(const unsigned short* 
     const unsigned short* A = D + (drow + 16 * mt + r) * ld;   // A[m][k] = delta[k][out]
     const unsigned short* B = H + (hrow + 16 * nt + r) * ld;   // B[k][n] = H[k][in]
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (long k = 16L * w; k < ld; k += 16L * kDwWaves) acc = mfma(load4(A + k + 4 * g), load4(B + k + 4 * g), acc);
+    // steps k = 16 (w + kDwWaves s), s < steps; eight steps' loads are issued
+    // before their products (the sum stays one chain in k order)
+    const int steps = (int)((ld - 16L * w + 16L * kDwWaves - 1) / (16L * kDwWaves));
+    int s = 0;
+    for (; s + 8 <= steps; s += 8) {
+        bf16x4 a[8], b[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const long k = 16L * (w + (long)kDwWaves * (s + q));
+            a[q] = load4(A + k + 4 * g);
+            b[q] = load4(B + k + 4 * g);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc = mfma(a[q], b[q], acc);
+    }
+    for (; s < steps; s++) {
+        const long k = 16L * (w + (long)kDwWaves * s);
+        acc = mfma(load4(A + k + 4 * g), load4(B + k + 4 * g), acc);
+    }
     part[w][lane] = acc;
     __syncthreads();
     if (w != 0) return;
@@ -337,10 +363,13 @@ __global__ __launch_bounds__(64 * kDbWaves) void k_mlp_db(const float* __restric
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     float s = 0.f;
-    if (i < kDbCols - 1)
+    if (i < kDbCols - 1) {
+#pragma unroll 8
         for (int t = w; t < tiles; t += kDbWaves) s += dbpart[(long)t * (16 + 3 * 128) + i];
-    else if (i == kDbCols - 1)
+    } else if (i == kDbCols - 1) {
+#pragma unroll 8
         for (int t = w; t < tiles; t += kDbWaves) s += losspart[t];
+    }
     part[w][lane] = s;
     __syncthreads();
     if (w != 0 || i >= kDbCols) return;
@@ -422,12 +451,11 @@ __global__ __launch_bounds__(256) void k_publish(const float* __restrict__ p, Ne
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-// Each wave keeps every weight tile in registers (the compiler hoists the
-// loop-invariant loads: 512 registers, one wave per SIMD), so waves should
-// stream many 16-particle tiles each: one workgroup per CU at most.
+// Training kernels: one workgroup per 16-particle tile, up to 1024 of them
+// (each then takes every 1024th tile).
 static int grid_for(int rows) {
     const int tiles = (rows + 15) / 16;
-    return std::max(1, std::min(256, (tiles + 3) / 4));
+    return std::max(1, std::min(1024, tiles));
 }
 
 hipError_t forward(const Net& net, const float* X, int rows, float* Y, bool save, float* Z, unsigned short* H, long ld,
@@ -439,7 +467,7 @@ hipError_t forward(const Net& net, const float* X, int rows, float* Y, bool save
         hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * kInferWaves), 0, st, net, X, rows, Y);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_mlp_fwd, dim3(grid_for(rows)), dim3(256), 0, st, net, X, rows, Y, 1, Z, H, ld);
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(grid_for(rows)), dim3(64 * kTrainWaves), 0, st, net, X, rows, Y, Z, H, ld);
     return hipGetLastError();
 }
 
@@ -447,7 +475,8 @@ hipError_t backward(const Net& net, const float* Y, const float* T, int rows, co
                     unsigned short* D, long ld, float* dbpart, float* losspart, float* grad, float* loss,
                     hipStream_t st) {
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_bwd, dim3(grid_for(rows)), dim3(256), 0, st, net, Y, T, rows, Z, D, ld, dbpart, losspart);
+    hipLaunchKernelGGL(k_mlp_bwd, dim3(grid_for(rows)), dim3(64 * kTrainWaves), 0, st, net, Y, T, rows, Z, D, ld, dbpart,
+                       losspart);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_mlp_dw, dim3(TILES), dim3(64 * kDwWaves), 0, st, H, (const unsigned short*)D, ld, grad);
