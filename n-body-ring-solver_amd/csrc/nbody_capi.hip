@@ -231,7 +231,8 @@ struct SymRegions {
 // segments (N = 262144, FP64 / mixed kernel ms at 3 / 12 / 24 / 48 / 96
 // ranges per CU: 24.2 / 23.6 / 23.0 / 23.5 / 23.8 and 18.6 / 16.6 / 16.0 /
 // 15.9 / 15.8, with the partial sums growing 0.60 / 0.75 / 0.95 / 1.35 /
-// 2.16 GB FP64).
+// 2.16 GB FP64 in 32-byte records; three quarters of that in the component
+// planes used now).
 static int symg_workgroups(const nb_ctx* c, long units) {
     return int(std::max(1L, std::min(long(c->num_cus) * 24, units)));
 }
@@ -292,12 +293,13 @@ static hipError_t symg_plan(nb_ctx* c, size_t n, int K, const nb_ctx::SymgPlan*&
 }
 
 // Partial sums of the grouped kernel: action slots (nseg of W blocks of S
-// records, 32 B) then reaction slots (nslots of S records: 32 B FP64, 16 B
-// FP32 for the mixed forms).  `nseg` at most workgroups + rows.
+// particles, 3 doubles each: 24 B) then reaction slots (nslots of S
+// particles: 3 doubles FP64, 3 floats for the mixed forms), every slot as
+// three component planes.  `nseg` at most workgroups + rows.
 static SymRegions symg_regions(long nB, long S, long nseg, int kform) {
     const long W = nbk::kGroupWaves;
-    const size_t rec = (kform == nbk::FORM_FP64) ? sizeof(P4) : sizeof(float4);
-    return {size_t(nseg) * size_t(W) * size_t(S) * sizeof(P4), size_t(symg_slots(nB, W)) * size_t(S) * rec};
+    const size_t rec = 3 * ((kform == nbk::FORM_FP64) ? sizeof(double) : sizeof(float));
+    return {size_t(nseg) * size_t(W) * size_t(S) * 3 * sizeof(double), size_t(symg_slots(nB, W)) * size_t(S) * rec};
 }
 
 // Partial sums of the symmetric kernel: the action slots (nB*W slots of S
@@ -426,18 +428,19 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
         L.nwg = plan->nwg;
         L.nB = plan->nB;
         L.eps2 = eps * eps;
-        L.act = (P4*)c->part.p;
-        if (kform == nbk::FORM_FP64) L.react64 = (P4*)react;
-        else L.react32 = (float4*)react;
+        L.act = (double*)c->part.p;
+        if (kform == nbk::FORM_FP64) L.react64 = (double*)react;
+        else L.react32 = (float*)react;
         L.eqm = scale != 0.0;
         out.sym = true;
-        out.part = L.act;
-        out.react64 = L.react64;
-        out.react32 = L.react32;
-        out.nB = plan->nB;
-        out.S = int(S);
-        out.rowseg = (const int2*)plan->rowseg.p;
-        out.scale = L.eqm ? scale : 1.0;
+        out.grouped = true;
+        out.gsums.act = L.act;
+        out.gsums.rowseg = (const int2*)plan->rowseg.p;
+        out.gsums.react64 = L.react64;
+        out.gsums.react32 = L.react32;
+        out.gsums.nB = plan->nB;
+        out.gsums.S = int(S);
+        out.gsums.scale = L.eqm ? scale : 1.0;
         note_kernel(c, NB_KERNEL_SYMMETRIC, kform, K, int(S), uint64_t(plan->nwg), R.act_bytes + R.react_bytes, true,
                     L.eqm);
         const long ev = timing_begin(c);
@@ -523,18 +526,6 @@ hipError_t force_eval(nb_ctx* c, int kform, double eps, ForceOut& out) {
     return e;
 }
 
-static nbk::SymgSums symg_sums(const ForceOut& out) {
-    nbk::SymgSums P;
-    P.act = out.part;
-    P.rowseg = out.rowseg;
-    P.react64 = out.react64;
-    P.react32 = out.react32;
-    P.nB = out.nB;
-    P.S = out.S;
-    P.scale = out.scale;
-    return P;
-}
-
 static nbk::SymSums sym_sums(const ForceOut& out) {
     nbk::SymSums P;
     P.part = out.part;
@@ -550,8 +541,8 @@ static nbk::SymSums sym_sums(const ForceOut& out) {
 // acc = sum of the partials of `out`.
 hipError_t finish_accel(nb_ctx* c, const ForceOut& out) {
     const int n = int(c->n);
-    if (out.sym && out.rowseg)
-        return nbk::launch_symg_finish(symg_sums(out), n, nullptr, nullptr, (P4*)c->acc.p, false, 0.0, 0.0, 0.0,
+    if (out.sym && out.grouped)
+        return nbk::launch_symg_finish(out.gsums, n, nullptr, nullptr, (P4*)c->acc.p, false, 0.0, 0.0, 0.0,
                                        c->stream);
     if (out.sym) return nbk::launch_sym_reduce(sym_sums(out), n, (P4*)c->acc.p, c->stream);
     if (out.S == 1) return hipSuccess;
@@ -564,8 +555,8 @@ hipError_t finish_kick_drift(nb_ctx* c, const ForceOut& out, double kdt, double 
     P4* pm = (P4*)c->pm.p;
     P4* vel = (P4*)c->vel.p;
     P4* acc = (P4*)c->acc.p;
-    if (out.sym && out.rowseg)
-        return nbk::launch_symg_finish(symg_sums(out), n, pm, vel, acc, true, kdt, c1dt, c2dt, c->stream);
+    if (out.sym && out.grouped)
+        return nbk::launch_symg_finish(out.gsums, n, pm, vel, acc, true, kdt, c1dt, c2dt, c->stream);
     if (out.sym)
         return nbk::launch_sym_kick_drift(sym_sums(out), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
     return nbk::launch_kick_drift(out.part, out.S, long(n), n, pm, vel, acc, kdt, c1dt, c2dt, c->stream);
@@ -945,9 +936,9 @@ static nb_status ring_round_eval(nb_ctx* c, int kf, double eps, long n, const P4
         L.nwg = plan->nwg;
         L.nB = plan->nB;
         L.eps2 = eps * eps;
-        L.act = (P4*)c->part.p;
-        if (kf == nbk::FORM_FP64) L.react64 = (P4*)react;
-        else L.react32 = (float4*)react;
+        L.act = (double*)c->part.p;
+        if (kf == nbk::FORM_FP64) L.react64 = (double*)react;
+        else L.react32 = (float*)react;
         const long ev = timing_begin(c);
         hipError_t e = nbk::launch_accel_symg(L, c->stream);
         timing_end(c, ev);

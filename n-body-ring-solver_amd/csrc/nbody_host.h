@@ -41,8 +41,9 @@ struct ForceOut {
     const float4* react32 = nullptr;      // symmetric, mixed forms: FP32 reaction slots
     int S = 1;          // tile kernel: number of j-splits; symmetric: block size
     int nB = 0, L = 1;  // symmetric: blocks, tiles per work item
-    const int2* rowseg = nullptr;   // grouped symmetric kernel: action segments per group row (else null)
     double scale = 1.0;             // symmetric: factor of the sums (the mass an equal-mass evaluation left out)
+    bool grouped = false;           // grouped symmetric kernel: its sums are `gsums` (the fields above unused)
+    nbk::SymgSums gsums;
 };
 
 // Context internals used by the learned-form module.

@@ -129,8 +129,9 @@ hipError_t launch_sym_kick_drift(const SymSums& P, int n, P4* pm, P4* vel, P4* a
 // Grouped symmetric kernel (nbody_sym.hip): workgroups of kGroupWaves waves
 // over W consecutive blocks, each walking a contiguous range of the chunk
 // sequence; ranges[wg] = (g, B * K + c, chunks, first action slot); action
-// slots of kGroupWaves * S records; reaction slots (g, B), B >= g W, of S
-// records (FP32 for the mixed forms).
+// slots of kGroupWaves blocks, each three component planes of S doubles;
+// reaction slots (g, B), B >= g W, of three planes of S values (doubles, or
+// floats for the mixed forms).
 // four waves: three workgroups a CU at three waves per SIMD; groups of two
 // measured slower (N = 262144 FP64 33.4 against 24.4 ms)
 constexpr int kGroupWaves = 4;
@@ -145,9 +146,9 @@ struct SymgLaunch {
     int nwg = 0;
     int nB = 0;
     double eps2 = 0.0;
-    P4* act = nullptr;
-    P4* react64 = nullptr;
-    float4* react32 = nullptr;
+    double* act = nullptr;       // action slots: W blocks of 3 planes (x, y, z) of S doubles
+    double* react64 = nullptr;   // FP64 form: reaction slots of 3 planes of S doubles
+    float* react32 = nullptr;    // mixed forms: reaction slots of 3 planes of S floats
     bool eqm = false;   // every mass equal: as SymLaunch::eqm
 };
 hipError_t launch_accel_symg(const SymgLaunch& L, hipStream_t st);
@@ -155,10 +156,10 @@ hipError_t launch_accel_symg(const SymgLaunch& L, hipStream_t st);
 // The grouped kernel's partial sums: rowseg[g] = (first action slot of row g,
 // number of its segments).
 struct SymgSums {
-    const P4* act = nullptr;
+    const double* act = nullptr;
     const int2* rowseg = nullptr;
-    const P4* react64 = nullptr;
-    const float4* react32 = nullptr;
+    const double* react64 = nullptr;
+    const float* react32 = nullptr;
     int nB = 0, S = 0;
     double scale = 1.0;   // as SymSums::scale
 };

@@ -274,18 +274,21 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_accel_sym(const P4* __r
 // once (its 256 doubles spread over the threads, prefetched a chunk ahead), every wave sweeps
 // them against its own block (B < A: nothing; B == A: the diagonal mask), the
 // waves' reaction sums meet in LDS and wave 0 adds them in wave order (FP64)
-// into one record per chunk particle of reaction slot (g, B),
-// g nB - W g(g-1)/2 + (B - gW).  The actions stay in registers along the row
-// and are written once per row segment, to action slot seg (W blocks of S
-// records); the segments of a row are consecutive.  Sums are in a fixed
-// order: results are bitwise reproducible.
+// into reaction slot (g, B), g nB - W g(g-1)/2 + (B - gW): three component
+// planes of S values (x, y, z; doubles, floats for the mixed forms), the
+// chunk's 64 at c*64.  The actions stay in registers along the row and are
+// written once per row segment, to action slot seg (W blocks, each three
+// planes of S doubles); the segments of a row are consecutive.  No padding
+// word is stored: 24 (12) bytes per partial sum.  Sums are in a fixed order:
+// results are bitwise reproducible.
 // ---------------------------------------------------------------------------
 
 template <int FORM, bool EPS0, int K, int W, bool TWO, bool EQM>
 __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm, const P4* __restrict__ pj, int n,
                                                        const int4* __restrict__ ranges, int nB, double eps2,
-                                                       double cshift, double coff, int incl, P4* __restrict__ act,
-                                                       P4* __restrict__ react64, float4* __restrict__ react32) {
+                                                       double cshift, double coff, int incl,
+                                                       double* __restrict__ act, double* __restrict__ react64,
+                                                       float* __restrict__ react32) {
     constexpr bool MIXED = (FORM != FORM_FP64);
     constexpr bool NR = (FORM == FORM_MIXED_NR);
     constexpr int S = 64 * K;
@@ -323,9 +326,13 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
         __builtin_amdgcn_s_waitcnt(0x0F70);
     };
     auto flush = [&]() {
-        P4* __restrict__ o = act + ((long)slot * W + wave) * S + lane;
+        double* __restrict__ o = act + ((long)slot * W + wave) * (3 * S) + lane;
 #pragma unroll
-        for (int k = 0; k < K; k++) o[k * 64] = P4{ax[k], ay[k], az[k], 0.0};
+        for (int k = 0; k < K; k++) {
+            o[k * 64] = ax[k];
+            o[S + k * 64] = ay[k];
+            o[2 * S + k * 64] = az[k];
+        }
     };
     // the chunk's 256 doubles spread over the threads (NPT each; indexes past
     // 255 stage nothing); the mass takes its factor when staged, so nothing
@@ -341,13 +348,24 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
     };
     // wave 0's summed reactions of the previous chunk: stored after the next
     // barrier, so the store is in flight during a sweep instead of being
-    // waited for at a barrier
+    // waited for at a barrier.  FP64: lane l holds chunk particle l's three
+    // sums (three 512-byte plane stores).  Mixed forms: lanes 0-47 hold four
+    // consecutive particles' sums of one component (lane l: component l / 16,
+    // particles 4 (l mod 16) ..), one 16-byte store each, so every 64 bytes
+    // of a plane come from one quarter-wave (wave-wide 4-byte plane stores,
+    // waited for at the next chunk, ran the kernel 9 % slower).
     long pend_at = -1;
     double px = 0.0, py = 0.0, pz = 0.0;
+    float4 pq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     auto store_pending = [&]() {
         if (pend_at < 0) return;
-        if constexpr (MIXED) react32[pend_at] = make_float4((float)px, (float)py, (float)pz, 0.0f);
-        else react64[pend_at] = P4{px, py, pz, 0.0};
+        if constexpr (MIXED) {
+            *reinterpret_cast<float4*>(react32 + pend_at) = pq;
+        } else {
+            react64[pend_at] = px;
+            react64[pend_at + S] = py;
+            react64[pend_at + 2 * S] = pz;
+        }
     };
     int A = g * W + wave;
     load_block(A);
@@ -390,13 +408,27 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
         red[wave][2][lane] = bz;
         __syncthreads();   // every wave's reactions of this chunk are in LDS; the chunk is free
         if (wave == 0) {
-            px = py = pz = 0.0;
-#pragma unroll
-            for (int w = 0; w < W; w++) {
-                px += red[w][0][lane]; py += red[w][1][lane]; pz += red[w][2][lane];
-            }
             const long rslot = (long)g * nB - (long)W * g * (g - 1) / 2 + (B - g * W);
-            pend_at = rslot * S + c * 64 + lane;
+            if constexpr (MIXED) {
+                if (lane < 48) {
+                    const int comp = lane >> 4, p0 = (lane & 15) * 4;
+                    double q[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int w = 0; w < W; w++) {
+#pragma unroll
+                        for (int u = 0; u < 4; u++) q[u] += red[w][comp][p0 + u];
+                    }
+                    pq = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+                    pend_at = rslot * (3 * S) + comp * S + c * 64 + p0;
+                }
+            } else {
+                px = py = pz = 0.0;
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    px += red[w][0][lane]; py += red[w][1][lane]; pz += red[w][2][lane];
+                }
+                pend_at = rslot * (3 * S) + c * 64 + lane;
+            }
         }
         if (g2 != g) {   // the row ends: its actions go to their slot, the next row's block is loaded
             flush();
@@ -412,61 +444,97 @@ __global__ __launch_bounds__(64 * W) void k_accel_symg(const P4* __restrict__ pm
 
 // Sums of the grouped form for particle i of block X (group gX = X / W):
 // reactions from slots (g, X), g = 0 .. gX; actions from the segments of row
-// gX (slots seg0 .. seg0 + nseg - 1, the wave X - gX W block of each).
+// gX (slots seg0 .. seg0 + nseg - 1, the wave X - gX W block of each).  Each
+// slot is three planes of S values; eight slots' values are loaded together
+// (in flight at once) and added one by one in slot order.
+template <typename T, int U>
+__device__ __forceinline__ void add_planes(const T* __restrict__ base, long stride, int count, int S, double& x,
+                                           double& y, double& z) {
+    int q = 0;
+    for (; q + U <= count; q += U) {
+        T vx[U], vy[U], vz[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const T* p = base + (long)(q + u) * stride;
+            vx[u] = p[0];
+            vy[u] = p[S];
+            vz[u] = p[2 * S];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            x += (double)vx[u]; y += (double)vy[u]; z += (double)vz[u];
+        }
+    }
+    for (; q < count; q++) {
+        const T* p = base + (long)q * stride;
+        x += (double)p[0]; y += (double)p[S]; z += (double)p[2 * S];
+    }
+}
+
 template <bool F32R, int W>
-__device__ __forceinline__ void symg_react(const P4* __restrict__ react64, const float4* __restrict__ react32,
+__device__ __forceinline__ void symg_react(const double* __restrict__ react64, const float* __restrict__ react32,
                                            int nB, int S, int i, double& bx, double& by, double& bz) {
-    constexpr int U = 8;   // slots loaded together, added in slot order
+    constexpr int U = 8;
     const int X = i / S, r = i - X * S, gX = X / W;
     bx = by = bz = 0.0;
-    auto at = [&](int g) { return ((long)g * nB - (long)W * g * (g - 1) / 2 + (X - g * W)) * S + r; };
+    // slot (g, X) = g nB - W g(g-1)/2 + (X - g W): consecutive g are nB - W g
+    // slots apart, so the slots are visited by a running index
+    long slot = X;
     int g = 0;
     for (; g + U <= gX + 1; g += U) {
-        if constexpr (F32R) {
-            float4 q[U];
+        long at[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) q[u] = react32[at(g + u)];
+        for (int u = 0; u < U; u++) {
+            at[u] = slot * (3L * S) + r;
+            slot += nB - (long)W * (g + u) - W;
+        }
+        if constexpr (F32R) {
+            float vx[U], vy[U], vz[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                bx += (double)q[u].x; by += (double)q[u].y; bz += (double)q[u].z;
+                vx[u] = react32[at[u]]; vy[u] = react32[at[u] + S]; vz[u] = react32[at[u] + 2 * S];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                bx += (double)vx[u]; by += (double)vy[u]; bz += (double)vz[u];
             }
         } else {
-            P4 q[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) q[u] = react64[at(g + u)];
+            double vx[U], vy[U], vz[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                bx += q[u].x; by += q[u].y; bz += q[u].z;
+                vx[u] = react64[at[u]]; vy[u] = react64[at[u] + S]; vz[u] = react64[at[u] + 2 * S];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                bx += vx[u]; by += vy[u]; bz += vz[u];
             }
         }
     }
     for (; g <= gX; g++) {
+        const long at = slot * (3L * S) + r;
+        slot += nB - (long)W * g - W;
         if constexpr (F32R) {
-            const float4 q = react32[at(g)];
-            bx += (double)q.x; by += (double)q.y; bz += (double)q.z;
+            bx += (double)react32[at]; by += (double)react32[at + S]; bz += (double)react32[at + 2 * S];
         } else {
-            const P4 q = react64[at(g)];
-            bx += q.x; by += q.y; bz += q.z;
+            bx += react64[at]; by += react64[at + S]; bz += react64[at + 2 * S];
         }
     }
 }
 
 template <int W>
-__device__ __forceinline__ void symg_act(const P4* __restrict__ act, const int2* __restrict__ rowseg, int S, int i,
-                                         double& ax, double& ay, double& az) {
+__device__ __forceinline__ void symg_act(const double* __restrict__ act, const int2* __restrict__ rowseg, int S,
+                                         int i, double& ax, double& ay, double& az) {
     const int X = i / S, r = i - X * S, gX = X / W, w = X - gX * W;
     const int2 sg = rowseg[gX];
     ax = ay = az = 0.0;
-    for (int q = 0; q < sg.y; q++) {
-        const P4 p = act[((long)(sg.x + q) * W + w) * S + r];
-        ax += p.x; ay += p.y; az += p.z;
-    }
+    add_planes<double, 8>(act + ((long)sg.x * W + w) * (3L * S) + r, (long)W * 3 * S, sg.y, S, ax, ay, az);
 }
 
 template <bool F32R, int W, bool KICK>
-__global__ __launch_bounds__(BLOCK) void k_symg_finish(const P4* __restrict__ act, const int2* __restrict__ rowseg,
-                                                       const P4* __restrict__ react64,
-                                                       const float4* __restrict__ react32, int nB, int S, int n,
+__global__ __launch_bounds__(BLOCK) void k_symg_finish(const double* __restrict__ act,
+                                                       const int2* __restrict__ rowseg,
+                                                       const double* __restrict__ react64,
+                                                       const float* __restrict__ react32, int nB, int S, int n,
                                                        double scale, P4* __restrict__ pm, P4* __restrict__ vel,
                                                        P4* __restrict__ acc, double kdt, double c1dt, double c2dt) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
@@ -498,9 +566,10 @@ __global__ __launch_bounds__(BLOCK) void k_symg_finish(const P4* __restrict__ ac
 // Two-set finish of the grouped form: local forces from the actions, the
 // received set's from the reactions, times the particles' masses, added.
 template <bool F32R, int W>
-__global__ __launch_bounds__(BLOCK) void k_symg_ring_finish(const P4* __restrict__ act, const int2* __restrict__ rowseg,
-                                                            const P4* __restrict__ react64,
-                                                            const float4* __restrict__ react32, int nB, int S, int n,
+__global__ __launch_bounds__(BLOCK) void k_symg_ring_finish(const double* __restrict__ act,
+                                                            const int2* __restrict__ rowseg,
+                                                            const double* __restrict__ react64,
+                                                            const float* __restrict__ react32, int nB, int S, int n,
                                                             const P4* __restrict__ pl, const P4* __restrict__ pr,
                                                             P4* __restrict__ fl, P4* __restrict__ fr) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
