@@ -195,6 +195,33 @@ NB_API nb_status nb_forces_ring_round_device(nb_ctx* ctx, int form, double eps, 
 NB_API nb_status nb_step(nb_ctx* ctx, size_t time_steps, double dt, const double* c, const double* d,
                   size_t lf_steps, int form, double eps);
 
+/* Whole run in the FPGA host's form: replaces lrbd_solver_no_sync (and its
+ * all-FP32 twin lrbd_solver_sp with form NB_FORM_FP32),
+ * FPGA/host/solver.cpp:1112, as FPGA/host/main.cpp:495-531 calls it.
+ * pos4, vel4, force4: n records of 4 doubles (x, y, z, unused: OpenCL's
+ * cl_double3 with its 32-byte stride); the 4th word is neither read nor
+ * written.  The pair force on particle i is K coeff_i coeff_j d/|d|^3
+ * (softened with eps), the kick divides it by mass_i, so coeff are the
+ * pair weights (NULL: coeff = mass, which is what the reference's host
+ * passes) and K the scalar factor (G); K coeff_i / mass_i must be one value
+ * for every particle (masses positive and finite), else NB_ERR_ARG before
+ * anything changes.  Runs time_steps DKD steps of dt with c[lf_steps],
+ * d[lf_steps-1] on the device (nb_step, the FPGA integrator's merged drifts,
+ * FPGA/device/ring_lrbd_lf_no_sync_local.cl:820-903), then writes pos4 and
+ * vel4 back and, if force4 is not NULL, force4 = the last force evaluation's
+ * K coeff_i a_i (the FPGA's local minus remote force,
+ * FPGA/host/solver.cpp:1563-1571).  *seconds (if not NULL) = the device time
+ * of the steps, from events on the context's stream (the reference returns
+ * its integrator kernel's time); host copies are outside it.  The run
+ * replaces the context's resident state (as nb_set_state with the pair
+ * weights as masses).  time_steps = 0 or n = 0 changes nothing.  The
+ * reference's MPI scatter over FPGAs, topology and power sampling have no
+ * counterpart on one GPU. */
+NB_API nb_status nb_run_records(nb_ctx* ctx, int form, double eps, double K, const double* mass,
+                                const double* coeff, double* pos4, double* vel4, double* force4, size_t n,
+                                size_t time_steps, double dt, const double* c, const double* d, size_t lf_steps,
+                                double* seconds);
+
 /* Advance the resident state by time_steps explicit Euler steps of size dt
  * (reduced_solver_omp, CPU/omp_solver.cpp:16-137, the reference CLI's default
  * integrator): per step, a = force/m of the current state, x += v dt with the

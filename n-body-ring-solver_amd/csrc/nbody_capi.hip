@@ -1123,6 +1123,81 @@ nb_status nb_step(nb_ctx* c, size_t time_steps, double dt, const double* cc, con
     return NB_OK;
 }
 
+// Whole run in the FPGA host's record form (lrbd_solver_no_sync,
+// FPGA/host/solver.cpp:1112): the records become the resident state with the
+// pair weights as masses, the kick's factor K coeff_i / mass_i (one value)
+// scales the kick coefficients, nb_step runs the steps, and the records come
+// back with force = K coeff_i a_i of the last evaluation.
+nb_status nb_run_records(nb_ctx* c, int form, double eps, double K, const double* mass, const double* coeff,
+                         double* pos4, double* vel4, double* force4, size_t n, size_t time_steps, double dt,
+                         const double* cc, const double* dd, size_t lf_steps, double* seconds) {
+    const char* fn = "nb_run_records";
+    if (seconds) *seconds = 0.0;
+    if (!c) return fail(NB_ERR_ARG, "nb_run_records: null context");
+    if (n > kMaxN) return fail(NB_ERR_ARG, "nb_run_records: N must stay below 1e8");
+    if (n > 0 && (!mass || !pos4 || !vel4)) return fail(NB_ERR_ARG, "nb_run_records: null mass, position or velocity records");
+    if (!cc || !dd) return fail(NB_ERR_ARG, "nb_run_records: null coefficient array");
+    if (lf_steps < 2 || lf_steps > 16) return fail(NB_ERR_ARG, "nb_run_records: lf_steps must be in [2, 16]");
+    if (!std::isfinite(dt)) return fail(NB_ERR_ARG, "nb_run_records: dt must be finite");
+    if (!std::isfinite(K)) return fail(NB_ERR_ARG, "nb_run_records: K must be finite");
+    if (nb_status s = check_eps(eps, fn)) return s;
+    if (form != NB_FORM_LEARNED && kernel_form(form) < 0) return fail(NB_ERR_ARG, "nb_run_records: unknown form");
+    const double* w = coeff ? coeff : mass;
+    double f = 1.0;
+    for (size_t i = 0; i < n; i++) {
+        if (!(mass[i] > 0.0) || !std::isfinite(mass[i]) || !std::isfinite(w[i]))
+            return fail(NB_ERR_ARG, "nb_run_records: masses must be positive and finite, coefficients finite");
+        const double fi = K * w[i] / mass[i];
+        if (i == 0) f = fi;
+        else if (fi != f)
+            return fail(NB_ERR_ARG, "nb_run_records: K coeff[i] / mass[i] must be the same for every particle "
+                                    "(the reference's host passes coeff = mass)");
+    }
+    if (n == 0 || time_steps == 0) return NB_OK;
+    std::vector<double> P(3 * n), V(3 * n);
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            P[k * n + i] = pos4[4 * i + k];
+            V[k * n + i] = vel4[4 * i + k];
+        }
+    double* const pp[3] = {P.data(), P.data() + n, P.data() + 2 * n};
+    double* const vp[3] = {V.data(), V.data() + n, V.data() + 2 * n};
+    if (nb_status s = nb_set_state(c, n, w, pp, vp)) return s;
+    std::vector<double> dk(lf_steps - 1);
+    for (size_t s = 0; s + 1 < lf_steps; s++) dk[s] = f * dd[s];
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    NB_HIP(hipEventCreate(&ev[0]), "hipEventCreate");
+    if (hipError_t e = hipEventCreate(&ev[1]); e != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return fail(NB_ERR_DEVICE, std::string("nb_run_records: hipEventCreate: ") + hipGetErrorString(e));
+    }
+    auto drop = [&]() {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    };
+    hipError_t e = hipEventRecord(ev[0], c->stream);
+    nb_status s = e == hipSuccess ? nb_step(c, time_steps, dt, cc, dk.data(), lf_steps, form, eps) : NB_ERR_DEVICE;
+    if (s == NB_OK && (e = hipEventRecord(ev[1], c->stream)) == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float ms = 0.0f;
+    if (s == NB_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    drop();
+    if (s != NB_OK) return s;
+    if (e != hipSuccess) return fail(NB_ERR_DEVICE, std::string("nb_run_records: timing events: ") + hipGetErrorString(e));
+    if (seconds) *seconds = 1e-3 * double(ms);
+    if (nb_status g = nb_get_state(c, pp, vp)) return g;
+    for (size_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            pos4[4 * i + k] = P[k * n + i];
+            vel4[4 * i + k] = V[k * n + i];
+        }
+    if (force4) {
+        if (nb_status g = download_acc(c, pp, false)) return g;
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) force4[4 * i + k] = K * w[i] * P[k * n + i];
+    }
+    return NB_OK;
+}
+
 // Shared argument checks of the Euler and Runge-Kutta drivers.
 static nb_status driver_args(nb_ctx* c, double dt, int form, double eps, const char* fn, int& kf) {
     if (!c) return fail(NB_ERR_ARG, std::string(fn) + ": null context");

@@ -43,7 +43,7 @@ EXPORTS = (
     "nb_snapshot_write", "nb_snapshot_frames", "nb_snapshot_read",
     "nb_step_euler", "nb_runge_kutta", "nb_step_rk", "nb_force_error", "nb_deviation", "nb_inertia",
     "nb_ring_predict", "nb_kernel_info_read", "nb_mlp_adam_step", "nb_forces_ring_round",
-    "nb_forces_ring_round_device", "nb_accel_block_device_noalloc",
+    "nb_forces_ring_round_device", "nb_accel_block_device_noalloc", "nb_run_records",
 )
 KERNELS = {"auto": 0, "tile": 1, "symmetric": 2}
 
@@ -117,6 +117,8 @@ def load_library(path: str = LIB_PATH):
     L.nb_snapshot_write.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_size_t, _D, _P3, _P3, _P3]
     L.nb_snapshot_frames.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.nb_snapshot_read.argtypes = [C.c_char_p, C.c_size_t, C.c_size_t, _D, _D, _P3, _P3, _P3]
+    L.nb_run_records.argtypes = [_VP, C.c_int, C.c_double, C.c_double, _D, _D, _D, _D, _D, _SZ, _SZ, C.c_double,
+                                 _D, _D, _SZ, _D]
     L.nb_step_euler.argtypes = [_VP, _SZ, C.c_double, C.c_int, C.c_double]
     L.nb_runge_kutta.argtypes = [C.c_int, _D, _D, C.POINTER(_SZ)]
     L.nb_step_rk.argtypes = [_VP, _SZ, C.c_double, _D, _D, _SZ, C.c_int, C.c_double]
@@ -372,6 +374,28 @@ class Solver:
         c = np.ascontiguousarray(c, dtype=np.float64)
         d = np.ascontiguousarray(d, dtype=np.float64)
         _check(self.L.nb_step(self.h, steps, dt, _p(c), _p(d), lf, _form(form), eps), "nb_step")
+
+    def run_records(self, mass, pos4, vel4, steps: int, dt: float, order: int = 2, form="fp64",
+                    eps: float = 0.0, K: float = 1.0, coeff=None, force4=None, coeffs=None) -> float:
+        """Whole run in the FPGA host's record form (nb_run_records): pos4 and
+        vel4 ((n, 4) float64, C-contiguous; column 3 is neither read nor
+        written) advance in place, force4 (same shape, optional) receives
+        K coeff_i a_i of the last force evaluation; returns the device
+        seconds of the steps."""
+        for a in (pos4, vel4) + (() if force4 is None else (force4,)):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"] and a.ndim == 2 and a.shape[1] == 4
+        n = pos4.shape[0]
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        cf = None if coeff is None else np.ascontiguousarray(coeff, dtype=np.float64)
+        assert vel4.shape == pos4.shape and mass.shape == (n,) and (cf is None or cf.shape == (n,))
+        c, d, lf = coeffs if coeffs is not None else yoshida(order)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        secs = C.c_double()
+        _check(self.L.nb_run_records(self.h, _form(form), eps, K, _p(mass), None if cf is None else _p(cf),
+                                     _p(pos4), _p(vel4), None if force4 is None else _p(force4), n, steps, dt,
+                                     _p(c), _p(d), lf, C.byref(secs)), "nb_run_records")
+        return secs.value
 
     def step_euler(self, steps: int, dt: float, form="fp64", eps: float = 0.0):
         _check(self.L.nb_step_euler(self.h, steps, dt, _form(form), eps), "nb_step_euler")

@@ -78,6 +78,7 @@ def test_argument_errors():
     assert L.nb_step(None, 1, 1e-3, None, None, 2, 0, 0.0) == 1
     assert L.nb_energy(None, 0.0, None, None) == 1
     assert L.nb_step_euler(None, 1, 1e-3, 0, 0.0) == 1
+    assert L.nb_run_records(None, 0, 0.0, 1.0, None, None, None, None, None, 0, 1, 1e-3, None, None, 2, None) == 1
     assert L.nb_step_rk(None, 1, 1e-3, None, None, 4, 0, 0.0) == 1
     assert L.nb_runge_kutta(4, None, None, None) == 1
     assert L.nb_kernel_info_read(None, None) == 1
