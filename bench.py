@@ -67,8 +67,13 @@ DE_BOUND = 1e-6
 LEARNED = {"period": 4, "threshold": 3e-3, "rc": 0.25, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
 LEARNED_P8 = {"period": 8, "threshold": 4e-3, "rc": 0.35, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
 HEADLINE_FORMS = ("fp64", "mixed", "learned")   # the ablation and the period-8 variant are reported beside
-TRAIN = {"n": 65536, "steps": 200, "seed": 2, "iters": 4000, "batch": 4096, "lr": 1e-3, "iters_fine": 4000,
-         "lr_fine": 2e-4}
+# The network's training: 16000 Adam iterations at lr 2e-3, then 8000 at 3e-4
+# (measured at N = 262144 over 400 learned steps, mean check-error ratio
+# cubic / network: 4000 + 4000 at 1e-3 / 2e-4 1.071, this schedule 1.092,
+# 24000 + 12000 1.093 with a lower training loss, 16000 + 8000 at 3e-3 /
+# 3e-4 1.093: the gain levels off near 1.09)
+TRAIN = {"n": 65536, "steps": 200, "seed": 2, "iters": 16000, "batch": 4096, "lr": 2e-3, "iters_fine": 8000,
+         "lr_fine": 3e-4}
 
 
 def log(*a):
