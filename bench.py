@@ -282,8 +282,40 @@ def ring_share(eps, seed, ni=131072, nj=1048576, j_block=131072, reps=3):
         res[form] = {"ms_per_share": ms / reps, "launches_per_share": launches // reps,
                      "tpairs_per_s": ni * nj / (ms / reps) / 1e9}
     res["mixed"]["max_rel_force_err_vs_fp64"] = rel_err(outs["mixed"], outs["fp64"])
+    # the reference's own share (hybrid_ring_solver, CPU/hybrid_solver.cpp:81-95):
+    # rank 0 of a ring of nj / j_block ranks runs one reduced round per block
+    # the ring delivers (nb_forces_ring_round_device: pairs q >= k or q > k
+    # evaluated once, both blocks' forces accumulate), about half the pairs of
+    # the full share; the rest reach rank 0 as the returning message's forces
+    comm = nj // j_block
+    reduced = {}
+    if ni == j_block:
+        for form in ("fp64", "mixed"):
+            fl = torch.zeros((ni, 4), dtype=torch.float64, device="cuda")
+            fr = torch.zeros((comm, j_block, 4), dtype=torch.float64, device="cuda")
+
+            def share():
+                for r in range(comm):
+                    s.forces_ring_round_device(form, eps, ni, pi.data_ptr(), fl.data_ptr(),
+                                               pj[r * j_block:(r + 1) * j_block].data_ptr(), fr[r].data_ptr(),
+                                               0, comm, r)
+            share()
+            s.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                share()
+            s.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            # round 0 is rank 0's own block (q > k), rounds 1.. have rank < partner (q >= k)
+            evaluated = ni * (ni - 1) / 2 + (comm - 1) * ni * (ni + 1) / 2
+            reduced[form] = {"ms_per_share": ms, "rounds": comm,
+                             "pair_evaluations_per_share": evaluated,
+                             "tpairs_per_s_effective": ni * nj / ms / 1e9,
+                             "counted_as": "ni x nj like the full share; about half of them are evaluated, "
+                                           "the others' forces return with the ring's message"}
     print(json.dumps({"workload": f"ring share: Plummer N={nj}, {ni} i-particles x {nj} j-particles in "
-                                  f"blocks of {j_block}, eps={eps}", "forms": res}), flush=True)
+                                  f"blocks of {j_block}, eps={eps}", "forms": res,
+                      "reduced_ring_rounds": reduced}), flush=True)
     s.close()
 
 
