@@ -530,15 +530,13 @@ __device__ __forceinline__ void symg_act(const double* __restrict__ act, const i
     add_planes<double, 8>(act + ((long)sg.x * W + w) * (3L * S) + r, (long)W * 3 * S, sg.y, S, ax, ay, az);
 }
 
+// The finish of particle i: its sums, the kick and the drift.
 template <bool F32R, int W, bool KICK>
-__global__ __launch_bounds__(BLOCK) void k_symg_finish(const double* __restrict__ act,
-                                                       const int2* __restrict__ rowseg,
-                                                       const double* __restrict__ react64,
-                                                       const float* __restrict__ react32, int nB, int S, int n,
-                                                       double scale, P4* __restrict__ pm, P4* __restrict__ vel,
-                                                       P4* __restrict__ acc, double kdt, double c1dt, double c2dt) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void symg_finish_one(const double* __restrict__ act, const int2* __restrict__ rowseg,
+                                                const double* __restrict__ react64,
+                                                const float* __restrict__ react32, int nB, int S, int i,
+                                                double scale, P4* __restrict__ pm, P4* __restrict__ vel,
+                                                P4* __restrict__ acc, double kdt, double c1dt, double c2dt) {
     double bx, by, bz, ax, ay, az;
     symg_react<F32R, W>(react64, react32, nB, S, i, bx, by, bz);
     symg_act<W>(act, rowseg, S, i, ax, ay, az);
@@ -563,17 +561,37 @@ __global__ __launch_bounds__(BLOCK) void k_symg_finish(const double* __restrict_
     }
 }
 
+// Thread t finishes particles t and n-1-t: a particle of block X adds
+// X / W + 1 reaction slots and its row's action segments, which grow from the
+// first blocks' ~95 to the last blocks' ~170 slots (N = 262144, mixed), so
+// one particle from each end gives every thread about the same work (every
+// wave of this kernel is resident at once, so it lasts as long as its
+// heaviest wave).  Each particle's sums keep their order.
+template <bool F32R, int W, bool KICK>
+__global__ __launch_bounds__(BLOCK) void k_symg_finish(const double* __restrict__ act,
+                                                       const int2* __restrict__ rowseg,
+                                                       const double* __restrict__ react64,
+                                                       const float* __restrict__ react32, int nB, int S, int n,
+                                                       double scale, P4* __restrict__ pm, P4* __restrict__ vel,
+                                                       P4* __restrict__ acc, double kdt, double c1dt, double c2dt) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= (n + 1) / 2) return;
+    symg_finish_one<F32R, W, KICK>(act, rowseg, react64, react32, nB, S, t, scale, pm, vel, acc, kdt, c1dt, c2dt);
+    if (n - 1 - t != t)
+        symg_finish_one<F32R, W, KICK>(act, rowseg, react64, react32, nB, S, n - 1 - t, scale, pm, vel, acc, kdt,
+                                       c1dt, c2dt);
+}
+
 // Two-set finish of the grouped form: local forces from the actions, the
-// received set's from the reactions, times the particles' masses, added.
+// received set's from the reactions, times the particles' masses, added;
+// thread t takes particles t and n-1-t, as k_symg_finish.
 template <bool F32R, int W>
-__global__ __launch_bounds__(BLOCK) void k_symg_ring_finish(const double* __restrict__ act,
-                                                            const int2* __restrict__ rowseg,
-                                                            const double* __restrict__ react64,
-                                                            const float* __restrict__ react32, int nB, int S, int n,
-                                                            const P4* __restrict__ pl, const P4* __restrict__ pr,
-                                                            P4* __restrict__ fl, P4* __restrict__ fr) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void symg_ring_finish_one(const double* __restrict__ act,
+                                                     const int2* __restrict__ rowseg,
+                                                     const double* __restrict__ react64,
+                                                     const float* __restrict__ react32, int nB, int S, int i,
+                                                     const P4* __restrict__ pl, const P4* __restrict__ pr,
+                                                     P4* __restrict__ fl, P4* __restrict__ fr) {
     double bx, by, bz, ax, ay, az;
     symg_react<F32R, W>(react64, react32, nB, S, i, bx, by, bz);
     symg_act<W>(act, rowseg, S, i, ax, ay, az);
@@ -584,6 +602,19 @@ __global__ __launch_bounds__(BLOCK) void k_symg_ring_finish(const double* __rest
     P4 h = fr[i];
     h.x += mr * bx; h.y += mr * by; h.z += mr * bz;
     fr[i] = h;
+}
+
+template <bool F32R, int W>
+__global__ __launch_bounds__(BLOCK) void k_symg_ring_finish(const double* __restrict__ act,
+                                                            const int2* __restrict__ rowseg,
+                                                            const double* __restrict__ react64,
+                                                            const float* __restrict__ react32, int nB, int S, int n,
+                                                            const P4* __restrict__ pl, const P4* __restrict__ pr,
+                                                            P4* __restrict__ fl, P4* __restrict__ fr) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= (n + 1) / 2) return;
+    symg_ring_finish_one<F32R, W>(act, rowseg, react64, react32, nB, S, t, pl, pr, fl, fr);
+    if (n - 1 - t != t) symg_ring_finish_one<F32R, W>(act, rowseg, react64, react32, nB, S, n - 1 - t, pl, pr, fl, fr);
 }
 
 // a_i for i in block X = sum of the reactions from blocks Y = 0 .. X-1, then
@@ -831,7 +862,7 @@ hipError_t launch_symg_finish(const SymgSums& P, int n, P4* pm, P4* vel, P4* acc
     if (n <= 0) return hipSuccess;
     if (!P.react64 == !P.react32) return hipErrorInvalidValue;
     constexpr int W = kGroupWaves;
-    const dim3 grid((n + BLOCK - 1) / BLOCK);
+    const dim3 grid(((n + 1) / 2 + BLOCK - 1) / BLOCK);   // a thread per pair of particles t, n-1-t
 #define NB_SYMG_FINISH(F32R, KICK)                                                                                 \
     hipLaunchKernelGGL((k_symg_finish<F32R, W, KICK>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,        \
                        P.react32, P.nB, P.S, n, P.scale, pm, vel, acc, kdt, c1dt, c2dt)
@@ -851,7 +882,7 @@ hipError_t launch_symg_ring_finish(const SymgSums& P, int n, const P4* pl, const
     if (n <= 0) return hipSuccess;
     if (!P.react64 == !P.react32) return hipErrorInvalidValue;
     constexpr int W = kGroupWaves;
-    const dim3 grid((n + BLOCK - 1) / BLOCK);
+    const dim3 grid(((n + 1) / 2 + BLOCK - 1) / BLOCK);   // a thread per pair of particles t, n-1-t
     if (P.react32)
         hipLaunchKernelGGL((k_symg_ring_finish<true, W>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,
                            P.react32, P.nB, P.S, n, pl, pr, fl, fr);
