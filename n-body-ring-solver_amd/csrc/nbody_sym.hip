@@ -561,30 +561,32 @@ __device__ __forceinline__ void symg_finish_one(const double* __restrict__ act, 
     }
 }
 
-// Thread t finishes particles t and n-1-t: a particle of block X adds
-// X / W + 1 reaction slots and its row's action segments, which grow from the
-// first blocks' ~95 to the last blocks' ~170 slots (N = 262144, mixed), so
-// one particle from each end gives every thread about the same work (every
-// wave of this kernel is resident at once, so it lasts as long as its
-// heaviest wave).  Each particle's sums keep their order.
+// With `pair`, thread t finishes particles t and n-1-t: a particle of block
+// X adds X / W + 1 reaction slots and its row's action segments, which grow
+// from the first blocks' ~95 to the last blocks' ~170 slots (N = 262144,
+// mixed), so one particle from each end gives every thread about the same
+// work (every wave of this kernel is resident at once, so it lasts as long
+// as its heaviest wave).  Else thread t finishes particle t.  Each
+// particle's sums keep their order.
 template <bool F32R, int W, bool KICK>
 __global__ __launch_bounds__(BLOCK) void k_symg_finish(const double* __restrict__ act,
                                                        const int2* __restrict__ rowseg,
                                                        const double* __restrict__ react64,
                                                        const float* __restrict__ react32, int nB, int S, int n,
-                                                       double scale, P4* __restrict__ pm, P4* __restrict__ vel,
-                                                       P4* __restrict__ acc, double kdt, double c1dt, double c2dt) {
+                                                       int pair, double scale, P4* __restrict__ pm,
+                                                       P4* __restrict__ vel, P4* __restrict__ acc, double kdt,
+                                                       double c1dt, double c2dt) {
     const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= (n + 1) / 2) return;
+    if (t >= (pair ? (n + 1) / 2 : n)) return;
     symg_finish_one<F32R, W, KICK>(act, rowseg, react64, react32, nB, S, t, scale, pm, vel, acc, kdt, c1dt, c2dt);
-    if (n - 1 - t != t)
+    if (pair && n - 1 - t != t)
         symg_finish_one<F32R, W, KICK>(act, rowseg, react64, react32, nB, S, n - 1 - t, scale, pm, vel, acc, kdt,
                                        c1dt, c2dt);
 }
 
 // Two-set finish of the grouped form: local forces from the actions, the
 // received set's from the reactions, times the particles' masses, added;
-// thread t takes particles t and n-1-t, as k_symg_finish.
+// with `pair` thread t takes particles t and n-1-t, as k_symg_finish.
 template <bool F32R, int W>
 __device__ __forceinline__ void symg_ring_finish_one(const double* __restrict__ act,
                                                      const int2* __restrict__ rowseg,
@@ -609,12 +611,14 @@ __global__ __launch_bounds__(BLOCK) void k_symg_ring_finish(const double* __rest
                                                             const int2* __restrict__ rowseg,
                                                             const double* __restrict__ react64,
                                                             const float* __restrict__ react32, int nB, int S, int n,
-                                                            const P4* __restrict__ pl, const P4* __restrict__ pr,
-                                                            P4* __restrict__ fl, P4* __restrict__ fr) {
+                                                            int pair, const P4* __restrict__ pl,
+                                                            const P4* __restrict__ pr, P4* __restrict__ fl,
+                                                            P4* __restrict__ fr) {
     const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= (n + 1) / 2) return;
+    if (t >= (pair ? (n + 1) / 2 : n)) return;
     symg_ring_finish_one<F32R, W>(act, rowseg, react64, react32, nB, S, t, pl, pr, fl, fr);
-    if (n - 1 - t != t) symg_ring_finish_one<F32R, W>(act, rowseg, react64, react32, nB, S, n - 1 - t, pl, pr, fl, fr);
+    if (pair && n - 1 - t != t)
+        symg_ring_finish_one<F32R, W>(act, rowseg, react64, react32, nB, S, n - 1 - t, pl, pr, fl, fr);
 }
 
 // a_i for i in block X = sum of the reactions from blocks Y = 0 .. X-1, then
@@ -857,15 +861,21 @@ hipError_t launch_accel_symg(const SymgLaunch& L, hipStream_t st) {
     }
 }
 
+// Pairing the two ends in one thread halves the finish's waves: it pays from
+// N = 262144 (FP64 172.6 -> 160.2 us, mixed 120.4 -> 115.4 us), where 2048
+// waves remain, and costs at N = 131072 (83.0 -> 86.1 us, 53.8 -> 57.4 us).
+static bool finish_pairs(int n) { return n >= 262144; }
+
 hipError_t launch_symg_finish(const SymgSums& P, int n, P4* pm, P4* vel, P4* acc, bool kick, double kdt, double c1dt,
                               double c2dt, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     if (!P.react64 == !P.react32) return hipErrorInvalidValue;
     constexpr int W = kGroupWaves;
-    const dim3 grid(((n + 1) / 2 + BLOCK - 1) / BLOCK);   // a thread per pair of particles t, n-1-t
+    const bool pair = finish_pairs(n);
+    const dim3 grid(((pair ? (n + 1) / 2 : n) + BLOCK - 1) / BLOCK);
 #define NB_SYMG_FINISH(F32R, KICK)                                                                                 \
     hipLaunchKernelGGL((k_symg_finish<F32R, W, KICK>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,        \
-                       P.react32, P.nB, P.S, n, P.scale, pm, vel, acc, kdt, c1dt, c2dt)
+                       P.react32, P.nB, P.S, n, pair ? 1 : 0, P.scale, pm, vel, acc, kdt, c1dt, c2dt)
     if (P.react32) {
         if (kick) NB_SYMG_FINISH(true, true);
         else NB_SYMG_FINISH(true, false);
@@ -882,13 +892,14 @@ hipError_t launch_symg_ring_finish(const SymgSums& P, int n, const P4* pl, const
     if (n <= 0) return hipSuccess;
     if (!P.react64 == !P.react32) return hipErrorInvalidValue;
     constexpr int W = kGroupWaves;
-    const dim3 grid(((n + 1) / 2 + BLOCK - 1) / BLOCK);   // a thread per pair of particles t, n-1-t
+    const bool pair = finish_pairs(n);
+    const dim3 grid(((pair ? (n + 1) / 2 : n) + BLOCK - 1) / BLOCK);
     if (P.react32)
         hipLaunchKernelGGL((k_symg_ring_finish<true, W>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,
-                           P.react32, P.nB, P.S, n, pl, pr, fl, fr);
+                           P.react32, P.nB, P.S, n, pair ? 1 : 0, pl, pr, fl, fr);
     else
         hipLaunchKernelGGL((k_symg_ring_finish<false, W>), grid, dim3(BLOCK), 0, st, P.act, P.rowseg, P.react64,
-                           P.react32, P.nB, P.S, n, pl, pr, fl, fr);
+                           P.react32, P.nB, P.S, n, pair ? 1 : 0, pl, pr, fl, fr);
     return hipGetLastError();
 }
 
