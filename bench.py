@@ -64,6 +64,8 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_PAIR = 20          # the conventional count for one softened pair interaction
 DE_BOUND = 1e-6
+SIMDS = 1024           # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9       # peak engine clock
 LEARNED = {"period": 4, "threshold": 3e-3, "rc": 0.25, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
 LEARNED_P8 = {"period": 8, "threshold": 4e-3, "rc": 0.35, "rho": 0.1, "kappa": 3e-3, "exact_form": "mixed"}
 HEADLINE_FORMS = ("fp64", "mixed", "learned")   # the ablation and the period-8 variant are reported beside
@@ -455,6 +457,17 @@ def main():
             traffic = profile_traffic("k_accel_symg_plummer_n262144_pmc.md",
                                       f"traffic_bytes_per_launch {k['name']}") if profiled else None
             rooflines[form] = roofline_entry(k, note, r["force_kernel_ms_avg"], n, peak, traffic)
+            # VALU issue utilisation: the committed counter summary's wave64
+            # VALU instructions per launch over this run's launch time, against
+            # one instruction per SIMD every 4 cycles at the 2.4 GHz peak clock
+            insts = profile_traffic("k_accel_symg_plummer_n262144_pmc.md",
+                                    f"valu_insts_per_launch {k['name']}") if profiled else None
+            if insts:
+                issue = SIMDS * CLOCK_HZ / 4.0 * r["force_kernel_ms_avg"] * 1e-3
+                rooflines[form]["valu_issue_frac"] = insts / issue
+                rooflines[form]["valu_issue_note"] = ("SQ_INSTS_VALU per launch (committed PMC summary) / "
+                                                      "(1024 SIMDs x 2.4 GHz / 4 x launch time); above 1 where "
+                                                      "transcendental instructions issue beside the stream")
     dominant = "mixed" if head.startswith("learned") or head == "mixed" else "fp64"
     roofline = rooflines.get(dominant) or rooflines.get("fp64")
 
