@@ -41,6 +41,22 @@ def test_run_records_vs_oracle(solver, orc, order):
         assert rel_err(f4[:, :3].T, f_ref) < 1e-9
 
 
+def test_run_records_unequal_masses_odd_n(solver, orc):
+    # masses apart (the kernels' general path) and an odd N; coeff = mass
+    rng = np.random.default_rng(4)
+    n = 777
+    _, pos, vel, _ = nbody_amd.init_system("plummer", n, 13)
+    m = rng.uniform(0.5, 1.5, n) / n
+    p4, v4, f4 = records(pos), records(vel), np.zeros((n, 4))
+    solver.run_records(m, p4, v4, 6, 1.0 / 128, 2, "fp64", 0.01, coeff=m.copy(), force4=f4)
+    po, vo = orc.leapfrog(m, pos, vel, 6, 1.0 / 128, 2, 0.01)
+    assert np.max(np.abs(p4[:, :3].T - po)) < 1e-12 * np.max(np.abs(po))
+    assert np.max(np.abs(v4[:, :3].T - vo)) < 1e-11 * np.max(np.abs(vo))
+    pp, vp = orc.leapfrog(m, pos, vel, 5, 1.0 / 128, 2, 0.01)
+    f_ref = m * orc.accel(m, pp + 0.5 / 128 * vp, 0.01)
+    assert rel_err(f4[:, :3].T, f_ref) < 1e-9
+
+
 def test_run_records_equals_the_resident_state_run(solver):
     # the same steps through nb_set_state / nb_step / nb_get_state: bitwise
     m, pos, vel, _ = nbody_amd.init_system("sphere", 3000, 2)
