@@ -78,7 +78,7 @@ struct Args {
     double threshold = 3e-3;
     double rc = 0.25;   // near-field radius of the learned form (0: no split)
     size_t train_steps = 200;
-    int train_iters = 4000;
+    int train_iters = 16000;   // then half as many at a lower rate
 };
 
 bool file_exists(const std::string& p) {
@@ -348,7 +348,10 @@ int run_omp(nb_ctx* ctx, Args& a) {
         printf("Training the extrapolator on %zu exact steps ...\n", a.train_steps);
         double loss = 0.0;
         CHECK(nb_learned_collect(ctx, a.train_steps, dt, c, d, lf, NB_FORM_MIXED, a.eps));
-        CHECK(nb_learned_train(ctx, a.train_iters, 4096, 1e-3, a.seed, &loss));
+        // the library's measured schedule: train_iters minibatches at 2e-3, then
+        // train_iters / 2 at 3e-4
+        CHECK(nb_learned_train(ctx, a.train_iters, 4096, 2e-3, a.seed, &loss));
+        if (a.train_iters / 2 > 0) CHECK(nb_learned_train(ctx, a.train_iters / 2, 4096, 3e-4, a.seed + 1, &loss));
         printf("final training loss %.4e\n", loss);
         CHECK(nb_set_state(ctx, N, sys.mass.data(), sys.pos.p, sys.vel.p));   // restart from the initial state
     }
