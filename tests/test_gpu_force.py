@@ -313,6 +313,23 @@ def test_full_size_vs_oracle_sampled(solver, orc):
     assert rel_err(solver.accel("mixed", 0.005)[:, rows], ref) < MIXED_TOL
 
 
+def test_large_n_grouped_kernel_sampled(solver, orc):
+    """N = 2^21 + 37 (a ragged last block): the grouped symmetric kernel with
+    ~26 GB (FP64) / ~17 GB (mixed) of partial sums and slot indices past
+    2^31 elements, FP64 and mixed, against the oracle on 256 sampled rows."""
+    n = (1 << 21) + 37
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 17, [1.0, 20.0])
+    upload(solver, m, pos)
+    rows = np.random.default_rng(6).choice(n, 256, replace=False)
+    rows[0], rows[1] = 0, n - 1   # the first and the last (padded) block
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    ref = orc.accel_block(pj[rows], pj, 0.005).T
+    a64 = solver.accel("fp64", 0.005)
+    assert solver.kernel_info()["kernel"] == "symmetric"
+    assert rel_err(a64[:, rows], ref) < FP64_TOL
+    assert rel_err(solver.accel("mixed", 0.005)[:, rows], ref) < MIXED_TOL
+
+
 def test_ring_share_full_size_sampled(solver, orc):
     """One rank's share of the ring at full size: the first 131072 of a
     Plummer N = 1048576 against all of them in j-blocks of 131072, FP64 and
