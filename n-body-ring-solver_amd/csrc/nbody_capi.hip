@@ -1153,6 +1153,7 @@ nb_status nb_run_records(nb_ctx* c, int form, double eps, double K, const double
             return fail(NB_ERR_ARG, "nb_run_records: K coeff[i] / mass[i] must be the same for every particle "
                                     "(the reference's host passes coeff = mass)");
     }
+    if (!std::isfinite(f)) return fail(NB_ERR_ARG, "nb_run_records: K coeff[i] / mass[i] is not finite");
     if (n == 0 || time_steps == 0) return NB_OK;
     std::vector<double> P(3 * n), V(3 * n);
     for (size_t i = 0; i < n; i++)

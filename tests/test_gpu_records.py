@@ -115,6 +115,8 @@ def test_run_records_errors_leave_the_state(solver):
         solver.run_records(bad, p4, v4, 3, 1e-3, 2, "fp64", 1e-3)
     with pytest.raises(nbody_amd.NBodyError):
         solver.run_records(m, p4, v4, 3, 1e-3, 2, "fp64", -1.0)
+    with pytest.raises(nbody_amd.NBodyError):   # K coeff / mass overflows
+        solver.run_records(m, p4, v4, 3, 1e-3, 2, "fp64", 1e-3, K=1e308, coeff=1e10 * m)
     assert np.all(p4 == records(pos)) and np.all(v4 == records(vel))
     p, v = solver.get_state()
     assert np.array_equal(p, p_before) and np.array_equal(v, v_before)
