@@ -1070,14 +1070,25 @@ nb_status near_accel_host(nb_ctx* c, int form, double rc, double eps, double* co
     Learned* L = get(c);
     if (!L) return nbh::set_error(NB_ERR_ALLOC, std::string(name) + ": out of host memory");
     hipStream_t st = nbh::ctx_stream(c);
-    nbh::DevBuf out;
-    LH(out.ensure(n * sizeof(P4)), "allocate output");
-    LH(L->near.near_accel(nbh::ctx_pm(c), (int)n, rc, eps, (P4*)out.p, st, kf != nbk::FORM_FP64, rebuild),
-       "near-field kernels");
+    nbh::DevBuf out;   // released on every path below
     std::vector<P4> h(n);
-    LH(hipMemcpyAsync(h.data(), out.p, n * sizeof(P4), hipMemcpyDeviceToHost, st), "download");
-    LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+    const char* what = "allocate output";
+    hipError_t e = out.ensure(n * sizeof(P4));
+    if (e == hipSuccess) {
+        what = "near-field kernels";
+        e = L->near.near_accel(nbh::ctx_pm(c), (int)n, rc, eps, (P4*)out.p, st, kf != nbk::FORM_FP64, rebuild);
+    }
+    if (e == hipSuccess) {
+        what = "download";
+        e = hipMemcpyAsync(h.data(), out.p, n * sizeof(P4), hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) {
+        what = "hipStreamSynchronize";
+        e = hipStreamSynchronize(st);
+    }
+    if (e != hipSuccess) (void)hipStreamSynchronize(st);   // nothing queued may still use the buffer
     out.release();
+    if (e != hipSuccess) return hip_err(e, what);
     for (size_t i = 0; i < n; i++) {
         acc[0][i] = h[i].x;
         acc[1][i] = h[i].y;
@@ -1244,17 +1255,31 @@ nb_status nb_learned_train(nb_ctx* c, int iters, int batch, double lr, uint64_t 
     // target scale: RMS of the collected residuals
     {
         const int nbk_ = 256;
-        nbh::DevBuf ss;
-        LH(ss.ensure(nbk_ * sizeof(double)), "allocate");
-        hipLaunchKernelGGL(nbl::k_sumsq, dim3(nbk_), dim3(nbl::BLK), 0, st, (const float*)L.dsT.p, L.ds_rows, (double*)ss.p);
+        nbh::DevBuf ss;   // released on every path below
         std::vector<double> h(nbk_);
-        LH(hipMemcpyAsync(h.data(), ss.p, nbk_ * sizeof(double), hipMemcpyDeviceToHost, st), "download");
-        LH(hipStreamSynchronize(st), "hipStreamSynchronize");
+        const char* what = "allocate";
+        hipError_t e = ss.ensure(nbk_ * sizeof(double));
+        if (e == hipSuccess) {
+            what = "sum-of-squares kernel";
+            hipLaunchKernelGGL(nbl::k_sumsq, dim3(nbk_), dim3(nbl::BLK), 0, st, (const float*)L.dsT.p, L.ds_rows,
+                               (double*)ss.p);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) {
+            what = "download";
+            e = hipMemcpyAsync(h.data(), ss.p, nbk_ * sizeof(double), hipMemcpyDeviceToHost, st);
+        }
+        if (e == hipSuccess) {
+            what = "hipStreamSynchronize";
+            e = hipStreamSynchronize(st);
+        }
+        if (e != hipSuccess) (void)hipStreamSynchronize(st);   // nothing queued may still use the buffer
+        ss.release();
+        if (e != hipSuccess) return hip_err(e, what);
         double s = 0.0;
         for (double v : h) s += v;
         const double rms = std::sqrt(s / (3.0 * (double)L.ds_rows));
         L.tau = (rms > 0 && std::isfinite(rms)) ? (float)rms : 1.0f;
-        ss.release();
     }
     const float clip = 8.0f;
     for (int it = 0; it < iters; it++) {
