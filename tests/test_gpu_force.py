@@ -139,16 +139,27 @@ def test_bitwise_reproducible(solver):
         assert np.array_equal(a1, a2), form
 
 
-def test_unequal_masses_momentum(solver):
+def test_unequal_masses_momentum(solver, orc):
     # full-size property: sum_i m_i a_i = 0 (Newton's third law) at N = 262144
+    # with masses apart (the grouped kernel's general path and the finish
+    # that takes one particle from each end), and sampled rows against the
+    # oracle
     n = 262144
-    m, pos, _, _ = nbody_amd.init_system("plummer", n, 99)
+    _, pos, _, _ = nbody_amd.init_system("plummer", n, 99)
+    m = np.random.default_rng(99).uniform(0.5, 1.5, n) / n
     upload(solver, m, pos)
-    for form, tol in (("fp64", 1e-13), ("mixed", 1e-6)):
+    rows = np.random.default_rng(8).choice(n, 128, replace=False)
+    rows[0], rows[1] = 0, n - 1
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    ref = orc.accel_block(pj[rows], pj, 0.005).T
+    for form, tol, ptol in (("fp64", 1e-13, FP64_TOL), ("mixed", 1e-6, MIXED_TOL)):
         a = solver.accel(form, 0.005)
+        name = solver.kernel_info()["name"]
+        assert name.startswith("k_accel_symg<") and "equal" not in name, name
         tot = np.abs((m * a).sum(axis=1))
         scale = (m * np.abs(a)).sum(axis=1)
         assert np.all(tot < tol * scale), (form, tot / scale)
+        assert rel_err(a[:, rows], ref) < ptol, form
 
 
 def test_full_size_mixed_vs_fp64(solver):
@@ -261,6 +272,35 @@ def test_symmetric_coincident_eps0(solver, orc):
         assert rel_err(solver.accel("fp64", 0.0), orc.accel(m, pos, 0.0)) < FP64_TOL
     finally:
         solver.set_kernel("auto")
+
+
+@pytest.mark.parametrize("equal", [True, False])
+def test_grouped_coincident_eps0(solver, orc, equal):
+    """The grouped kernel (N >= 131072) at eps = 0 with zero-separation pairs:
+    inside one block, across the waves of one group (blocks 1 and 2 for the
+    FP64 form's 512-particle blocks), across groups and in the padded last
+    block; the equal-mass variant and the general path (one mass changed by
+    2^-50).  Such pairs contribute nothing, as in the oracle."""
+    n = 131072 + 131
+    m, pos, _, _ = nbody_amd.init_system("plummer", n, 23)
+    pairs = [(50, 100), (600, 1500), (10, 70000), (n - 2, n - 1), (5, n - 3)]
+    for a, b in pairs:
+        pos[:, b] = pos[:, a]
+    pos = np.ascontiguousarray(pos)
+    if not equal:
+        m = m.copy()
+        m[n // 3] *= 1.0 + 2.0 ** -50
+    solver.set_state(m, pos)
+    rows = np.random.default_rng(3).choice(n, 200, replace=False)
+    rows = np.unique(np.concatenate([rows, np.array(pairs).ravel()]))
+    pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+    ref = orc.accel_block(pj[rows], pj, 0.0).T
+    for form, tol in (("fp64", FP64_TOL), ("mixed", MIXED_TOL)):
+        a = solver.accel(form, 0.0)
+        name = solver.kernel_info()["name"]
+        assert name.startswith("k_accel_symg<") and name.endswith("equal masses>") == equal, name
+        assert np.all(np.isfinite(a)), form
+        assert rel_err(a[:, rows], ref) < tol, form
 
 
 def test_symmetric_beyond_16_gib_of_partials(solver):
