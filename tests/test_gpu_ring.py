@@ -61,13 +61,18 @@ def test_ring_round_vs_oracle(solver, orc, eps, form, tol):
     assert worst < tol
 
 
+@pytest.mark.parametrize("n,equal", [(131072, True), (131072 + 77, False)])
 @pytest.mark.parametrize("form,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
-def test_ring_invariant_full_size(solver, form, tol):
+def test_ring_invariant_full_size(solver, orc, form, tol, n, equal):
     """Two ranks of 131072 (the ring share's block size) over a Plummer
     N = 262144: the local forces of both rounds plus the received forces
-    that come back equal the whole system's forces (nb_compute_forces)."""
-    n, P = 131072, 2
+    that come back equal the whole system's forces (nb_compute_forces).
+    Also two ranks of 131149 (ragged blocks) with masses apart, whose summed
+    forces are checked against the oracle on sampled rows as well."""
+    P = 2
     m, pos, _, _ = nbody_amd.init_system("plummer", P * n, 5, [1.0, 20.0])
+    if not equal:
+        m = np.random.default_rng(n).uniform(0.5, 1.5, P * n) / (P * n)
     blocks = [np.ascontiguousarray(pos[:, r * n:(r + 1) * n]) for r in range(P)]
     F = np.zeros((3, P * n))
     for rank in range(P):
@@ -82,6 +87,13 @@ def test_ring_invariant_full_size(solver, form, tol):
     full = solver.compute_forces("fp64", 0.005)
     err = np.max(np.linalg.norm(F - full, axis=0) / np.linalg.norm(full, axis=0))
     assert err < tol, err
+    if not equal:
+        rows = np.random.default_rng(2).choice(P * n, 128, replace=False)
+        rows[:4] = [0, n - 1, n, P * n - 1]   # the first and the ragged last block of each rank
+        pj = np.stack([pos[0], pos[1], pos[2], m], axis=1)
+        ref = m[rows] * orc.accel_block(pj[rows], pj, 0.005).T
+        err = np.max(np.linalg.norm(F[:, rows] - ref, axis=0) / np.linalg.norm(ref, axis=0))
+        assert err < tol, err
 
 
 def test_ring_round_device_matches_host(solver):
